@@ -1,0 +1,199 @@
+/*
+ * integral_hip.h -- C ABI of the MI355X (gfx950) implementation of the
+ * two-electron hot path of the Rust library `integral` (v0.4.3).
+ *
+ * Plain C types only (pointers, sizes, doubles); no HIP or torch types cross
+ * this boundary except the opaque `void *stream` (a hipStream_t, or NULL for
+ * the default stream).  Every entry point returns an `int` status
+ * (IG_OK == 0) and never unwinds or aborts across the ABI; the message of the
+ * last failure on the calling thread is available from ig_last_error_message().
+ *
+ * Conventions are the reference's (crates/integral/src):
+ *   - Cartesian order lx descending then ly descending (math/am.rs:1-31);
+ *   - primitive normalisation N(alpha; l,0,0), contractions not renormalised,
+ *     zero exponents exempt (shell.rs:34-47, :182-187, math/norm.rs:42-49);
+ *   - spherical components in m order -l..l, p = (x,y,z)
+ *     (math/solid_harmonics.rs:145-172, spherical.rs:183-194);
+ *   - AO offsets by n_func in shell order; atoms = distinct bitwise-equal
+ *     centres in order of first appearance (shell.rs:226-275);
+ *   - dense tensors row-major ((i*n+j)*n+k)*n+l (integrals.rs:641-662);
+ *   - canonical shell pairs (i,j), i>=j, index i(i+1)/2+j (integrals.rs:1252-1260).
+ * J/K/gradient outputs are ACCUMULATED (+=) as in direct.rs:398-402 and
+ * direct_grad.rs:476-484; the caller zeroes them.
+ */
+#ifndef INTEGRAL_HIP_H
+#define INTEGRAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: one per IntegralError variant (lib.rs:83-123) and one per
+ * panic category of the direct contractors (direct.rs:141-159,
+ * direct_grad.rs:337-346, integrals.rs:316-321). */
+enum {
+  IG_OK = 0,
+  IG_ERR_EMPTY_CONTRACTION = 1,        /* IntegralError::EmptyContraction */
+  IG_ERR_MISMATCHED_CONTRACTION = 2,   /* IntegralError::MismatchedContraction */
+  IG_ERR_L_TOO_HIGH = 3,               /* IntegralError::AngularMomentumTooHigh */
+  IG_ERR_L_TOO_HIGH_FOR_GRADIENT = 4,  /* IntegralError::AngularMomentumTooHighForGradient */
+  IG_ERR_GAMMA_LENGTH = 5,             /* IntegralError::GammaLengthMismatch */
+  IG_ERR_NO_DENSITY = 10,              /* "needs at least one density" */
+  IG_ERR_TOO_MANY_DENSITIES = 11,      /* "at most 64 simultaneous densities" */
+  IG_ERR_LENGTH_MISMATCH = 12,         /* assert_eq! on buffer lengths */
+  IG_ERR_NON_CANONICAL_BRA = 13,       /* bra not a pair from bra_pairs() */
+  IG_ERR_FOREIGN_DENSITY = 14,         /* density prepared by another plan */
+  IG_ERR_INVALID_OMEGA = 15,           /* EriKernel::Erf with omega <= 0 / NaN */
+  IG_ERR_INVALID_ARGUMENT = 16,        /* null pointer, bad enum, bad range */
+  IG_ERR_UNSUPPORTED = 17,             /* valid input outside what this build implements */
+  IG_ERR_DEVICE = 20,                  /* HIP runtime failure (allocation, launch) */
+};
+
+/* Engine knob, integrals.rs:330-338.  The GPU evaluates every class with its
+ * own Rys-quadrature kernels; results are engine-transparent to the
+ * reference's cross-engine tolerance (integrals.rs:323-329). */
+enum { IG_ENGINE_AUTO = 0, IG_ENGINE_OS_HGP = 1, IG_ENGINE_RYS = 2 };
+
+/* ShellKind, shell.rs:17-27 */
+enum { IG_CARTESIAN = 0, IG_SPHERICAL = 1 };
+
+/* Returns the ABI version; replaces integral-sys/src/lib.rs:19-23 (which
+ * returns 0).  This build's ABI is version 1. */
+uint32_t integral_abi_version(void);
+
+/* Message for the last non-OK status returned on this thread. */
+const char *ig_last_error_message(void);
+
+/* Number of visible HIP devices (0 when no GPU); never fails. */
+int ig_device_count(void);
+
+/* ---- basis (shell.rs:64-118 Shell::with_kind, :205 Basis::new) ----------- */
+typedef struct ig_shell_desc {
+  int32_t l;             /* angular momentum, 0..=6 (engine/os.rs:34 MAX_L) */
+  int32_t kind;          /* IG_CARTESIAN or IG_SPHERICAL */
+  double center[3];      /* bohr */
+  int32_t n_exponents;   /* number of exponents */
+  int32_t n_coefficients;/* number of raw coefficients (must equal n_exponents) */
+  const double *exponents;
+  const double *coefficients; /* raw, unnormalised contraction coefficients */
+} ig_shell_desc;
+
+typedef struct ig_basis ig_basis;
+
+/* Validates every shell like Shell::with_kind and copies all data. */
+int ig_basis_create(const ig_shell_desc *shells, size_t n_shells, ig_basis **out);
+void ig_basis_destroy(ig_basis *basis);
+size_t ig_basis_nao(const ig_basis *basis);      /* Basis::nao, shell.rs:236-240 */
+size_t ig_basis_nao_cart(const ig_basis *basis); /* Basis::nao_cart, shell.rs:229-233 */
+size_t ig_basis_natom(const ig_basis *basis);    /* Basis::atoms().len(), shell.rs:258-266 */
+size_t ig_basis_nshell(const ig_basis *basis);
+
+/* ---- dense ERI tensor and Schwarz bounds --------------------------------- */
+/* Basis::eri_with, integrals.rs:671-754: out has nao^4 doubles (host memory),
+ * overwritten. */
+int ig_eri_dense(const ig_basis *basis, int engine, double *out, size_t out_len, void *stream);
+/* Basis::eri_screened_with, integrals.rs:874-983: quartets with
+ * Q_ij*Q_kl < tau are left zero; stats[0]=total canonical quartets,
+ * stats[1]=skipped (may be NULL). */
+int ig_eri_screened(const ig_basis *basis, int engine, double tau, double *out, size_t out_len,
+                    int64_t *stats, void *stream);
+/* Basis::schwarz_bounds_with, integrals.rs:842-862: nshell^2 doubles, host. */
+int ig_schwarz_bounds(const ig_basis *basis, int engine, double *out, size_t out_len,
+                      void *stream);
+
+/* ---- integral-direct J/K: DirectContractor (direct.rs:39-218) ------------ */
+typedef struct ig_direct ig_direct;
+
+/* DirectContractor::with_engine (direct.rs:59-87): caches shell data, the
+ * primitive-pair tables (sorted by angular class) and the Schwarz bounds on
+ * the device.  Basis::direct_contractor (direct.rs:331) is engine AUTO. */
+int ig_direct_create(const ig_basis *basis, int engine, ig_direct **out);
+void ig_direct_destroy(ig_direct *direct);
+size_t ig_direct_nao(const ig_direct *direct);
+/* DirectContractor::bra_pairs: writes n_pairs (i,j) pairs as 2*n ints. */
+size_t ig_direct_n_bra_pairs(const ig_direct *direct);
+int ig_direct_bra_pairs(const ig_direct *direct, int32_t *pairs, size_t n_pairs);
+/* DirectContractor::density_shell_bounds (direct.rs:103-125): densities are
+ * ndens host pointers to nao*nao row-major matrices; bounds receives
+ * ndens*nshell*nshell doubles (max |D| per ordered shell block). */
+int ig_direct_density_shell_bounds(const ig_direct *direct, const double *const *densities,
+                                   size_t ndens, double *bounds, size_t bounds_len);
+/* DirectContractor::accumulate_bra_pair (direct.rs:131-218), one canonical
+ * bra pair (i,j) per call; host pointers; J (and K if want_exchange) of each
+ * density accumulated.  `bounds` as produced above (ndens*nshell^2). */
+int ig_direct_accumulate_bra_pair(const ig_direct *direct, int32_t i, int32_t j,
+                                  const double *const *densities, const double *bounds,
+                                  size_t ndens, double threshold, int want_exchange,
+                                  double *const *coulomb, double *const *exchange,
+                                  void *stream);
+/* Bulk entry (the performance path): every canonical bra pair with index in
+ * [bra_begin, bra_end) (bra_end < 0 means all), same result as calling
+ * accumulate_bra_pair on each.  Host pointers; exchange may be NULL when
+ * want_exchange == 0.  Density bounds are computed internally. */
+int ig_direct_jk(const ig_direct *direct, const double *const *densities, size_t ndens,
+                 double threshold, int want_exchange, double *const *coulomb,
+                 double *const *exchange, int64_t bra_begin, int64_t bra_end, void *stream);
+/* Device-resident variant of ig_direct_jk for the full bra range: d_dens,
+ * d_J, d_K are DEVICE pointers to ndens contiguous nao*nao matrices each
+ * (d_K may be NULL when want_exchange == 0).  Asynchronous on `stream`. */
+int ig_direct_jk_device(const ig_direct *direct, const double *d_dens, size_t ndens,
+                        double threshold, int want_exchange, double *d_J, double *d_K,
+                        void *stream);
+/* Number of canonical quartets accepted by the screening of the last call on
+ * this plan ([0]) and primitive quartets evaluated ([1]). */
+int ig_direct_last_stats(const ig_direct *direct, int64_t *stats2);
+
+/* Per-class attribution of the last J/K call (the analogue of the
+ * reference's `eri_profile` classes mode, benches/examples/eri_profile.rs:
+ * 396-531).  With profiling enabled each quartet-class launch is bracketed by
+ * HIP events on the call's stream; ig_direct_profile_read fills up to `cap`
+ * records and sets *n to the number available. */
+typedef struct ig_class_profile {
+  int32_t la, lb, lc, ld;   /* kernel orientation, pair class (la,lb) >= (lc,ld) */
+  int64_t quartets;         /* canonical quartets accepted (density mask != 0) */
+  int64_t prim_quartets;    /* primitive quartets evaluated */
+  double ms;                /* kernel time from HIP events (0 when not profiling) */
+} ig_class_profile;
+int ig_direct_set_profiling(ig_direct *direct, int enable);
+int ig_direct_profile_read(const ig_direct *direct, ig_class_profile *out, size_t cap, size_t *n);
+
+/* ---- ERI-derivative contraction: DirectGradContractor (direct_grad.rs) --- */
+typedef struct ig_grad ig_grad;
+typedef struct ig_grad_density ig_grad_density;
+
+/* DirectGradContractor::with_engine (direct_grad.rs:180-247); fails with
+ * IG_ERR_L_TOO_HIGH_FOR_GRADIENT when a shell has l > 5 (grad.rs:50). */
+int ig_grad_create(const ig_basis *basis, int engine, ig_grad **out);
+void ig_grad_destroy(ig_grad *grad);
+size_t ig_grad_natom(const ig_grad *grad);
+size_t ig_grad_nao(const ig_grad *grad);
+/* DirectGradContractor::density (direct_grad.rs:272-290): Coulomb-type terms
+ * factor*left[mn]*right[ls] and exchange-type factor*left[ml]*right[ns];
+ * left/right are symmetric nao*nao host matrices, copied to the device. */
+int ig_grad_density_create(const ig_grad *grad, size_t n_coulomb, const double *coulomb_factors,
+                           const double *const *coulomb_left, const double *const *coulomb_right,
+                           size_t n_exchange, const double *exchange_factors,
+                           const double *const *exchange_left,
+                           const double *const *exchange_right, ig_grad_density **out);
+void ig_grad_density_destroy(ig_grad_density *density);
+/* EriKernel (integrals.rs:304-312): IG_KERNEL_ERF takes omega, which must be
+ * finite and > 0 (integrals.rs:316-321), else IG_ERR_INVALID_OMEGA. */
+enum { IG_KERNEL_COULOMB = 0, IG_KERNEL_ERF = 1 };
+/* DirectGradContractor::accumulate_bra_pair (direct_grad.rs:328-373):
+ * out is natom*3 doubles (host), accumulated. */
+int ig_grad_accumulate_bra_pair(const ig_grad *grad, const ig_grad_density *density, int32_t i,
+                                int32_t j, int kernel, double omega, double threshold,
+                                double *out, size_t out_len, void *stream);
+/* DirectGradContractor::contract (direct_grad.rs:380-392): all bra pairs,
+ * accumulated into out (natom*3, host). */
+int ig_grad_contract(const ig_grad *grad, const ig_grad_density *density, int kernel,
+                     double omega, double threshold, double *out, size_t out_len, void *stream);
+int ig_grad_last_stats(const ig_grad *grad, int64_t *stats2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

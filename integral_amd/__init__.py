@@ -1,0 +1,549 @@
+"""integral_amd -- MI355X (gfx950) implementation of the two-electron hot path
+of the Rust library `integral` (v0.4.3), behind a C ABI (include/integral_hip.h).
+
+This module mirrors the reference's public API for that path over ctypes:
+
+    Shell / Basis                 crates/integral/src/shell.rs:49-275
+    Basis.eri / eri_screened / schwarz_bounds
+                                  crates/integral/src/integrals.rs:664-983
+    DirectContractor              crates/integral/src/direct.rs:39-331
+    DirectGradContractor          crates/integral/src/direct_grad.rs:144-650
+    Engine, EriKernel             crates/integral/src/integrals.rs:304-338
+    IntegralError                 crates/integral/src/lib.rs:83-123
+
+Every call runs on the GPU through libintegral_hip.so; there is no CPU
+fallback.  Importing works without a GPU (so the CPU test suite can check
+the boundary), but any compute call then raises DeviceError.
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libintegral_hip.so")
+
+__all__ = [
+    "Engine", "ShellKind", "EriKernel", "Shell", "Basis", "DirectContractor", "DirectWorkspace",
+    "DirectBuffers", "DirectGradContractor", "DirectGradWorkspace", "GradDensityTerm",
+    "IntegralError", "EmptyContraction", "MismatchedContraction", "AngularMomentumTooHigh",
+    "AngularMomentumTooHighForGradient", "DeviceError", "UnsupportedError", "MAX_L",
+    "MAX_GRAD_L", "abi_version", "library",
+]
+
+MAX_L = 6
+MAX_GRAD_L = 5
+
+# status codes, include/integral_hip.h
+OK, E_EMPTY, E_MISMATCH, E_LHIGH, E_LGRAD, E_GAMMA = 0, 1, 2, 3, 4, 5
+E_NODENS, E_TOOMANY, E_LEN, E_NONCANON, E_FOREIGN, E_OMEGA, E_ARG, E_UNSUP, E_DEVICE = (
+    10, 11, 12, 13, 14, 15, 16, 17, 20)
+
+
+class Engine(enum.IntEnum):
+    """integrals.rs:330-338."""
+    Auto = 0
+    OsHgp = 1
+    Rys = 2
+
+
+class ShellKind(enum.IntEnum):
+    """shell.rs:17-27."""
+    Cartesian = 0
+    Spherical = 1
+
+
+class EriKernel:
+    """integrals.rs:304-312: Coulomb, or the long-range erf(omega r)/r."""
+
+    def __init__(self, omega=None):
+        self.omega = omega
+
+    @classmethod
+    def coulomb(cls):
+        return cls(None)
+
+    @classmethod
+    def erf(cls, omega):
+        return cls(float(omega))
+
+    def _args(self):
+        """(kernel id, omega) for the C ABI: IG_KERNEL_COULOMB=0, IG_KERNEL_ERF=1."""
+        return (0, 0.0) if self.omega is None else (1, float(self.omega))
+
+
+class IntegralError(Exception):
+    """lib.rs:83-123 and the panics of the direct contractors."""
+    code = None
+
+
+class EmptyContraction(IntegralError):
+    code = E_EMPTY
+
+
+class MismatchedContraction(IntegralError):
+    code = E_MISMATCH
+
+
+class AngularMomentumTooHigh(IntegralError):
+    code = E_LHIGH
+
+
+class AngularMomentumTooHighForGradient(IntegralError):
+    code = E_LGRAD
+
+
+class GammaLengthMismatch(IntegralError):
+    code = E_GAMMA
+
+
+class DeviceError(IntegralError):
+    code = E_DEVICE
+
+
+class UnsupportedError(IntegralError):
+    code = E_UNSUP
+
+
+class UsageError(IntegralError, ValueError):
+    """Misuse the reference rejects with a panic (assert!)."""
+
+
+_ERRORS = {c.code: c for c in (EmptyContraction, MismatchedContraction, AngularMomentumTooHigh,
+                                AngularMomentumTooHighForGradient, GammaLengthMismatch,
+                                DeviceError, UnsupportedError)}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_dpp = ctypes.POINTER(_dp)
+_vp = ctypes.c_void_p
+
+
+class _ShellDesc(ctypes.Structure):
+    _fields_ = [("l", ctypes.c_int32), ("kind", ctypes.c_int32), ("center", ctypes.c_double * 3),
+                ("n_exponents", ctypes.c_int32), ("n_coefficients", ctypes.c_int32),
+                ("exponents", _dp), ("coefficients", _dp)]
+
+
+class _ClassProfile(ctypes.Structure):
+    _fields_ = [("la", ctypes.c_int32), ("lb", ctypes.c_int32), ("lc", ctypes.c_int32),
+                ("ld", ctypes.c_int32), ("quartets", ctypes.c_int64),
+                ("prim_quartets", ctypes.c_int64), ("ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def library():
+    """Load libintegral_hip.so (built by __graft_entry__.build or `make -C integral_amd/csrc`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("integral_amd: %s is not built; run `make -C integral_amd/csrc`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    sz = ctypes.c_size_t
+    i64 = ctypes.c_int64
+    lib.integral_abi_version.restype = ctypes.c_uint32
+    lib.ig_last_error_message.restype = ctypes.c_char_p
+    lib.ig_basis_create.argtypes = [ctypes.POINTER(_ShellDesc), sz, ctypes.POINTER(_vp)]
+    lib.ig_basis_destroy.argtypes = [_vp]
+    for f in ("ig_basis_nao", "ig_basis_nao_cart", "ig_basis_natom", "ig_basis_nshell"):
+        getattr(lib, f).argtypes = [_vp]
+        getattr(lib, f).restype = sz
+    lib.ig_eri_dense.argtypes = [_vp, ctypes.c_int, _dp, sz, _vp]
+    lib.ig_eri_screened.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, sz,
+                                    ctypes.POINTER(i64), _vp]
+    lib.ig_schwarz_bounds.argtypes = [_vp, ctypes.c_int, _dp, sz, _vp]
+    lib.ig_direct_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+    lib.ig_direct_destroy.argtypes = [_vp]
+    lib.ig_direct_nao.argtypes = [_vp]
+    lib.ig_direct_nao.restype = sz
+    lib.ig_direct_n_bra_pairs.argtypes = [_vp]
+    lib.ig_direct_n_bra_pairs.restype = sz
+    lib.ig_direct_bra_pairs.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), sz]
+    lib.ig_direct_density_shell_bounds.argtypes = [_vp, _dpp, sz, _dp, sz]
+    lib.ig_direct_accumulate_bra_pair.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dpp, _dp,
+                                                  sz, ctypes.c_double, ctypes.c_int, _dpp, _dpp,
+                                                  _vp]
+    lib.ig_direct_jk.argtypes = [_vp, _dpp, sz, ctypes.c_double, ctypes.c_int, _dpp, _dpp, i64,
+                                 i64, _vp]
+    lib.ig_direct_jk_device.argtypes = [_vp, _vp, sz, ctypes.c_double, ctypes.c_int, _vp, _vp,
+                                        _vp]
+    lib.ig_direct_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
+    lib.ig_direct_set_profiling.argtypes = [_vp, ctypes.c_int]
+    lib.ig_direct_profile_read.argtypes = [_vp, ctypes.POINTER(_ClassProfile), sz,
+                                           ctypes.POINTER(sz)]
+    lib.ig_grad_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
+    lib.ig_grad_destroy.argtypes = [_vp]
+    lib.ig_grad_natom.argtypes = [_vp]
+    lib.ig_grad_natom.restype = sz
+    lib.ig_grad_nao.argtypes = [_vp]
+    lib.ig_grad_nao.restype = sz
+    lib.ig_grad_density_create.argtypes = [_vp, sz, _dp, _dpp, _dpp, sz, _dp, _dpp, _dpp,
+                                           ctypes.POINTER(_vp)]
+    lib.ig_grad_density_destroy.argtypes = [_vp]
+    lib.ig_grad_accumulate_bra_pair.argtypes = [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                _dp, sz, _vp]
+    lib.ig_grad_contract.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     _dp, sz, _vp]
+    lib.ig_grad_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
+    _lib = lib
+    return lib
+
+
+def abi_version():
+    return int(library().integral_abi_version())
+
+
+def _raise(rc):
+    if rc == OK:
+        return
+    msg = library().ig_last_error_message().decode()
+    cls = _ERRORS.get(rc)
+    if cls is None:
+        raise UsageError("%s (status %d)" % (msg, rc))
+    raise cls(msg)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _ptrs(arrs):
+    return (_dp * len(arrs))(*[_ptr(a) for a in arrs]) if arrs else None
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
+
+
+class Shell:
+    """A contracted Gaussian shell (shell.rs:49-118).  Validation happens in
+    the library when the shell joins a Basis; the same checks run here first
+    so construction fails like Shell::with_kind."""
+
+    def __init__(self, l, center, exponents, coefficients, kind=ShellKind.Cartesian):
+        exponents = [float(x) for x in exponents]
+        coefficients = [float(x) for x in coefficients]
+        if not exponents:
+            raise EmptyContraction("empty contraction")
+        if len(exponents) != len(coefficients):
+            raise MismatchedContraction("%d exponents vs %d coefficients"
+                                        % (len(exponents), len(coefficients)))
+        if l > MAX_L:
+            raise AngularMomentumTooHigh("l=%d exceeds max %d" % (l, MAX_L))
+        self.l = int(l)
+        self.center = tuple(float(c) for c in center)
+        self.exponents = exponents
+        self.coefficients = coefficients
+        self.kind = ShellKind(kind)
+
+    @classmethod
+    def new(cls, l, center, exponents, coefficients):
+        return cls(l, center, exponents, coefficients, ShellKind.Cartesian)
+
+    @classmethod
+    def new_spherical(cls, l, center, exponents, coefficients):
+        return cls(l, center, exponents, coefficients, ShellKind.Spherical)
+
+    def n_cart(self):
+        return (self.l + 1) * (self.l + 2) // 2
+
+    def n_func(self):
+        return 2 * self.l + 1 if self.kind == ShellKind.Spherical else self.n_cart()
+
+    def n_prim(self):
+        return len(self.exponents)
+
+
+class Basis:
+    """Basis::new (shell.rs:198-275) backed by an ig_basis handle."""
+
+    def __init__(self, shells):
+        lib = library()
+        self._shells = list(shells)
+        descs = (_ShellDesc * max(len(self._shells), 1))()
+        self._keep = []
+        for d, s in zip(descs, self._shells):
+            e, c = _f64(s.exponents), _f64(s.coefficients)
+            self._keep += [e, c]
+            d.l, d.kind = s.l, int(s.kind)
+            d.center[:] = list(s.center)
+            d.n_exponents, d.n_coefficients = len(s.exponents), len(s.coefficients)
+            d.exponents, d.coefficients = _ptr(e), _ptr(c)
+        h = _vp()
+        _raise(lib.ig_basis_create(descs, len(self._shells), ctypes.byref(h)))
+        self._h = h
+        self._keep = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ig_basis_destroy(self._h)
+            self._h = None
+
+    def shells(self):
+        return list(self._shells)
+
+    def nao(self):
+        return int(library().ig_basis_nao(self._h))
+
+    def nao_cart(self):
+        return int(library().ig_basis_nao_cart(self._h))
+
+    def atoms(self):
+        seen = []
+        for s in self._shells:
+            if s.center not in seen:
+                seen.append(s.center)
+        return seen
+
+    def eri_with(self, engine):
+        n = self.nao()
+        out = np.zeros(n ** 4)
+        _raise(library().ig_eri_dense(self._h, int(engine), _ptr(out), out.size, None))
+        return out
+
+    def eri(self):
+        return self.eri_with(Engine.Auto)
+
+    def eri_screened_with(self, engine, tau):
+        n = self.nao()
+        out = np.zeros(n ** 4)
+        st = (ctypes.c_int64 * 2)()
+        _raise(library().ig_eri_screened(self._h, int(engine), float(tau), _ptr(out), out.size,
+                                         st, None))
+        return out, {"shell_quartets_total": st[0], "shell_quartets_skipped": st[1],
+                     "tau": float(tau)}
+
+    def eri_screened(self, tau):
+        return self.eri_screened_with(Engine.Auto, tau)
+
+    def schwarz_bounds_with(self, engine):
+        nsh = len(self._shells)
+        out = np.zeros(nsh * nsh)
+        _raise(library().ig_schwarz_bounds(self._h, int(engine), _ptr(out), out.size, None))
+        return out
+
+    def schwarz_bounds(self):
+        return self.schwarz_bounds_with(Engine.Auto)
+
+    def direct_contractor(self):
+        return DirectContractor(self)
+
+    def direct_grad_contractor(self):
+        return DirectGradContractor(self)
+
+
+class DirectWorkspace:
+    """direct.rs:22-33.  The device plan owns all scratch; kept for API parity."""
+
+
+class DirectBuffers:
+    """direct.rs:14-17: caller-owned J (and optionally K), accumulated."""
+
+    def __init__(self, coulomb, exchange=None):
+        self.coulomb = coulomb
+        self.exchange = exchange
+
+
+class DirectContractor:
+    """DirectContractor (direct.rs:39-218) over an ig_direct plan."""
+
+    def __init__(self, basis, engine=Engine.Auto):
+        h = _vp()
+        _raise(library().ig_direct_create(basis._h, int(engine), ctypes.byref(h)))
+        self._h = h
+        self._nsh = len(basis._shells)
+
+    @classmethod
+    def with_engine(cls, basis, engine):
+        return cls(basis, engine)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ig_direct_destroy(self._h)
+            self._h = None
+
+    def nao(self):
+        return int(library().ig_direct_nao(self._h))
+
+    def bra_pairs(self):
+        n = int(library().ig_direct_n_bra_pairs(self._h))
+        buf = (ctypes.c_int32 * (2 * n))()
+        _raise(library().ig_direct_bra_pairs(self._h, buf, n))
+        return [(buf[2 * k], buf[2 * k + 1]) for k in range(n)]
+
+    def density_shell_bounds(self, densities):
+        ds = [_f64(d).ravel() for d in densities]
+        out = np.zeros(len(ds) * self._nsh * self._nsh)
+        _raise(library().ig_direct_density_shell_bounds(self._h, _ptrs(ds), len(ds), _ptr(out),
+                                                        out.size))
+        return [out[k * self._nsh ** 2:(k + 1) * self._nsh ** 2] for k in range(len(ds))]
+
+    @staticmethod
+    def _check_out(arrs, n2):
+        for a in arrs:
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+                    and a.size == n2):
+                raise UsageError("J/K buffers must be C-contiguous float64 arrays of nao*nao")
+
+    def accumulate_bra_pair(self, workspace, bra, densities, density_bounds, threshold,
+                            want_exchange, outputs, stream=None):
+        ds = [_f64(d).ravel() for d in densities]
+        if len(outputs) != len(ds) or len(density_bounds) != len(ds):
+            raise UsageError("densities, bounds and outputs differ in length")
+        n2 = self.nao() ** 2
+        js = [o.coulomb for o in outputs]
+        ks = [o.exchange for o in outputs] if want_exchange else []
+        self._check_out(js + ks, n2)
+        bounds = _f64(np.concatenate([np.ravel(b) for b in density_bounds])) if ds else _f64([0.0])
+        _raise(library().ig_direct_accumulate_bra_pair(
+            self._h, int(bra[0]), int(bra[1]), _ptrs(ds), _ptr(bounds), len(ds), float(threshold),
+            1 if want_exchange else 0, _ptrs(js), _ptrs(ks) if want_exchange else None,
+            _stream_ptr(stream)))
+
+    def jk(self, densities, threshold=0.0, want_exchange=True, bra_range=(0, -1), J=None, K=None,
+           stream=None):
+        """Bulk J/K over canonical bra pairs [bra_range) (all by default);
+        accumulates into J/K when given, else returns fresh matrices."""
+        ds = [_f64(d).ravel() for d in densities]
+        n = self.nao()
+        J = [np.zeros(n * n) for _ in ds] if J is None else J
+        K = ([np.zeros(n * n) for _ in ds] if K is None else K) if want_exchange else None
+        self._check_out(J + (K or []), n * n)
+        _raise(library().ig_direct_jk(self._h, _ptrs(ds), len(ds), float(threshold),
+                                      1 if want_exchange else 0, _ptrs(J),
+                                      _ptrs(K) if want_exchange else None, int(bra_range[0]),
+                                      int(bra_range[1]), _stream_ptr(stream)))
+        return J, K
+
+    def jk_device(self, dens, J, K=None, threshold=0.0, stream=None):
+        """Device-resident J/K: dens, J, K are torch.float64 CUDA tensors of
+        shape (ndens, nao, nao); J/K are accumulated.  Asynchronous."""
+        for t in [dens, J] + ([K] if K is not None else []):
+            if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8
+                    and t.is_contiguous()):
+                raise UsageError("device buffers must be contiguous float64 CUDA tensors")
+        _raise(library().ig_direct_jk_device(
+            self._h, _vp(dens.data_ptr()), dens.shape[0], float(threshold),
+            1 if K is not None else 0, _vp(J.data_ptr()),
+            _vp(K.data_ptr()) if K is not None else None, _stream_ptr(stream)))
+
+    def last_stats(self):
+        st = (ctypes.c_int64 * 2)()
+        _raise(library().ig_direct_last_stats(self._h, st))
+        return int(st[0]), int(st[1])
+
+    def set_profiling(self, enable):
+        _raise(library().ig_direct_set_profiling(self._h, 1 if enable else 0))
+
+    def profile(self):
+        """Per-class records of the last call: dicts with la..ld, quartets,
+        prim_quartets and ms (event time when profiling is on)."""
+        cap = 512
+        buf = (_ClassProfile * cap)()
+        n = ctypes.c_size_t()
+        _raise(library().ig_direct_profile_read(self._h, buf, cap, ctypes.byref(n)))
+        return [{"cls": (r.la, r.lb, r.lc, r.ld), "quartets": r.quartets,
+                 "prim_quartets": r.prim_quartets, "ms": r.ms} for r in buf[:min(n.value, cap)]]
+
+
+class GradDensityTerm:
+    """direct_grad.rs:22-26: factor * left (x) right."""
+
+    def __init__(self, factor, left, right):
+        self.factor = float(factor)
+        self.left = left
+        self.right = right
+
+
+class DirectGradWorkspace:
+    """direct_grad.rs:51-67.  The device plan owns all scratch."""
+
+
+class GradDensity:
+    def __init__(self, owner, handle, keep):
+        self._owner = owner
+        self._h = handle
+        self._keep = keep
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ig_grad_density_destroy(self._h)
+            self._h = None
+
+
+class DirectGradContractor:
+    """DirectGradContractor (direct_grad.rs:144-392) over an ig_grad plan."""
+
+    def __init__(self, basis, engine=Engine.Auto):
+        h = _vp()
+        _raise(library().ig_grad_create(basis._h, int(engine), ctypes.byref(h)))
+        self._h = h
+        self._nsh = len(basis._shells)
+
+    @classmethod
+    def with_engine(cls, basis, engine):
+        return cls(basis, engine)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ig_grad_destroy(self._h)
+            self._h = None
+
+    def natom(self):
+        return int(library().ig_grad_natom(self._h))
+
+    def nao(self):
+        return int(library().ig_grad_nao(self._h))
+
+    def bra_pairs(self):
+        return [(i, j) for i in range(self._nsh) for j in range(i + 1)]
+
+    def density(self, coulomb, exchange):
+        def unpack(terms):
+            f = _f64([t.factor for t in terms] or [0.0])
+            left = [_f64(t.left).ravel() for t in terms]
+            right = [_f64(t.right).ravel() for t in terms]
+            return f, left, right
+        cf, cl, cr = unpack(coulomb)
+        xf, xl, xr = unpack(exchange)
+        n2 = self.nao() ** 2
+        for m in cl + cr + xl + xr:
+            if m.size != n2:
+                raise UsageError("density matrices must be nao x nao")
+        h = _vp()
+        _raise(library().ig_grad_density_create(self._h, len(coulomb), _ptr(cf), _ptrs(cl),
+                                                _ptrs(cr), len(exchange), _ptr(xf), _ptrs(xl),
+                                                _ptrs(xr), ctypes.byref(h)))
+        return GradDensity(self, h, None)
+
+    def accumulate_bra_pair(self, workspace, bra, density, kernel, threshold, out, stream=None):
+        out = np.asarray(out)
+        if out.dtype != np.float64 or not out.flags.c_contiguous:
+            raise UsageError("out must be a C-contiguous float64 array of natom x 3")
+        kernel = kernel or EriKernel.coulomb()
+        _raise(library().ig_grad_accumulate_bra_pair(self._h, density._h, int(bra[0]),
+                                                     int(bra[1]), *kernel._args(),
+                                                     float(threshold), _ptr(out), out.size,
+                                                     _stream_ptr(stream)))
+
+    def contract(self, density, kernel=None, threshold=0.0, stream=None):
+        kernel = kernel or EriKernel.coulomb()
+        out = np.zeros((self.natom(), 3))
+        _raise(library().ig_grad_contract(self._h, density._h, *kernel._args(), float(threshold),
+                                          _ptr(out), out.size, _stream_ptr(stream)))
+        return out
+
+    def last_stats(self):
+        st = (ctypes.c_int64 * 2)()
+        _raise(library().ig_grad_last_stats(self._h, st))
+        return int(st[0]), int(st[1])

@@ -1,0 +1,127 @@
+// ig_device.h -- device building blocks shared by the quartet kernels:
+// Rys roots/weights, per-axis Rys 2D integral tables, wave-local team
+// synchronisation and screened-quartet decoding.
+#pragma once
+
+#include "ig_internal.h"
+
+namespace ig {
+
+constexpr double kTwoPi52 = 34.98683665524972;  // 2 pi^{5/2}
+
+// Lanes of one team always sit in one wavefront, so ordering LDS traffic
+// between phases only needs a wavefront-scope fence plus a scheduling barrier.
+__device__ __forceinline__ void team_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Half-c0 Clenshaw sum of a 17-term Chebyshev series (math/rys.rs:259-269).
+__device__ __forceinline__ double cheb17(const double *__restrict__ c, double xi) {
+  double b1 = 0.0, b2 = 0.0, two = 2.0 * xi;
+#pragma unroll
+  for (int k = kChebTerms - 1; k >= 1; --k) {
+    double t = two * b1 - b2 + c[k];
+    b2 = b1;
+    b1 = t;
+  }
+  return xi * b1 - b2 + 0.5 * c[0];
+}
+
+// Series `idx` of the n-point Rys rule at T: idx < n is squared root idx,
+// otherwise weight idx - n.  Finite branch: Chebyshev cell lookup; T at or
+// beyond 33 + 2.5(2n-1): scaled Gauss-Laguerre(-1/2) (math/rys.rs:174-185).
+__device__ __forceinline__ double rys_series(const RysDev &R, int n, double T, int idx) {
+  const double hi = 33.0 + 2.5 * double(2 * n - 1);
+  const bool is_w = idx >= n;
+  const int i = is_w ? idx - n : idx;
+  if (T >= hi) {
+    const double *L = R.buf + R.lag_off[n];
+    return is_w ? L[n + i] * (0.5 / sqrt(T)) : L[i] * (1.0 / T);
+  }
+  const int nsub = R.nsub[n];
+  const double width = hi / double(nsub);
+  int s = int(T / width);
+  s = s < nsub ? s : nsub - 1;
+  const double a = double(s) * width, b = a + width;
+  const double xi = (2.0 * T - (a + b)) / (b - a);
+  const double *c = R.buf + (is_w ? R.wt_off[n] : R.root_off[n]) + (s * n + i) * kChebTerms;
+  return cheb17(c, xi);
+}
+
+// Rys 2D integrals of one axis for one root: I[i][j][k][l], i<=LA, j<=LB,
+// k<=LC, l<=LD, from the vertical recurrence on (bra, ket) powers and the two
+// horizontal transfers (engine/rys.rs:234-308 recurrences), written to `out`
+// as ((i*(LB+1)+j)*(LC+1)+k)*(LD+1)+l.  Rolling levels keep the register
+// footprint at two (NB+1)x(NK+1) slabs.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void rys_axis_table(double c00, double cp0, double b10, double b01,
+                                               double b00, double ab, double cd,
+                                               double *__restrict__ out) {
+  constexpr int NB = LA + LB, NK = LC + LD;
+  double g[NB + 1][NK + 1];
+  g[0][0] = 1.0;
+  if constexpr (NB >= 1) g[1][0] = c00;
+#pragma unroll
+  for (int n = 1; n < NB; ++n) g[n + 1][0] = c00 * g[n][0] + double(n) * b10 * g[n - 1][0];
+#pragma unroll
+  for (int m = 0; m < NK; ++m) {
+#pragma unroll
+    for (int n = 0; n <= NB; ++n) {
+      double t = cp0 * g[n][m];
+      if (m >= 1) t += double(m) * b01 * g[n][m - 1];
+      if (n >= 1) t += double(n) * b00 * g[n - 1][m];
+      g[n][m + 1] = t;
+    }
+  }
+  // bra transfer level by level (j = 0..LB); each finished (i, j) row feeds
+  // the ket transfer immediately.
+#pragma unroll
+  for (int j = 0; j <= LB; ++j) {
+    if (j > 0) {
+#pragma unroll
+      for (int i = 0; i <= NB - j; ++i)
+#pragma unroll
+        for (int m = 0; m <= NK; ++m) g[i][m] = g[i + 1][m] + ab * g[i][m];
+    }
+#pragma unroll
+    for (int i = 0; i <= LA; ++i) {
+      double t[NK + 1];
+#pragma unroll
+      for (int k = 0; k <= NK; ++k) t[k] = g[i][k];
+#pragma unroll
+      for (int l = 0; l <= LD; ++l) {
+        if (l > 0) {
+#pragma unroll
+          for (int k = 0; k <= NK - l; ++k) t[k] = t[k + 1] + cd * t[k];
+        }
+#pragma unroll
+        for (int k = 0; k <= LC; ++k)
+          out[((i * (LB + 1) + j) * (LC + 1) + k) * (LD + 1) + l] = t[k];
+      }
+    }
+  }
+}
+
+// Decode flat quartet index t of a QuartetSet into (P, Q) (relative indices).
+__device__ __forceinline__ void decode_quartet(const QuartetSet &qs, long long t, int &P, int &Q) {
+  if (qs.list) {
+    int2 v = qs.list[t];
+    P = v.x;
+    Q = v.y;
+    return;
+  }
+  int lo = 0, hi = qs.x_n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (qs.rowptr[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  P = lo;
+  Q = int(t - qs.rowptr[lo]);
+}
+
+}  // namespace ig

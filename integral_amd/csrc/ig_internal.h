@@ -1,0 +1,142 @@
+// ig_internal.h -- shared declarations of the HIP implementation (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ig {
+
+constexpr int kMaxL = 6;        // engine/os.rs:34
+constexpr int kMaxGradL = 5;    // grad.rs:50
+constexpr int kMaxRys = 13;     // math/rys.rs:104
+constexpr int kChebTerms = 17;  // degree-16 Chebyshev series
+constexpr int kMaxDens = 64;    // direct.rs:145-148
+constexpr int kMaxKernelL = 2;  // shells up to d have compiled quartet kernels in this build
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int pair_class(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }
+constexpr int kNumPairClass = pair_class(kMaxL, kMaxL) + 1;  // 28
+
+// Cartesian component i of shell l: lx descending, then ly descending (am.rs:23-31).
+__host__ __device__ inline void cart_powers(int l, int i, int &lx, int &ly, int &lz) {
+  int d = 0;
+  while ((d + 1) * (d + 2) / 2 <= i) ++d;  // d = l - lx
+  lx = l - d;
+  lz = i - d * (d + 1) / 2;
+  ly = d - lz;
+}
+
+// ---------------------------------------------------------------------------
+// host tables
+struct RysHostTables {
+  std::vector<double> data;  // all coefficients + Laguerre nodes/weights
+  int nsub[kMaxRys + 1] = {};
+  int root_off[kMaxRys + 1] = {}, wt_off[kMaxRys + 1] = {}, lag_off[kMaxRys + 1] = {};
+  double worst[kMaxRys + 1] = {};
+};
+const RysHostTables &rys_host_tables();
+void rys_host_eval(int n, double T, double *x, double *w);
+double shell_norm(double alpha, int l);
+std::vector<double> c2s_scaled(int l);  // (2l+1) x ncart(l), includes sqrt(4pi/(2l+1))
+
+// ---------------------------------------------------------------------------
+// device-visible views (passed by value as kernel arguments)
+struct RysDev {
+  const double *buf;
+  int nsub[kMaxRys + 1];
+  int root_off[kMaxRys + 1], wt_off[kMaxRys + 1], lag_off[kMaxRys + 1];
+};
+
+struct BasisDev {
+  int nsh, nao, nao_cart;
+  const double *xyz;     // nsh x 3
+  const int *l;          // nsh
+  const int *kind;       // nsh (0 Cartesian, 1 spherical)
+  const int *off_func;   // nsh
+  const int *off_cart;   // nsh
+  const int *atom;       // nsh
+  const double *c2s;     // packed scaled c2s matrices for l=0..6
+  int c2s_off[kMaxL + 1];
+};
+
+// Shell pairs in kernel orientation (first shell has the larger l), grouped
+// by pair class and, inside a class, sorted by Schwarz bound (descending).
+struct PairsDev {
+  int n;
+  const int *sh1, *sh2;  // shells, kernel orientation
+  const int *canon;      // canonical pair index i(i+1)/2+j (i >= j)
+  const double *q;       // Schwarz bound Q[i,j]
+  const int *pp_off, *pp_n;  // primitive-pair range
+  // primitive pairs, structure of arrays
+  const double *zeta, *px, *py, *pz, *kc, *alpha, *beta;
+};
+
+// One launch covers the quartets {P, Q} with P in bra class X and Q in ket
+// class Y (class(X) >= class(Y)); either implicit (per-row prefix counts,
+// Q a prefix of Y's Schwarz-sorted list) or an explicit (P, Q) list.
+struct QuartetSet {
+  int x_off, x_n;  // bra pairs [x_off, x_off + x_n) of the sorted pair list
+  int y_off, y_n;
+  int same;        // X == Y
+  const long long *rowptr;  // x_n + 1 prefix sums (implicit mode) or null
+  const int2 *list;         // explicit (P, Q) pairs relative to x_off / y_off, or null
+  long long total;
+};
+
+struct DensDev {
+  int ndens;            // original densities
+  int neff;             // effective densities (symmetric + antisymmetric parts)
+  const double *dc;     // neff x nao_cart^2 Cartesian densities
+  double *jc, *kc;      // neff x nao_cart^2 accumulators (kc may be null)
+  const int *eff_src;   // neff: original density index
+  const int *eff_anti;  // neff: 1 if antisymmetric part (K only)
+  const double *bound;  // ndens x nsh^2, symmetrised max |D| per shell block
+  double tau;
+  int want_k;
+};
+
+struct GradDev {
+  int ncoul, nexch;
+  const double *fac;     // ncoul + nexch
+  const double *mats;    // (ncoul+nexch) x 2 x nao_cart^2 Cartesian left/right
+  const double *bmax;    // (ncoul+nexch) x 2 x nsh^2 symmetrised block maxima (function space)
+  double *out;           // natom x 3 accumulator
+  double tau;
+  double omega;          // <= 0: Coulomb
+};
+
+// ---------------------------------------------------------------------------
+// kernel launchers (ig_kernels.hip)
+bool class_supported(int la, int lb, int lc, int ld);
+hipError_t launch_schwarz(int lcls1, int lcls2, const BasisDev &b, const PairsDev &p,
+                          const RysDev &r, int first, int count, double *qout, hipStream_t s);
+hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                     const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
+                     unsigned long long *counters, hipStream_t s);
+hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                      const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s);
+hipError_t launch_grad(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                       const RysDev &r, const QuartetSet &qs, const GradDev &g,
+                       unsigned long long *counters, hipStream_t s);
+
+// small dense helpers
+// ds/da: ndens x nao^2 symmetric/antisymmetric parts (da used only when
+// has_anti); dc: (1 + has_anti) * ndens Cartesian matrices (symmetric parts
+// first); bound: ndens x nsh^2 symmetrised block maxima of d_func.
+hipError_t launch_density_prep(const BasisDev &b, const double *d_func, int ndens, int has_anti,
+                               double *ds, double *da, double *dc, double *bound,
+                               hipStream_t s);
+hipError_t launch_fill(double *x, size_t n, double v, hipStream_t s);
+// *flag = 1 if any of the ndens n x n matrices differs from its transpose
+hipError_t launch_asym_flag(const double *d, int n, int ndens, int *flag, hipStream_t s);
+hipError_t launch_block_max(const BasisDev &b, const double *m, int nmats, double *bmax,
+                            hipStream_t s);
+hipError_t launch_to_cart(const BasisDev &b, const double *m_func, int nmats, double *m_cart,
+                          hipStream_t s);
+hipError_t launch_jk_finish(const BasisDev &b, const DensDev &d, double *d_J, double *d_K,
+                            hipStream_t s);
+
+}  // namespace ig

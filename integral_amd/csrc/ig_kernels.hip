@@ -1,0 +1,1012 @@
+// ig_kernels.hip -- gfx950 shell-quartet kernels (Rys quadrature) with fused
+// epilogues: Schwarz diagonal, integral-direct J/K digestion, dense tensor
+// scatter.  One workgroup is one wavefront; a wavefront holds 64/G teams and
+// a team of G lanes owns one shell quartet at a time:
+//   phase 1  the team's lanes evaluate the 2n Rys root/weight series   -> LDS
+//   phase 2  the lanes build the 3n per-axis 2D integral tables         -> LDS
+//   phase 3  every lane assembles its slice of the Cartesian block
+//            (sum over roots of w Ix Iy Iz) into registers
+// repeated over the primitive quartets (early contraction), then the block
+// goes to LDS for the epilogue.  G is the smallest power of two that keeps
+// the block at <= 32 doubles per lane.
+#include "ig_device.h"
+
+namespace ig {
+
+__host__ __device__ constexpr int pick_team(int n4) {
+  return n4 <= 32 ? 1 : n4 <= 64 ? 2 : n4 <= 128 ? 4 : n4 <= 256 ? 8 : n4 <= 512 ? 16
+                                                                   : n4 <= 1024 ? 32 : 64;
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <int LA_, int LB_, int LC_, int LD_>
+struct Cls {
+  static constexpr int LA = LA_, LB = LB_, LC = LC_, LD = LD_;
+  static constexpr int na = ncart(LA), nb = ncart(LB), nc = ncart(LC), nd = ncart(LD);
+  static constexpr int N4 = na * nb * nc * nd;
+  static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+  static constexpr int T = (LA + 1) * (LB + 1) * (LC + 1) * (LD + 1);
+  static constexpr int G = pick_team(N4);
+  static constexpr int E = (N4 + G - 1) / G;
+  static constexpr int TEAMS = 64 / G;
+  static constexpr int WORK = 2 * NR + 3 * NR * T;  // roots/weights + tables
+};
+
+// Packed per-element table offsets (x | y<<10 | z<<20) of element e.
+template <class C>
+__device__ __forceinline__ int elem_offsets(int e) {
+  int id = e % C::nd;
+  int r = e / C::nd;
+  int ic = r % C::nc;
+  r /= C::nc;
+  int ib = r % C::nb;
+  int ia = r / C::nb;
+  int p[4][3];
+  cart_powers(C::LA, ia, p[0][0], p[0][1], p[0][2]);
+  cart_powers(C::LB, ib, p[1][0], p[1][1], p[1][2]);
+  cart_powers(C::LC, ic, p[2][0], p[2][1], p[2][2]);
+  cart_powers(C::LD, id, p[3][0], p[3][1], p[3][2]);
+  int packed = 0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    int o = ((p[0][ax] * (C::LB + 1) + p[1][ax]) * (C::LC + 1) + p[2][ax]) * (C::LD + 1) + p[3][ax];
+    packed |= o << (10 * ax);
+  }
+  return packed;
+}
+
+struct QuartetGeom {
+  int sA, sB, sC, sD;
+  double A[3], B[3], C[3], D[3];
+};
+
+__device__ __forceinline__ void load_geom(const BasisDev &b, int sA, int sB, int sC, int sD,
+                                          QuartetGeom &g) {
+  g.sA = sA;
+  g.sB = sB;
+  g.sC = sC;
+  g.sD = sD;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    g.A[x] = b.xyz[3 * sA + x];
+    g.B[x] = b.xyz[3 * sB + x];
+    g.C[x] = b.xyz[3 * sC + x];
+    g.D[x] = b.xyz[3 * sD + x];
+  }
+}
+
+// Contracted Cartesian block of quartet (pb | pk) accumulated into acc[]
+// (element tl + G*k).  `work` is this team's LDS area (C::WORK doubles).
+template <class C>
+__device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R, int pb, int pk,
+                                              const QuartetGeom &g, double omega, int tl,
+                                              const int *offs, double *acc,
+                                              double *__restrict__ work) {
+  constexpr int G = C::G, NR = C::NR, T = C::T;
+  double *rw = work;
+  double *tab = work + 2 * NR;
+  double AB[3], CD[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    AB[x] = g.A[x] - g.B[x];
+    CD[x] = g.C[x] - g.D[x];
+  }
+  const int ob = p.pp_off[pb], nbp = p.pp_n[pb];
+  const int ok = p.pp_off[pk], nkp = p.pp_n[pk];
+  for (int ib = 0; ib < nbp; ++ib) {
+    const double pz = p.zeta[ob + ib];
+    const double P[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
+    const double kab = p.kc[ob + ib];
+    for (int ik = 0; ik < nkp; ++ik) {
+      const double qz = p.zeta[ok + ik];
+      const double Q[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
+      const double kcd = p.kc[ok + ik];
+      const double pq = pz + qz;
+      const double rho = pz * qz / pq;
+      double PQ[3], r2 = 0.0;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        PQ[x] = P[x] - Q[x];
+        r2 += PQ[x] * PQ[x];
+      }
+      double Tq = rho * r2;
+      double pref = kTwoPi52 / (pz * qz * sqrt(pq)) * kab * kcd;
+      double s_att = 1.0;
+      if (omega > 0.0) {  // erf(w r)/r: roots/weights at sT, x -> s x, w -> sqrt(s) w
+        s_att = omega * omega / (rho + omega * omega);
+        Tq *= s_att;
+        pref *= sqrt(s_att);
+      }
+      // phase 1: roots and weights
+      for (int idx = tl; idx < 2 * NR; idx += G) {
+        double v = rys_series(R, NR, Tq, idx);
+        rw[idx] = (idx < NR) ? v * s_att : v;
+      }
+      team_sync();
+      // phase 2: per (root, axis) 2D tables
+      for (int task = tl; task < 3 * NR; task += G) {
+        const int r = task / 3, ax = task - 3 * r;
+        // per-axis scalars by select, keeping the small arrays in registers
+        auto pick = [ax](const double *v) { return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]); };
+        const double x = rw[r];
+        const double b00 = x / (2.0 * pq);
+        const double b10 = (1.0 - x * qz / pq) / (2.0 * pz);
+        const double b01 = (1.0 - x * pz / pq) / (2.0 * qz);
+        const double pqa = pick(PQ);
+        const double c00 = (pick(P) - pick(g.A)) - (x * qz / pq) * pqa;
+        const double cp0 = (pick(Q) - pick(g.C)) + (x * pz / pq) * pqa;
+        rys_axis_table<C::LA, C::LB, C::LC, C::LD>(c00, cp0, b10, b01, b00, pick(AB), pick(CD),
+                                                   tab + task * T);
+      }
+      team_sync();
+      // phase 3: assembly, one root at a time so only 3E table reads are live
+#pragma unroll 1
+      for (int r = 0; r < NR; ++r) {
+        const double pw = pref * rw[NR + r];
+        const double *tx = tab + (3 * r) * T, *ty = tx + T, *tz = ty + T;
+#pragma unroll
+        for (int k = 0; k < C::E; ++k) {
+          if (tl + G * k < C::N4) {
+            const int o = offs[k];
+            acc[k] += pw * (tx[o & 1023] * ty[(o >> 10) & 1023] * tz[o >> 20]);
+          }
+        }
+      }
+      team_sync();
+    }
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void init_offsets(int tl, int *offs) {
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) {
+    int e = tl + C::G * k;
+    offs[k] = e < C::N4 ? elem_offsets<C>(e) : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// J/K: Jp[ab] += 2f g D[cd], Jp[cd] += 2f g D[ab], Kp[ac] += f g D[bd],
+// Kp[bc] += f g D[ad], Kp[ad] += f g D[bc], Kp[bd] += f g D[ac]  with
+// f = 1/|stabiliser| of the shell quartet; J = Jp + Jp^T, K = Kp +- Kp^T.
+// This equals the element scatter of direct.rs:284-404 summed over the
+// canonical quartets (exact for any density; the antisymmetric part of a
+// non-symmetric D only enters K).
+template <class C>
+__global__ void __launch_bounds__(64) jk_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs,
+                                                DensDev dd, double omega,
+                                                unsigned long long *counters) {
+  constexpr int G = C::G, PER = cmax(C::WORK, C::N4);
+  __shared__ double lds[C::TEAMS * PER];
+  const int lane = threadIdx.x, team = lane / G, tl = lane % G;
+  double *work = lds + team * PER;
+  int offs[C::E];
+  init_offsets<C>(tl, offs);
+  const int ncart2 = b.nao_cart * b.nao_cart;
+  unsigned long long my_quartets = 0, my_prims = 0;
+  for (long long t = (long long)blockIdx.x * C::TEAMS + team; t < qs.total;
+       t += (long long)gridDim.x * C::TEAMS) {
+    int P, Q;
+    decode_quartet(qs, t, P, Q);
+    const int pb = qs.x_off + P, pk = qs.y_off + Q;
+    QuartetGeom g;
+    load_geom(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk], g);
+    // density-weighted Schwarz mask (direct.rs:176-190) on symmetrised bounds
+    const double pair_bound = p.q[pb] * p.q[pk];
+    unsigned long long mask = 0;
+    const int nsh = b.nsh;
+    for (int d = 0; d < dd.ndens; ++d) {
+      const double *bd = dd.bound + (size_t)d * nsh * nsh;
+      double m = bd[g.sA * nsh + g.sB];
+      m = fmax(m, bd[g.sC * nsh + g.sD]);
+      m = fmax(m, bd[g.sA * nsh + g.sC]);
+      m = fmax(m, bd[g.sA * nsh + g.sD]);
+      m = fmax(m, bd[g.sB * nsh + g.sC]);
+      m = fmax(m, bd[g.sB * nsh + g.sD]);
+      if (pair_bound * m >= dd.tau) mask |= 1ull << d;
+    }
+    if (!mask) continue;
+    double acc[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) acc[k] = 0.0;
+    quartet_block<C>(p, R, pb, pk, g, omega, tl, offs, acc, work);
+    if (tl == 0) {
+      ++my_quartets;
+      my_prims += (unsigned long long)p.pp_n[pb] * p.pp_n[pk];
+    }
+    // block -> LDS
+    double *blk = work;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k)
+      if (tl + G * k < C::N4) blk[tl + G * k] = acc[k];
+    team_sync();
+    const double f = (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (pb == pk ? 0.5 : 1.0);
+    const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
+              d0 = b.off_cart[g.sD];
+    const int ncrt = b.nao_cart;
+    constexpr int na = C::na, nb = C::nb, nc = C::nc, nd = C::nd;
+    constexpr int NJ = na * nb + nc * nd;
+    const int NO = NJ + (dd.want_k ? na * nc + nb * nc + na * nd + nb * nd : 0);
+    for (int e = 0; e < dd.neff; ++e) {
+      if (!((mask >> dd.eff_src[e]) & 1ull)) continue;
+      const bool anti = dd.eff_anti[e];
+      const double *D = dd.dc + (size_t)e * ncart2;
+      double *J = dd.jc + (size_t)e * ncart2;
+      double *K = dd.kc ? dd.kc + (size_t)e * ncart2 : nullptr;
+      for (int o = tl; o < NO; o += G) {
+        double v = 0.0;
+        int row, col;
+        double *dst;
+        if (o < na * nb) {  // Jp[a,b]
+          if (anti) continue;
+          const int a = o / nb, bb = o % nb;
+          for (int c = 0; c < nc; ++c)
+            for (int d = 0; d < nd; ++d)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(c0 + c) * ncrt + d0 + d];
+          v *= 2.0 * f;
+          row = a0 + a;
+          col = b0 + bb;
+          dst = J;
+        } else if (o < NJ) {  // Jp[c,d]
+          if (anti) continue;
+          const int q = o - na * nb, c = q / nd, d = q % nd;
+          for (int a = 0; a < na; ++a)
+            for (int bb = 0; bb < nb; ++bb)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + b0 + bb];
+          v *= 2.0 * f;
+          row = c0 + c;
+          col = d0 + d;
+          dst = J;
+        } else {
+          int q = o - NJ;
+          if (q < na * nc) {  // Kp[a,c] += g D[b,d]
+            const int a = q / nc, c = q % nc;
+            for (int bb = 0; bb < nb; ++bb)
+              for (int d = 0; d < nd; ++d)
+                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * ncrt + d0 + d];
+            row = a0 + a;
+            col = c0 + c;
+          } else if ((q -= na * nc) < nb * nc) {  // Kp[b,c] += g D[a,d]
+            const int bb = q / nc, c = q % nc;
+            for (int a = 0; a < na; ++a)
+              for (int d = 0; d < nd; ++d)
+                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + d0 + d];
+            row = b0 + bb;
+            col = c0 + c;
+          } else if ((q -= nb * nc) < na * nd) {  // Kp[a,d] += g D[b,c]
+            const int a = q / nd, d = q % nd;
+            for (int bb = 0; bb < nb; ++bb)
+              for (int c = 0; c < nc; ++c)
+                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * ncrt + c0 + c];
+            row = a0 + a;
+            col = d0 + d;
+          } else {  // Kp[b,d] += g D[a,c]
+            q -= na * nd;
+            const int bb = q / nd, d = q % nd;
+            for (int a = 0; a < na; ++a)
+              for (int c = 0; c < nc; ++c)
+                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + c0 + c];
+            row = b0 + bb;
+            col = d0 + d;
+          }
+          v *= f;
+          dst = K;
+        }
+        atomicAdd(dst + (size_t)row * ncrt + col, v);
+      }
+    }
+    team_sync();
+  }
+  if (counters && tl == 0) {
+    atomicAdd(counters, my_quartets);
+    atomicAdd(counters + 1, my_prims);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Schwarz: Q[P] = sqrt(max_{mu,nu} |(mu nu|mu nu)|) in function space
+// (integrals.rs:842-862) from the Cartesian (P|P) block.
+template <class C>
+__global__ void __launch_bounds__(64) schwarz_kernel(BasisDev b, PairsDev p, RysDev R, int first,
+                                                     int count, double *qout) {
+  static_assert(C::LA == C::LC && C::LB == C::LD, "diagonal class");
+  constexpr int G = C::G, PER = cmax(C::WORK, C::N4 + G);
+  __shared__ double lds[C::TEAMS * PER];
+  const int lane = threadIdx.x, team = lane / G, tl = lane % G;
+  double *work = lds + team * PER;
+  int offs[C::E];
+  init_offsets<C>(tl, offs);
+  for (int t = blockIdx.x * C::TEAMS + team; t < count; t += gridDim.x * C::TEAMS) {
+    const int pr = first + t;
+    QuartetGeom g;
+    load_geom(b, p.sh1[pr], p.sh2[pr], p.sh1[pr], p.sh2[pr], g);
+    double acc[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) acc[k] = 0.0;
+    quartet_block<C>(p, R, pr, pr, g, 0.0, tl, offs, acc, work);
+    double *blk = work;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k)
+      if (tl + G * k < C::N4) blk[tl + G * k] = acc[k];
+    team_sync();
+    constexpr int na = C::na, nb = C::nb;
+    const bool sa = b.kind[g.sA], sb = b.kind[g.sB];
+    const int nfa = sa ? 2 * C::LA + 1 : na, nfb = sb ? 2 * C::LB + 1 : nb;
+    const double *Ma = b.c2s + b.c2s_off[C::LA], *Mb = b.c2s + b.c2s_off[C::LB];
+    double mx = 0.0;
+    for (int o = tl; o < nfa * nfb; o += G) {
+      const int mu = o / nfb, nu = o % nfb;
+      double v = 0.0;
+      for (int a = 0; a < na; ++a) {
+        const double ma = sa ? Ma[mu * na + a] : (a == mu ? 1.0 : 0.0);
+        if (ma == 0.0) continue;
+        for (int bb = 0; bb < nb; ++bb) {
+          const double mb = sb ? Mb[nu * nb + bb] : (bb == nu ? 1.0 : 0.0);
+          if (mb == 0.0) continue;
+          for (int c = 0; c < na; ++c) {
+            const double mc = sa ? Ma[mu * na + c] : (c == mu ? 1.0 : 0.0);
+            if (mc == 0.0) continue;
+            for (int d = 0; d < nb; ++d) {
+              const double md = sb ? Mb[nu * nb + d] : (d == nu ? 1.0 : 0.0);
+              if (md == 0.0) continue;
+              v += ma * mb * mc * md * blk[((a * nb + bb) * na + c) * nb + d];
+            }
+          }
+        }
+      }
+      mx = fmax(mx, fabs(v));
+    }
+    double *red = work + C::N4;
+    team_sync();
+    red[tl] = mx;
+    team_sync();
+    if (tl == 0) {
+      for (int i = 1; i < G; ++i) mx = fmax(mx, red[i]);
+      qout[pr] = sqrt(mx);
+    }
+    team_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense tensor: c2s in LDS, then each distinct shell permutation of the
+// quartet written once (integrals.rs:1271-1312).
+__device__ __constant__ int kPerms8[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2},
+                                             {1, 0, 3, 2}, {2, 3, 0, 1}, {2, 3, 1, 0},
+                                             {3, 2, 0, 1}, {3, 2, 1, 0}};
+
+template <class C>
+__global__ void __launch_bounds__(64) eri_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs,
+                                                 double *out) {
+  constexpr int G = C::G, PER = cmax(C::WORK, 2 * C::N4);
+  __shared__ double lds[C::TEAMS * PER];
+  const int lane = threadIdx.x, team = lane / G, tl = lane % G;
+  double *work = lds + team * PER;
+  int offs[C::E];
+  init_offsets<C>(tl, offs);
+  const size_t nao = (size_t)b.nao;
+  for (long long t = (long long)blockIdx.x * C::TEAMS + team; t < qs.total;
+       t += (long long)gridDim.x * C::TEAMS) {
+    int P, Q;
+    decode_quartet(qs, t, P, Q);
+    const int pb = qs.x_off + P, pk = qs.y_off + Q;
+    QuartetGeom g;
+    load_geom(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk], g);
+    double acc[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) acc[k] = 0.0;
+    quartet_block<C>(p, R, pb, pk, g, 0.0, tl, offs, acc, work);
+    double *cur = work, *nxt = work + C::N4;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k)
+      if (tl + G * k < C::N4) cur[tl + G * k] = acc[k];
+    team_sync();
+    const int sh[4] = {g.sA, g.sB, g.sC, g.sD};
+    const int ls[4] = {C::LA, C::LB, C::LC, C::LD};
+    int dims[4] = {C::na, C::nb, C::nc, C::nd};
+    for (int ax = 0; ax < 4; ++ax) {
+      if (!b.kind[sh[ax]]) continue;
+      const int nin = dims[ax], nout = 2 * ls[ax] + 1;
+      const double *M = b.c2s + b.c2s_off[ls[ax]];
+      int outer = 1, inner = 1;
+      for (int x = 0; x < ax; ++x) outer *= dims[x];
+      for (int x = ax + 1; x < 4; ++x) inner *= dims[x];
+      for (int o = tl; o < outer * nout * inner; o += G) {
+        const int i = o % inner, rq = o / inner, q = rq % nout, oo = rq / nout;
+        double v = 0.0;
+        for (int j = 0; j < nin; ++j) v += M[q * nin + j] * cur[(oo * nin + j) * inner + i];
+        nxt[o] = v;
+      }
+      dims[ax] = nout;
+      double *tmp = cur;
+      cur = nxt;
+      nxt = tmp;
+      team_sync();
+    }
+    const int nf = dims[0] * dims[1] * dims[2] * dims[3];
+    int offf[4];
+    for (int x = 0; x < 4; ++x) offf[x] = b.off_func[sh[x]];
+    int seen[8][4];
+    int nseen = 0;
+    for (int pi = 0; pi < 8; ++pi) {
+      int tup[4];
+      for (int q = 0; q < 4; ++q) tup[q] = sh[kPerms8[pi][q]];
+      bool dup = false;
+      for (int s = 0; s < nseen; ++s)
+        dup |= (seen[s][0] == tup[0] && seen[s][1] == tup[1] && seen[s][2] == tup[2] &&
+                seen[s][3] == tup[3]);
+      if (dup) continue;
+      for (int q = 0; q < 4; ++q) seen[nseen][q] = tup[q];
+      ++nseen;
+      size_t stride[4];
+      size_t base = 0;
+      for (int q = 0; q < 4; ++q) {
+        const int src = kPerms8[pi][q];
+        size_t st = 1;
+        for (int e = 0; e < 3 - q; ++e) st *= nao;
+        stride[src] = st;
+        base += (size_t)offf[src] * st;
+      }
+      for (int e = tl; e < nf; e += G) {
+        const int id = e % dims[3], r1 = e / dims[3], ic = r1 % dims[2], r2 = r1 / dims[2],
+                  ibb = r2 % dims[1], ia = r2 / dims[1];
+        out[base + ia * stride[0] + ibb * stride[1] + ic * stride[2] + id * stride[3]] = cur[e];
+      }
+    }
+    team_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ERI-derivative contraction (adjoint form).  For each canonical quartet the
+// Cartesian weight block W (the permutation-averaged two-particle density of
+// direct_grad.rs:490-547, built from Cartesian-transformed density matrices)
+// is contracted on the fly with the centre derivatives
+//   d/dA_x (ab|cd) = 2 alpha (a+1x b|cd) - a_x (a-1x b|cd)   (B, C alike)
+// evaluated from 2D Rys tables raised by one on A, B and C; D follows from
+// translational invariance (direct_grad.rs:425-484).  Nothing of size nao^4
+// is formed and the tables are recomputed per primitive quartet.
+template <class C>
+struct GradCls {
+  using R = Cls<C::LA + 1, C::LB + 1, C::LC + 1, C::LD>;  // raised table extents
+  static constexpr int NR = (C::LA + C::LB + C::LC + C::LD + 1) / 2 + 1;
+  static constexpr int T = R::T;
+  static constexpr int SA = (C::LB + 2) * (C::LC + 2) * (C::LD + 1);
+  static constexpr int SB = (C::LC + 2) * (C::LD + 1);
+  static constexpr int SC = C::LD + 1;
+  static constexpr int WORK = 2 * NR + 3 * NR * T;
+};
+
+// element e -> packed raised-table offsets and packed powers (2 bits each:
+// ax ay az bx by bz cx cy cz)
+template <class C>
+__device__ __forceinline__ void grad_elem(int e, int &offs, int &pows) {
+  using GC = GradCls<C>;
+  int id = e % C::nd;
+  int r = e / C::nd;
+  int ic = r % C::nc;
+  r /= C::nc;
+  int ib = r % C::nb;
+  int ia = r / C::nb;
+  int p[4][3];
+  cart_powers(C::LA, ia, p[0][0], p[0][1], p[0][2]);
+  cart_powers(C::LB, ib, p[1][0], p[1][1], p[1][2]);
+  cart_powers(C::LC, ic, p[2][0], p[2][1], p[2][2]);
+  cart_powers(C::LD, id, p[3][0], p[3][1], p[3][2]);
+  offs = 0;
+  pows = 0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    int o = p[0][ax] * GC::SA + p[1][ax] * GC::SB + p[2][ax] * GC::SC + p[3][ax];
+    offs |= o << (10 * ax);
+    pows |= p[0][ax] << (2 * ax);
+    pows |= p[1][ax] << (6 + 2 * ax);
+    pows |= p[2][ax] << (12 + 2 * ax);
+  }
+}
+
+template <class C>
+__global__ void __launch_bounds__(64) grad_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs,
+                                                  GradDev gd, unsigned long long *counters) {
+  using GC = GradCls<C>;
+  constexpr int G = C::G, NR = GC::NR, T = GC::T, PER = cmax(GC::WORK, 9 * G);
+  __shared__ double lds[C::TEAMS * PER];
+  const int lane = threadIdx.x, team = lane / G, tl = lane % G;
+  double *work = lds + team * PER;
+  int offs[C::E], pows[C::E];
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) {
+    const int e = tl + G * k;
+    offs[k] = 0;
+    pows[k] = 0;
+    if (e < C::N4) grad_elem<C>(e, offs[k], pows[k]);
+  }
+  const int nsh = b.nsh;
+  const size_t nc2 = (size_t)b.nao_cart * b.nao_cart;
+  const int ncrt = b.nao_cart;
+  const int nterm = gd.ncoul + gd.nexch;
+  unsigned long long my_q = 0, my_p = 0;
+  for (long long t = (long long)blockIdx.x * C::TEAMS + team; t < qs.total;
+       t += (long long)gridDim.x * C::TEAMS) {
+    int P, Q;
+    decode_quartet(qs, t, P, Q);
+    const int pb = qs.x_off + P, pk = qs.y_off + Q;
+    QuartetGeom g;
+    load_geom(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk], g);
+    const int atA = b.atom[g.sA], atB = b.atom[g.sB], atC = b.atom[g.sC], atD = b.atom[g.sD];
+    if (atA == atB && atB == atC && atC == atD) continue;  // direct_grad.rs:356-358
+    const double deg = (g.sA == g.sB ? 1.0 : 2.0) * (g.sC == g.sD ? 1.0 : 2.0) * (pb == pk ? 1.0 : 2.0);
+    // density block bound (direct_grad.rs:626-644), symmetrised maxima
+    double bound = 0.0;
+    {
+      const int i = g.sA, j = g.sB, k = g.sC, l = g.sD;
+      for (int tm = 0; tm < nterm; ++tm) {
+        const double *Lm = gd.bmax + (size_t)(2 * tm) * nsh * nsh;
+        const double *Rm = Lm + (size_t)nsh * nsh;
+        if (tm < gd.ncoul)
+          bound += 0.5 * fabs(gd.fac[tm]) *
+                   (Lm[i * nsh + j] * Rm[k * nsh + l] + Lm[k * nsh + l] * Rm[i * nsh + j]);
+        else
+          bound += 0.25 * fabs(gd.fac[tm]) *
+                   (Lm[i * nsh + k] * Rm[j * nsh + l] + Lm[j * nsh + k] * Rm[i * nsh + l] +
+                    Lm[i * nsh + l] * Rm[j * nsh + k] + Lm[j * nsh + l] * Rm[i * nsh + k]);
+      }
+    }
+    if (p.q[pb] * p.q[pk] * deg * bound < gd.tau) continue;
+    // weight block for this lane's elements (Cartesian)
+    const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
+              d0 = b.off_cart[g.sD];
+    double wv[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      const int e = tl + G * k;
+      double w = 0.0;
+      if (e < C::N4) {
+        const int id = e % C::nd, r1 = e / C::nd, ic = r1 % C::nc, r2 = r1 / C::nc,
+                  ib = r2 % C::nb, ia = r2 / C::nb;
+        const size_t A = a0 + ia, B = b0 + ib, Cc = c0 + ic, D = d0 + id;
+        for (int tm = 0; tm < nterm; ++tm) {
+          const double *Lc = gd.mats + (size_t)(2 * tm) * nc2, *Rc = Lc + nc2;
+          if (tm < gd.ncoul)
+            w += 0.5 * deg * gd.fac[tm] *
+                 (Lc[A * ncrt + B] * Rc[Cc * ncrt + D] + Rc[A * ncrt + B] * Lc[Cc * ncrt + D]);
+          else
+            w += 0.25 * deg * gd.fac[tm] *
+                 (Lc[A * ncrt + Cc] * Rc[B * ncrt + D] + Lc[B * ncrt + Cc] * Rc[A * ncrt + D] +
+                  Lc[A * ncrt + D] * Rc[B * ncrt + Cc] + Lc[B * ncrt + D] * Rc[A * ncrt + Cc]);
+        }
+      }
+      wv[k] = w;
+    }
+    // 9 accumulators: A(x,y,z), B(x,y,z), C(x,y,z)
+    double ga[9];
+#pragma unroll
+    for (int x = 0; x < 9; ++x) ga[x] = 0.0;
+    double AB[3], CD[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      AB[x] = g.A[x] - g.B[x];
+      CD[x] = g.C[x] - g.D[x];
+    }
+    double *rw = work, *tab = work + 2 * NR;
+    const int ob = p.pp_off[pb], nbp = p.pp_n[pb], ok = p.pp_off[pk], nkp = p.pp_n[pk];
+    for (int ib = 0; ib < nbp; ++ib) {
+      const double pz = p.zeta[ob + ib], al = p.alpha[ob + ib], be = p.beta[ob + ib];
+      const double Pv[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
+      const double kab = p.kc[ob + ib];
+      for (int ik = 0; ik < nkp; ++ik) {
+        const double qz = p.zeta[ok + ik], ga_ = p.alpha[ok + ik];
+        const double Qv[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
+        const double kcd = p.kc[ok + ik];
+        const double pq = pz + qz, rho = pz * qz / pq;
+        double PQ[3], r2 = 0.0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          PQ[x] = Pv[x] - Qv[x];
+          r2 += PQ[x] * PQ[x];
+        }
+        double Tq = rho * r2;
+        double pref = kTwoPi52 / (pz * qz * sqrt(pq)) * kab * kcd;
+        double s_att = 1.0;
+        if (gd.omega > 0.0) {
+          s_att = gd.omega * gd.omega / (rho + gd.omega * gd.omega);
+          Tq *= s_att;
+          pref *= sqrt(s_att);
+        }
+        for (int idx = tl; idx < 2 * NR; idx += G) {
+          double v = rys_series(R, NR, Tq, idx);
+          rw[idx] = (idx < NR) ? v * s_att : v;
+        }
+        team_sync();
+        for (int task = tl; task < 3 * NR; task += G) {
+          const int r = task / 3, ax = task - 3 * r;
+          auto pick = [ax](const double *v) { return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]); };
+          const double x = rw[r];
+          const double b00 = x / (2.0 * pq);
+          const double b10 = (1.0 - x * qz / pq) / (2.0 * pz);
+          const double b01 = (1.0 - x * pz / pq) / (2.0 * qz);
+          const double pqa = pick(PQ);
+          const double c00 = (pick(Pv) - pick(g.A)) - (x * qz / pq) * pqa;
+          const double cp0 = (pick(Qv) - pick(g.C)) + (x * pz / pq) * pqa;
+          rys_axis_table<C::LA + 1, C::LB + 1, C::LC + 1, C::LD>(c00, cp0, b10, b01, b00,
+                                                                 pick(AB), pick(CD), tab + task * T);
+        }
+        team_sync();
+        const double ta = 2.0 * al, tb = 2.0 * be, tc = 2.0 * ga_;
+#pragma unroll 1
+        for (int r = 0; r < NR; ++r) {
+          const double pw = pref * rw[NR + r];
+          const double *tt[3] = {tab + (3 * r) * T, tab + (3 * r + 1) * T, tab + (3 * r + 2) * T};
+#pragma unroll
+          for (int k = 0; k < C::E; ++k) {
+            if (tl + G * k < C::N4) {
+              const int o = offs[k], pw2 = pows[k];
+              double v0[3], dA[3], dB[3], dC[3];
+#pragma unroll
+              for (int ax = 0; ax < 3; ++ax) {
+                const double *tx = tt[ax];
+                const int oo = (o >> (10 * ax)) & 1023;
+                const int pa = (pw2 >> (2 * ax)) & 3, pb_ = (pw2 >> (6 + 2 * ax)) & 3,
+                          pc = (pw2 >> (12 + 2 * ax)) & 3;
+                v0[ax] = tx[oo];
+                // lowered reads stay in bounds: the offset only steps down when
+                // the power is non-zero (otherwise the term is multiplied by 0)
+                dA[ax] = ta * tx[oo + GC::SA] - double(pa) * tx[oo - (pa ? GC::SA : 0)];
+                dB[ax] = tb * tx[oo + GC::SB] - double(pb_) * tx[oo - (pb_ ? GC::SB : 0)];
+                dC[ax] = tc * tx[oo + GC::SC] - double(pc) * tx[oo - (pc ? GC::SC : 0)];
+              }
+              const double s = pw * wv[k];
+              const double yz = v0[1] * v0[2], xz = v0[0] * v0[2], xy = v0[0] * v0[1];
+              ga[0] += s * dA[0] * yz;
+              ga[1] += s * dA[1] * xz;
+              ga[2] += s * dA[2] * xy;
+              ga[3] += s * dB[0] * yz;
+              ga[4] += s * dB[1] * xz;
+              ga[5] += s * dB[2] * xy;
+              ga[6] += s * dC[0] * yz;
+              ga[7] += s * dC[1] * xz;
+              ga[8] += s * dC[2] * xy;
+            }
+          }
+        }
+        team_sync();
+      }
+    }
+    if (tl == 0) {
+      ++my_q;
+      my_p += (unsigned long long)nbp * nkp;
+    }
+    // team reduction of the 9 sums, then per-atom accumulation
+    double *red = work;
+#pragma unroll
+    for (int x = 0; x < 9; ++x) red[x * G + tl] = ga[x];
+    team_sync();
+    for (int comp = tl; comp < 9; comp += G) {
+      double s = 0.0;
+      for (int i = 0; i < G; ++i) s += red[comp * G + i];
+      const int which = comp / 3, ax = comp % 3;
+      const int at = which == 0 ? atA : (which == 1 ? atB : atC);
+      atomicAdd(gd.out + 3 * at + ax, s);
+      atomicAdd(gd.out + 3 * atD + ax, -s);
+    }
+    team_sync();
+  }
+  if (counters && tl == 0) {
+    atomicAdd(counters, my_q);
+    atomicAdd(counters + 1, my_p);
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t grad_one(const BasisDev &b, const PairsDev &p, const RysDev &r, const QuartetSet &qs,
+                    const GradDev &g, unsigned long long *ctr, hipStream_t s) {
+  using C = Cls<LA, LB, LC, LD>;
+  if (qs.total <= 0) return hipSuccess;
+  long long blocks = (qs.total + C::TEAMS - 1) / C::TEAMS;
+  if (blocks > 256 * 64) blocks = 256 * 64;
+  hipLaunchKernelGGL(grad_kernel<C>, dim3(int(blocks)), dim3(64), 0, s, b, p, r, qs, g, ctr);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// dispatch tables over (la, lb, lc, ld) with la >= lb, lc >= ld, l <= kMaxKernelL
+
+namespace {
+
+template <int LA, int LB, int LC, int LD>
+inline int grid_for(long long total) {
+  using C = Cls<LA, LB, LC, LD>;
+  long long blocks = (total + C::TEAMS - 1) / C::TEAMS;
+  const long long cap = 256 * 64;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  return int(blocks);
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t jk_one(const BasisDev &b, const PairsDev &p, const RysDev &r, const QuartetSet &qs,
+                  const DensDev &d, double omega, unsigned long long *ctr, hipStream_t s) {
+  using C = Cls<LA, LB, LC, LD>;
+  if (qs.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(jk_kernel<C>, dim3(grid_for<LA, LB, LC, LD>(qs.total)), dim3(64), 0, s, b,
+                     p, r, qs, d, omega, ctr);
+  return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD>
+hipError_t eri_one(const BasisDev &b, const PairsDev &p, const RysDev &r, const QuartetSet &qs,
+                   double *out, hipStream_t s) {
+  using C = Cls<LA, LB, LC, LD>;
+  if (qs.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(eri_kernel<C>, dim3(grid_for<LA, LB, LC, LD>(qs.total)), dim3(64), 0, s, b,
+                     p, r, qs, out);
+  return hipGetLastError();
+}
+
+template <int LA, int LB>
+hipError_t schwarz_one(const BasisDev &b, const PairsDev &p, const RysDev &r, int first,
+                       int count, double *q, hipStream_t s) {
+  using C = Cls<LA, LB, LA, LB>;
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(schwarz_kernel<C>, dim3(grid_for<LA, LB, LA, LB>(count)), dim3(64), 0, s, b,
+                     p, r, first, count, q);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Class pairs with pair_class(la,lb) >= pair_class(lc,ld), shells up to d:
+// the kernel orientation the plan guarantees.
+#define IG_FOR_CLASSES(X)                                                                    \
+  X(0, 0, 0, 0)                                                                              \
+  X(1, 0, 0, 0) X(1, 0, 1, 0)                                                                \
+  X(1, 1, 0, 0) X(1, 1, 1, 0) X(1, 1, 1, 1)                                                  \
+  X(2, 0, 0, 0) X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0)                                    \
+  X(2, 1, 0, 0) X(2, 1, 1, 0) X(2, 1, 1, 1) X(2, 1, 2, 0) X(2, 1, 2, 1)                      \
+  X(2, 2, 0, 0) X(2, 2, 1, 0) X(2, 2, 1, 1) X(2, 2, 2, 0) X(2, 2, 2, 1) X(2, 2, 2, 2)
+
+bool class_supported(int la, int lb, int lc, int ld) {
+  return la <= kMaxKernelL && lb <= la && lc <= kMaxKernelL && ld <= lc &&
+         pair_class(la, lb) >= pair_class(lc, ld);
+}
+
+hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                     const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
+                     unsigned long long *ctr, hipStream_t s) {
+#define IG_CASE(A, B, Cc, D) \
+  if (la == A && lb == B && lc == Cc && ld == D) return jk_one<A, B, Cc, D>(b, p, r, qs, d, omega, ctr, s);
+  IG_FOR_CLASSES(IG_CASE)
+#undef IG_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                      const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s) {
+#define IG_CASE(A, B, Cc, D) \
+  if (la == A && lb == B && lc == Cc && ld == D) return eri_one<A, B, Cc, D>(b, p, r, qs, out, s);
+  IG_FOR_CLASSES(IG_CASE)
+#undef IG_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_grad(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                       const RysDev &r, const QuartetSet &qs, const GradDev &g,
+                       unsigned long long *ctr, hipStream_t s) {
+#define IG_CASE(A, B, Cc, D) \
+  if (la == A && lb == B && lc == Cc && ld == D) return grad_one<A, B, Cc, D>(b, p, r, qs, g, ctr, s);
+  IG_FOR_CLASSES(IG_CASE)
+#undef IG_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_schwarz(int l1, int l2, const BasisDev &b, const PairsDev &p, const RysDev &r,
+                          int first, int count, double *q, hipStream_t s) {
+  if (l1 == 0 && l2 == 0) return schwarz_one<0, 0>(b, p, r, first, count, q, s);
+  if (l1 == 1 && l2 == 0) return schwarz_one<1, 0>(b, p, r, first, count, q, s);
+  if (l1 == 1 && l2 == 1) return schwarz_one<1, 1>(b, p, r, first, count, q, s);
+  if (l1 == 2 && l2 == 0) return schwarz_one<2, 0>(b, p, r, first, count, q, s);
+  if (l1 == 2 && l2 == 1) return schwarz_one<2, 1>(b, p, r, first, count, q, s);
+  if (l1 == 2 && l2 == 2) return schwarz_one<2, 2>(b, p, r, first, count, q, s);
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// dense helpers
+
+// shell of function-space / Cartesian index via binary search on offsets
+__device__ __forceinline__ int shell_of(const int *off, int nsh, int idx) {
+  int lo = 0, hi = nsh - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= idx)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// M^T X M per shell block: Cartesian (nao_cart^2) from function space (nao^2)
+__global__ void to_cart_kernel(BasisDev b, const double *__restrict__ mf, int nmats,
+                               double *__restrict__ mc) {
+  const size_t nc2 = (size_t)b.nao_cart * b.nao_cart;
+  const size_t total = nc2 * nmats;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int m = int(t / nc2);
+    const int rem = int(t % nc2);
+    const int I = rem / b.nao_cart, J = rem % b.nao_cart;
+    const int S = shell_of(b.off_cart, b.nsh, I), U = shell_of(b.off_cart, b.nsh, J);
+    const int a = I - b.off_cart[S], c = J - b.off_cart[U];
+    const int lS = b.l[S], lU = b.l[U];
+    const int nS = b.kind[S] ? 2 * lS + 1 : ncart(lS), nU = b.kind[U] ? 2 * lU + 1 : ncart(lU);
+    const double *MS = b.c2s + b.c2s_off[lS], *MU = b.c2s + b.c2s_off[lU];
+    const double *X = mf + (size_t)m * b.nao * b.nao;
+    double v = 0.0;
+    for (int mu = 0; mu < nS; ++mu) {
+      const double ms = b.kind[S] ? MS[mu * ncart(lS) + a] : (mu == a ? 1.0 : 0.0);
+      if (ms == 0.0) continue;
+      const double *row = X + (size_t)(b.off_func[S] + mu) * b.nao + b.off_func[U];
+      for (int nu = 0; nu < nU; ++nu) {
+        const double mu_ = b.kind[U] ? MU[nu * ncart(lU) + c] : (nu == c ? 1.0 : 0.0);
+        v += ms * row[nu] * mu_;
+      }
+    }
+    mc[t] = v;
+  }
+}
+
+// symmetric / antisymmetric split of densities (function space)
+__global__ void split_kernel(int nao, const double *__restrict__ d, int ndens, int has_anti,
+                             double *__restrict__ ds, double *__restrict__ da) {
+  const size_t n2 = (size_t)nao * nao;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n2 * ndens;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t m = t / n2, r = t % n2, i = r / nao, j = r % nao;
+    const double x = d[t], y = d[m * n2 + j * nao + i];
+    ds[t] = 0.5 * (x + y);
+    if (has_anti) da[t] = 0.5 * (x - y);
+  }
+}
+
+// max |X| per shell block, symmetrised over (i,j)/(j,i); X in function space
+__global__ void block_max_kernel(BasisDev b, const double *__restrict__ x, int nmats,
+                                 double *__restrict__ out) {
+  const int nsh = b.nsh;
+  const size_t total = (size_t)nsh * nsh * nmats;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int m = int(t / ((size_t)nsh * nsh));
+    const int r = int(t % ((size_t)nsh * nsh));
+    const int i = r / nsh, j = r % nsh;
+    const double *X = x + (size_t)m * b.nao * b.nao;
+    const int ni = b.kind[i] ? 2 * b.l[i] + 1 : ncart(b.l[i]);
+    const int nj = b.kind[j] ? 2 * b.l[j] + 1 : ncart(b.l[j]);
+    double mx = 0.0;
+    for (int a = 0; a < ni; ++a)
+      for (int c = 0; c < nj; ++c) {
+        mx = fmax(mx, fabs(X[(size_t)(b.off_func[i] + a) * b.nao + b.off_func[j] + c]));
+        mx = fmax(mx, fabs(X[(size_t)(b.off_func[j] + c) * b.nao + b.off_func[i] + a]));
+      }
+    out[t] = mx;
+  }
+}
+
+// J_d += M (Jp + Jp^T) M^T, K_d += M (Kp_s + Kp_s^T + Kp_a - Kp_a^T) M^T; one
+// thread per (density, mu, nu) sums every effective density of that source.
+__global__ void jk_finish_kernel(BasisDev b, DensDev d, double *__restrict__ J,
+                                 double *__restrict__ K) {
+  const size_t n2 = (size_t)b.nao * b.nao;
+  const size_t nc2 = (size_t)b.nao_cart * b.nao_cart;
+  const int ncrt = b.nao_cart;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n2 * d.ndens;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int src = int(t / n2);
+    const int r = int(t % n2);
+    const int mu = r / b.nao, nu = r % b.nao;
+    const int S = shell_of(b.off_func, b.nsh, mu), U = shell_of(b.off_func, b.nsh, nu);
+    const int p = mu - b.off_func[S], q = nu - b.off_func[U];
+    const int lS = b.l[S], lU = b.l[U], nS = ncart(lS), nU = ncart(lU);
+    const double *MS = b.c2s + b.c2s_off[lS], *MU = b.c2s + b.c2s_off[lU];
+    double vj = 0.0, vk = 0.0;
+    for (int e = 0; e < d.neff; ++e) {
+      if (d.eff_src[e] != src) continue;
+      const bool anti = d.eff_anti[e];
+      const double *Jp = d.jc + (size_t)e * nc2;
+      const double *Kp = d.kc ? d.kc + (size_t)e * nc2 : nullptr;
+      for (int a = 0; a < nS; ++a) {
+        const double ms = b.kind[S] ? MS[p * nS + a] : (a == p ? 1.0 : 0.0);
+        if (ms == 0.0) continue;
+        const size_t I = (size_t)b.off_cart[S] + a;
+        for (int c = 0; c < nU; ++c) {
+          const double mu_ = b.kind[U] ? MU[q * nU + c] : (c == q ? 1.0 : 0.0);
+          if (mu_ == 0.0) continue;
+          const size_t Jx = (size_t)b.off_cart[U] + c;
+          const double w = ms * mu_;
+          if (!anti) vj += w * (Jp[I * ncrt + Jx] + Jp[Jx * ncrt + I]);
+          if (Kp)
+            vk += w * (anti ? Kp[I * ncrt + Jx] - Kp[Jx * ncrt + I]
+                            : Kp[I * ncrt + Jx] + Kp[Jx * ncrt + I]);
+        }
+      }
+    }
+    J[t] += vj;
+    if (d.kc && K) K[t] += vk;
+  }
+}
+
+__global__ void fill_kernel(double *__restrict__ x, size_t n, double v) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x)
+    x[t] = v;
+}
+
+static int grid1d(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  return int(g < 1 ? 1 : g);
+}
+
+hipError_t launch_to_cart(const BasisDev &b, const double *mf, int nmats, double *mc,
+                          hipStream_t s) {
+  size_t n = (size_t)b.nao_cart * b.nao_cart * nmats;
+  hipLaunchKernelGGL(to_cart_kernel, dim3(grid1d(n)), dim3(256), 0, s, b, mf, nmats, mc);
+  return hipGetLastError();
+}
+
+__global__ void asym_flag_kernel(const double *__restrict__ d, int n, int ndens, int *flag) {
+  const size_t n2 = (size_t)n * n;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n2 * ndens;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t m = t / n2, r = t % n2, i = r / n, j = r % n;
+    if (d[t] != d[m * n2 + j * n + i]) atomicOr(flag, 1);
+  }
+}
+
+hipError_t launch_asym_flag(const double *d, int n, int ndens, int *flag, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
+  if (e != hipSuccess) return e;
+  size_t total = (size_t)n * n * ndens;
+  hipLaunchKernelGGL(asym_flag_kernel, dim3(grid1d(total)), dim3(256), 0, s, d, n, ndens, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double *x, size_t n, double v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid1d(n)), dim3(256), 0, s, x, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_density_prep(const BasisDev &b, const double *d_func, int ndens, int has_anti,
+                               double *ds, double *da, double *dc, double *bound,
+                               hipStream_t s) {
+  const size_t n2 = (size_t)b.nao * b.nao, nc2 = (size_t)b.nao_cart * b.nao_cart;
+  hipLaunchKernelGGL(split_kernel, dim3(grid1d(n2 * ndens)), dim3(256), 0, s, b.nao, d_func,
+                     ndens, has_anti, ds, da);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if ((e = launch_to_cart(b, ds, ndens, dc, s)) != hipSuccess) return e;
+  if (has_anti && (e = launch_to_cart(b, da, ndens, dc + (size_t)ndens * nc2, s)) != hipSuccess)
+    return e;
+  size_t nb = (size_t)b.nsh * b.nsh * ndens;
+  hipLaunchKernelGGL(block_max_kernel, dim3(grid1d(nb)), dim3(256), 0, s, b, d_func, ndens,
+                     bound);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_max(const BasisDev &b, const double *m, int nmats, double *bmax,
+                            hipStream_t s) {
+  size_t nb = (size_t)b.nsh * b.nsh * nmats;
+  hipLaunchKernelGGL(block_max_kernel, dim3(grid1d(nb)), dim3(256), 0, s, b, m, nmats, bmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_jk_finish(const BasisDev &b, const DensDev &d, double *J, double *K,
+                            hipStream_t s) {
+  size_t n = (size_t)b.nao * b.nao * d.ndens;
+  hipLaunchKernelGGL(jk_finish_kernel, dim3(grid1d(n)), dim3(256), 0, s, b, d, J, K);
+  return hipGetLastError();
+}
+
+}  // namespace ig

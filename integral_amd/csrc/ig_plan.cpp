@@ -1,0 +1,1158 @@
+// ig_plan.cpp -- host side of the C ABI (include/integral_hip.h): basis
+// validation, device plans (shell data, primitive-pair tables sorted by
+// angular class and Schwarz bound), screened quartet sets, kernel launches.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/integral_hip.h"
+#include "ig_internal.h"
+
+using namespace ig;
+
+// ---------------------------------------------------------------------------
+// errors
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+struct DeviceError {
+  int code;
+  std::string msg;
+};
+
+void check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw DeviceError{IG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const DeviceError &e) {
+    return fail(e.code, e.msg);
+  } catch (const std::bad_alloc &) {
+    return fail(IG_ERR_DEVICE, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(IG_ERR_DEVICE, e.what());
+  }
+}
+
+// device buffer owning hipMalloc'd memory
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    if (p) check(hipFree(p), "hipFree");
+    p = nullptr;
+    bytes = 0;
+    check(hipMalloc(&p, n ? n : 8), "hipMalloc");
+    bytes = n;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+  template <class T>
+  void upload(const std::vector<T> &v) {
+    reserve(v.size() * sizeof(T));
+    if (!v.empty()) check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+  }
+};
+
+int64_t pair_count(int nsh) { return int64_t(nsh) * (nsh + 1) / 2; }
+void pair_of(int64_t p, int &i, int &j) {
+  int64_t ii = int64_t((std::sqrt(8.0 * double(p) + 1.0) - 1.0) / 2.0);
+  while (ii * (ii + 1) / 2 > p) --ii;
+  while ((ii + 1) * (ii + 2) / 2 <= p) ++ii;
+  i = int(ii);
+  j = int(p - ii * (ii + 1) / 2);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// basis
+struct HostShell {
+  int l, kind, nprim;
+  double center[3];
+  std::vector<double> exps, coefs, eff;
+  int off_func, off_cart, atom;
+  int nfunc() const { return kind ? 2 * l + 1 : ncart(l); }
+};
+
+struct ig_basis {
+  std::vector<HostShell> shells;
+  int nao = 0, nao_cart = 0, natom = 0;
+};
+
+extern "C" uint32_t integral_abi_version(void) { return 1u; }
+extern "C" const char *ig_last_error_message(void) { return g_last_error.c_str(); }
+extern "C" int ig_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" int ig_basis_create(const ig_shell_desc *shells, size_t n, ig_basis **out) {
+  if (!out || (n && !shells)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    auto b = std::make_unique<ig_basis>();
+    std::vector<std::array<double, 3>> atoms;
+    for (size_t s = 0; s < n; ++s) {
+      const ig_shell_desc &d = shells[s];
+      // Shell::with_kind order of checks (shell.rs:100-111)
+      if (d.n_exponents <= 0)
+        return fail(IG_ERR_EMPTY_CONTRACTION, "shell " + std::to_string(s) + ": empty contraction");
+      if (d.n_exponents != d.n_coefficients)
+        return fail(IG_ERR_MISMATCHED_CONTRACTION,
+                    "shell " + std::to_string(s) + ": " + std::to_string(d.n_exponents) +
+                        " exponents vs " + std::to_string(d.n_coefficients) + " coefficients");
+      if (d.l < 0 || d.l > kMaxL)
+        return fail(IG_ERR_L_TOO_HIGH, "shell " + std::to_string(s) + ": l=" +
+                                           std::to_string(d.l) + " exceeds max 6");
+      if (d.kind != IG_CARTESIAN && d.kind != IG_SPHERICAL)
+        return fail(IG_ERR_INVALID_ARGUMENT, "bad shell kind");
+      if (!d.exponents || !d.coefficients) return fail(IG_ERR_INVALID_ARGUMENT, "null primitives");
+      HostShell h;
+      h.l = d.l;
+      h.kind = d.kind;
+      h.nprim = d.n_exponents;
+      std::memcpy(h.center, d.center, sizeof(h.center));
+      h.exps.assign(d.exponents, d.exponents + h.nprim);
+      h.coefs.assign(d.coefficients, d.coefficients + h.nprim);
+      h.eff.resize(h.nprim);
+      for (int i = 0; i < h.nprim; ++i)  // Shell::primitive_coeff, shell.rs:182-187
+        h.eff[i] = h.exps[i] == 0.0 ? h.coefs[i] : h.coefs[i] * shell_norm(h.exps[i], h.l);
+      h.off_func = b->nao;
+      h.off_cart = b->nao_cart;
+      b->nao += h.nfunc();
+      b->nao_cart += ncart(h.l);
+      int a = -1;
+      for (size_t k = 0; k < atoms.size(); ++k)
+        if (std::memcmp(atoms[k].data(), h.center, sizeof(h.center)) == 0) {
+          a = int(k);
+          break;
+        }
+      if (a < 0) {
+        atoms.push_back({h.center[0], h.center[1], h.center[2]});
+        a = int(atoms.size()) - 1;
+      }
+      h.atom = a;
+      b->shells.push_back(std::move(h));
+    }
+    b->natom = int(atoms.size());
+    *out = b.release();
+    return int(IG_OK);
+  });
+}
+
+extern "C" void ig_basis_destroy(ig_basis *b) { delete b; }
+extern "C" size_t ig_basis_nao(const ig_basis *b) { return b ? size_t(b->nao) : 0; }
+extern "C" size_t ig_basis_nao_cart(const ig_basis *b) { return b ? size_t(b->nao_cart) : 0; }
+extern "C" size_t ig_basis_natom(const ig_basis *b) { return b ? size_t(b->natom) : 0; }
+extern "C" size_t ig_basis_nshell(const ig_basis *b) { return b ? b->shells.size() : 0; }
+
+// ---------------------------------------------------------------------------
+// device plan: everything a quartet launch needs, built once per contractor
+namespace {
+
+struct RysDevice {
+  DBuf buf;
+  RysDev view{};
+};
+
+RysDevice &rys_device() {
+  // one copy per process on the current device (single-GPU library)
+  static RysDevice *r = nullptr;
+  static std::mutex m;
+  std::lock_guard<std::mutex> g(m);
+  if (!r) {
+    auto *x = new RysDevice();
+    const RysHostTables &t = rys_host_tables();
+    x->buf.upload(t.data);
+    x->view.buf = x->buf.as<double>();
+    for (int n = 0; n <= kMaxRys; ++n) {
+      x->view.nsub[n] = t.nsub[n];
+      x->view.root_off[n] = t.root_off[n];
+      x->view.wt_off[n] = t.wt_off[n];
+      x->view.lag_off[n] = t.lag_off[n];
+    }
+    r = x;
+  }
+  return *r;
+}
+
+struct ClassRange {
+  int off = 0, n = 0, l1 = 0, l2 = 0;
+};
+
+struct Plan {
+  const ig_basis *basis;  // shells are copied below; pointer only for sizes
+  int nsh, nao, nao_cart, natom;
+  std::vector<int> sh_l, sh_kind, sh_atom, sh_off_func, sh_off_cart, sh_nprim;
+  // device basis
+  DBuf d_xyz, d_l, d_kind, d_off_func, d_off_cart, d_atom, d_c2s;
+  BasisDev bdev{};
+  // pairs, final (class, Q-descending) order
+  int npairs = 0;
+  std::vector<int> p_sh1, p_sh2, p_canon, p_ppoff, p_ppn;
+  std::vector<double> p_q;
+  std::vector<int> canon_to_pos;  // canonical pair index -> sorted position
+  ClassRange cls[kNumPairClass];
+  DBuf d_sh1, d_sh2, d_canon, d_q, d_ppoff, d_ppn;
+  DBuf d_zeta, d_px, d_py, d_pz, d_kc, d_alpha, d_beta;
+  PairsDev pdev{};
+  std::vector<double> schwarz;  // nsh x nsh (function space)
+  std::mutex mu;                // serialises calls that use the scratch below
+  DBuf s_dfunc, s_ds, s_da, s_dc, s_bound, s_jc, s_kc, s_J, s_K, s_eff, s_rowptr, s_list,
+      s_counters, s_flag;
+  double cached_tau = NAN;
+  std::vector<long long> cached_rowptr_host;
+  struct CP {
+    int x, y;
+    size_t rp_off;
+    long long total;
+  };
+  std::vector<CP> cached_cps;
+  int eff_key = -1;
+  // per-launch attribution of the last call (counters on the device, events
+  // when profiling); finalised lazily on the stream the call used
+  static constexpr int kMaxLaunch = kNumPairClass * (kNumPairClass + 1) / 2;
+  struct Launch {
+    int la, lb, lc, ld;
+  };
+  std::vector<Launch> launches;
+  std::vector<unsigned long long> launch_ctr;  // 2 per launch, host copy
+  std::vector<float> launch_ms;
+  bool profiling = false;
+  std::vector<hipEvent_t> events;  // 2 per launch slot
+  bool stats_pending = false;
+  hipStream_t stats_stream = nullptr;
+  int64_t last_stats[2] = {0, 0};
+  ~Plan() {
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  }
+};
+
+void begin_launches(Plan &P, hipStream_t s) {
+  P.launches.clear();
+  P.s_counters.reserve(2 * Plan::kMaxLaunch * sizeof(unsigned long long));
+  check(hipMemsetAsync(P.s_counters.p, 0, 2 * Plan::kMaxLaunch * sizeof(unsigned long long), s),
+        "zero counters");
+  if (P.profiling && P.events.empty()) {
+    P.events.resize(2 * Plan::kMaxLaunch);
+    for (hipEvent_t &e : P.events) check(hipEventCreate(&e), "hipEventCreate");
+  }
+}
+
+unsigned long long *next_launch(Plan &P, const ClassRange &X, const ClassRange &Y, hipStream_t s) {
+  const int k = int(P.launches.size());
+  if (k >= Plan::kMaxLaunch) throw DeviceError{IG_ERR_DEVICE, "too many class launches"};
+  P.launches.push_back({X.l1, X.l2, Y.l1, Y.l2});
+  if (P.profiling) check(hipEventRecord(P.events[2 * k], s), "event");
+  return P.s_counters.as<unsigned long long>() + 2 * k;
+}
+
+void end_launch(Plan &P, hipStream_t s) {
+  if (P.profiling) check(hipEventRecord(P.events[2 * (P.launches.size() - 1) + 1], s), "event");
+}
+
+void finalize_stats(Plan &P) {
+  if (!P.stats_pending) return;
+  const size_t n = P.launches.size();
+  P.launch_ctr.assign(2 * std::max<size_t>(n, 1), 0);
+  if (n)
+    check(hipMemcpyAsync(P.launch_ctr.data(), P.s_counters.p, 2 * n * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, P.stats_stream),
+          "copy counters");
+  check(hipStreamSynchronize(P.stats_stream), "stats sync");
+  P.last_stats[0] = P.last_stats[1] = 0;
+  P.launch_ms.assign(n, 0.0f);
+  for (size_t k = 0; k < n; ++k) {
+    P.last_stats[0] += int64_t(P.launch_ctr[2 * k]);
+    P.last_stats[1] += int64_t(P.launch_ctr[2 * k + 1]);
+    if (P.profiling)
+      check(hipEventElapsedTime(&P.launch_ms[k], P.events[2 * k], P.events[2 * k + 1]),
+            "event time");
+  }
+  P.stats_pending = false;
+}
+
+void upload_basis(Plan &P, const ig_basis &b) {
+  P.nsh = int(b.shells.size());
+  P.nao = b.nao;
+  P.nao_cart = b.nao_cart;
+  P.natom = b.natom;
+  std::vector<double> xyz;
+  for (const HostShell &s : b.shells) {
+    P.sh_l.push_back(s.l);
+    P.sh_kind.push_back(s.kind);
+    P.sh_atom.push_back(s.atom);
+    P.sh_off_func.push_back(s.off_func);
+    P.sh_off_cart.push_back(s.off_cart);
+    P.sh_nprim.push_back(s.nprim);
+    xyz.insert(xyz.end(), s.center, s.center + 3);
+  }
+  std::vector<double> c2s;
+  for (int l = 0; l <= kMaxL; ++l) {
+    P.bdev.c2s_off[l] = int(c2s.size());
+    std::vector<double> m = c2s_scaled(l);
+    c2s.insert(c2s.end(), m.begin(), m.end());
+  }
+  P.d_xyz.upload(xyz);
+  P.d_l.upload(P.sh_l);
+  P.d_kind.upload(P.sh_kind);
+  P.d_off_func.upload(P.sh_off_func);
+  P.d_off_cart.upload(P.sh_off_cart);
+  P.d_atom.upload(P.sh_atom);
+  P.d_c2s.upload(c2s);
+  P.bdev.nsh = P.nsh;
+  P.bdev.nao = P.nao;
+  P.bdev.nao_cart = P.nao_cart;
+  P.bdev.xyz = P.d_xyz.as<double>();
+  P.bdev.l = P.d_l.as<int>();
+  P.bdev.kind = P.d_kind.as<int>();
+  P.bdev.off_func = P.d_off_func.as<int>();
+  P.bdev.off_cart = P.d_off_cart.as<int>();
+  P.bdev.atom = P.d_atom.as<int>();
+  P.bdev.c2s = P.d_c2s.as<double>();
+}
+
+void upload_pairs(Plan &P) {
+  P.d_sh1.upload(P.p_sh1);
+  P.d_sh2.upload(P.p_sh2);
+  P.d_canon.upload(P.p_canon);
+  P.d_q.upload(P.p_q);
+  P.d_ppoff.upload(P.p_ppoff);
+  P.d_ppn.upload(P.p_ppn);
+  P.pdev.n = P.npairs;
+  P.pdev.sh1 = P.d_sh1.as<int>();
+  P.pdev.sh2 = P.d_sh2.as<int>();
+  P.pdev.canon = P.d_canon.as<int>();
+  P.pdev.q = P.d_q.as<double>();
+  P.pdev.pp_off = P.d_ppoff.as<int>();
+  P.pdev.pp_n = P.d_ppn.as<int>();
+}
+
+// Build the plan: pairs in kernel orientation, primitive-pair SoA tables,
+// Schwarz bounds on the device, then the final (class, Q desc) order.
+std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
+  auto P = std::make_unique<Plan>();
+  P->basis = &b;
+  upload_basis(*P, b);
+  const int nsh = P->nsh;
+  const int64_t np = pair_count(nsh);
+  P->npairs = int(np);
+  // primitive pairs (os_eri.rs:221-249 quantities; K c1 c2 folded together)
+  std::vector<double> zeta, px, py, pz, kc, al, be;
+  struct Tmp {
+    int sh1, sh2, canon, off, n, cls;
+  };
+  std::vector<Tmp> tmp(np);
+  for (int64_t c = 0; c < np; ++c) {
+    int i, j;
+    pair_of(c, i, j);
+    const HostShell *s1 = &b.shells[i], *s2 = &b.shells[j];
+    int a1 = i, a2 = j;
+    if (s1->l < s2->l) {
+      std::swap(s1, s2);
+      std::swap(a1, a2);
+    }
+    Tmp t{a1, a2, int(c), int(zeta.size()), 0, pair_class(s1->l, s2->l)};
+    double d2 = 0.0;
+    for (int x = 0; x < 3; ++x) d2 += (s1->center[x] - s2->center[x]) * (s1->center[x] - s2->center[x]);
+    for (int p1 = 0; p1 < s1->nprim; ++p1)
+      for (int p2 = 0; p2 < s2->nprim; ++p2) {
+        double e1 = s1->exps[p1], e2 = s2->exps[p2], z = e1 + e2;
+        double K = std::exp(-(e1 * e2 / z) * d2);
+        double w = K * s1->eff[p1] * s2->eff[p2];
+        if (K * std::fabs(s1->eff[p1] * s2->eff[p2]) < 1e-32) continue;  // os_eri.rs:211
+        zeta.push_back(z);
+        px.push_back((e1 * s1->center[0] + e2 * s2->center[0]) / z);
+        py.push_back((e1 * s1->center[1] + e2 * s2->center[1]) / z);
+        pz.push_back((e1 * s1->center[2] + e2 * s2->center[2]) / z);
+        kc.push_back(w);
+        al.push_back(e1);
+        be.push_back(e2);
+        ++t.n;
+      }
+    tmp[c] = t;
+  }
+  P->d_zeta.upload(zeta);
+  P->d_px.upload(px);
+  P->d_py.upload(py);
+  P->d_pz.upload(pz);
+  P->d_kc.upload(kc);
+  P->d_alpha.upload(al);
+  P->d_beta.upload(be);
+  P->pdev.zeta = P->d_zeta.as<double>();
+  P->pdev.px = P->d_px.as<double>();
+  P->pdev.py = P->d_py.as<double>();
+  P->pdev.pz = P->d_pz.as<double>();
+  P->pdev.kc = P->d_kc.as<double>();
+  P->pdev.alpha = P->d_alpha.as<double>();
+  P->pdev.beta = P->d_beta.as<double>();
+  // class order (stable by canonical index)
+  std::vector<int> order(np);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return tmp[x].cls < tmp[y].cls; });
+  auto fill = [&](const std::vector<int> &ord) {
+    P->p_sh1.resize(np);
+    P->p_sh2.resize(np);
+    P->p_canon.resize(np);
+    P->p_ppoff.resize(np);
+    P->p_ppn.resize(np);
+    for (int64_t k = 0; k < np; ++k) {
+      const Tmp &t = tmp[ord[k]];
+      P->p_sh1[k] = t.sh1;
+      P->p_sh2[k] = t.sh2;
+      P->p_canon[k] = t.canon;
+      P->p_ppoff[k] = t.off;
+      P->p_ppn[k] = t.n;
+    }
+  };
+  fill(order);
+  for (int c = 0; c < kNumPairClass; ++c) P->cls[c] = ClassRange{};
+  for (int64_t k = 0; k < np; ++k) {
+    int c = tmp[order[k]].cls;
+    if (P->cls[c].n == 0) P->cls[c].off = int(k);
+    P->cls[c].n++;
+  }
+  for (int l1 = 0; l1 <= kMaxL; ++l1)
+    for (int l2 = 0; l2 <= l1; ++l2) {
+      ClassRange &r = P->cls[pair_class(l1, l2)];
+      r.l1 = l1;
+      r.l2 = l2;
+      if (r.n && !class_supported(l1, l2, l1, l2))
+        throw DeviceError{IG_ERR_UNSUPPORTED,
+                          "shell pairs with l=(" + std::to_string(l1) + "," + std::to_string(l2) +
+                              ") have no compiled quartet kernel in this build (max l = " +
+                              std::to_string(kMaxKernelL) + ")"};
+    }
+  P->p_q.assign(np, 0.0);
+  upload_pairs(*P);
+  // Schwarz on the device, per pair class
+  DBuf dq;
+  dq.reserve(np * sizeof(double));
+  RysDevice &R = rys_device();
+  for (int c = 0; c < kNumPairClass; ++c) {
+    const ClassRange &r = P->cls[c];
+    if (!r.n) continue;
+    check(launch_schwarz(r.l1, r.l2, P->bdev, P->pdev, R.view, r.off, r.n, dq.as<double>(), s),
+          "schwarz launch");
+  }
+  std::vector<double> q(np);
+  check(hipMemcpyAsync(q.data(), dq.p, np * sizeof(double), hipMemcpyDeviceToHost, s), "copy Q");
+  check(hipStreamSynchronize(s), "schwarz sync");
+  // final order: class, then Q descending (stable)
+  std::vector<int> order2(np);
+  std::iota(order2.begin(), order2.end(), 0);
+  std::stable_sort(order2.begin(), order2.end(), [&](int x, int y) {
+    int cx = tmp[order[x]].cls, cy = tmp[order[y]].cls;
+    if (cx != cy) return cx < cy;
+    return q[x] > q[y];
+  });
+  std::vector<int> final_order(np);
+  std::vector<double> qs(np);
+  for (int64_t k = 0; k < np; ++k) {
+    final_order[k] = order[order2[k]];
+    qs[k] = q[order2[k]];
+  }
+  fill(final_order);
+  P->p_q = qs;
+  P->canon_to_pos.assign(np, 0);
+  for (int64_t k = 0; k < np; ++k) P->canon_to_pos[P->p_canon[k]] = int(k);
+  upload_pairs(*P);
+  // nsh x nsh Schwarz matrix (symmetric; canonical orientation computed)
+  P->schwarz.assign(size_t(nsh) * nsh, 0.0);
+  for (int64_t k = 0; k < np; ++k) {
+    int i, j;
+    pair_of(P->p_canon[k], i, j);
+    P->schwarz[size_t(i) * nsh + j] = P->p_q[k];
+    P->schwarz[size_t(j) * nsh + i] = P->p_q[k];
+  }
+  return P;
+}
+
+// Number of kets of class y (Q-sorted) with qb*q >= tau; binary search on the
+// same floating-point predicate the reference evaluates (direct.rs:172-175).
+int count_kets(const double *qy, int ny, double qb, double tau) {
+  int lo = 0, hi = ny;  // first index where qb*q < tau
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (qb * qy[mid] < tau)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// implicit quartet sets for every (X >= Y) class pair at threshold tau;
+// pairs with Q == 0 contribute nothing (their integrals vanish) and are skipped.
+void build_rowptrs(Plan &P, double tau) {
+  if (P.cached_tau == tau && !P.cached_cps.empty()) return;
+  P.cached_cps.clear();
+  P.cached_rowptr_host.clear();
+  for (int x = 0; x < kNumPairClass; ++x) {
+    const ClassRange &X = P.cls[x];
+    if (!X.n) continue;
+    for (int y = 0; y <= x; ++y) {
+      const ClassRange &Y = P.cls[y];
+      if (!Y.n) continue;
+      Plan::CP cp{x, y, P.cached_rowptr_host.size(), 0};
+      long long acc = 0;
+      P.cached_rowptr_host.push_back(0);
+      const double *qy = P.p_q.data() + Y.off;
+      for (int r = 0; r < X.n; ++r) {
+        double qb = P.p_q[X.off + r];
+        int cnt = 0;
+        if (qb != 0.0) {
+          cnt = count_kets(qy, Y.n, qb, tau);
+          if (x == y) cnt = std::min(cnt, r + 1);
+          while (cnt > 0 && qy[cnt - 1] == 0.0) --cnt;  // Q == 0 kets
+        }
+        acc += cnt;
+        P.cached_rowptr_host.push_back(acc);
+      }
+      cp.total = acc;
+      P.cached_cps.push_back(cp);
+    }
+  }
+  P.s_rowptr.upload(P.cached_rowptr_host);
+  P.cached_tau = tau;
+}
+
+QuartetSet implicit_set(Plan &P, const Plan::CP &cp) {
+  QuartetSet qs{};
+  qs.x_off = P.cls[cp.x].off;
+  qs.x_n = P.cls[cp.x].n;
+  qs.y_off = P.cls[cp.y].off;
+  qs.y_n = P.cls[cp.y].n;
+  qs.same = cp.x == cp.y;
+  qs.rowptr = P.s_rowptr.as<long long>() + cp.rp_off;
+  qs.list = nullptr;
+  qs.total = cp.total;
+  return qs;
+}
+
+// explicit lists for the canonical bra pairs [b0, b1): kets k <= i,
+// l <= (k == i ? j : k) passing Q_ij Q_kl >= tau (direct.rs:169-175)
+struct ExplicitSets {
+  std::vector<std::pair<int, int>> cls;  // (x, y)
+  std::vector<std::vector<int2>> lists;
+};
+
+ExplicitSets explicit_sets(const Plan &P, int64_t b0, int64_t b1, double tau) {
+  ExplicitSets out;
+  int index[kNumPairClass][kNumPairClass];
+  for (auto &r : index)
+    for (int &v : r) v = -1;
+  const int nsh = P.nsh;
+  for (int64_t bc = b0; bc < b1; ++bc) {
+    int i, j;
+    pair_of(bc, i, j);
+    double qab = P.schwarz[size_t(i) * nsh + j];
+    if (qab == 0.0) continue;  // direct.rs:165-168
+    int pb = P.canon_to_pos[bc];
+    for (int k = 0; k <= i; ++k) {
+      int ltop = (k == i) ? j : k;
+      for (int l = 0; l <= ltop; ++l) {
+        double qkl = P.schwarz[size_t(k) * nsh + l];
+        if (qab * qkl < tau || qkl == 0.0) continue;
+        int pk = P.canon_to_pos[int64_t(k) * (k + 1) / 2 + l];
+        int cb = pair_class(P.sh_l[P.p_sh1[pb]], P.sh_l[P.p_sh2[pb]]);
+        int ck = pair_class(P.sh_l[P.p_sh1[pk]], P.sh_l[P.p_sh2[pk]]);
+        int x = cb, y = ck, px = pb, py = pk;
+        if (cb < ck) {
+          std::swap(x, y);
+          std::swap(px, py);
+        }
+        if (index[x][y] < 0) {
+          index[x][y] = int(out.cls.size());
+          out.cls.push_back({x, y});
+          out.lists.emplace_back();
+        }
+        out.lists[index[x][y]].push_back(int2{px - P.cls[x].off, py - P.cls[y].off});
+      }
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// public plan objects
+struct ig_direct {
+  std::unique_ptr<Plan> plan;
+  int engine;
+};
+
+struct ig_grad {
+  std::unique_ptr<Plan> plan;
+  int engine;
+};
+
+struct ig_grad_density {
+  const ig_grad *owner;
+  int ncoul = 0, nexch = 0;
+  DBuf d_fac, d_mats, d_bmax;
+  std::vector<double> fac;
+};
+
+namespace {
+
+int check_engine(int engine) {
+  if (engine != IG_ENGINE_AUTO && engine != IG_ENGINE_OS_HGP && engine != IG_ENGINE_RYS)
+    return fail(IG_ERR_INVALID_ARGUMENT, "unknown engine");
+  return IG_OK;
+}
+
+hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+// Run J/K for ndens densities already on the device (d_dens, function space).
+void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau, bool want_k,
+            double *d_J, double *d_K, const ExplicitSets *ex, hipStream_t s) {
+  const size_t n2 = size_t(P.nao) * P.nao, nc2 = size_t(P.nao_cart) * P.nao_cart;
+  const int neff = ndens * (has_anti ? 2 : 1);
+  P.s_ds.reserve(n2 * ndens * sizeof(double));
+  P.s_da.reserve(n2 * ndens * sizeof(double) * (has_anti ? 1 : 0) + 8);
+  P.s_dc.reserve(nc2 * neff * sizeof(double));
+  P.s_bound.reserve(size_t(P.nsh) * P.nsh * ndens * sizeof(double));
+  P.s_jc.reserve(nc2 * neff * sizeof(double));
+  if (want_k) P.s_kc.reserve(nc2 * neff * sizeof(double));
+  if (P.eff_key != neff * 128 + ndens) {  // (source, antisymmetric) per effective density
+    std::vector<int> eff(2 * neff);
+    for (int e = 0; e < neff; ++e) {
+      eff[e] = e % ndens;
+      eff[neff + e] = e >= ndens;
+    }
+    P.s_eff.upload(eff);
+    P.eff_key = neff * 128 + ndens;
+  }
+  begin_launches(P, s);
+  check(launch_density_prep(P.bdev, d_dens, ndens, has_anti, P.s_ds.as<double>(),
+                            P.s_da.as<double>(), P.s_dc.as<double>(), P.s_bound.as<double>(), s),
+        "density prep");
+  check(launch_fill(P.s_jc.as<double>(), nc2 * neff, 0.0, s), "zero J");
+  if (want_k) check(launch_fill(P.s_kc.as<double>(), nc2 * neff, 0.0, s), "zero K");
+  DensDev dd{};
+  dd.ndens = ndens;
+  dd.neff = neff;
+  dd.dc = P.s_dc.as<double>();
+  dd.jc = P.s_jc.as<double>();
+  dd.kc = want_k ? P.s_kc.as<double>() : nullptr;
+  dd.eff_src = P.s_eff.as<int>();
+  dd.eff_anti = P.s_eff.as<int>() + neff;
+  dd.bound = P.s_bound.as<double>();
+  dd.tau = tau;
+  dd.want_k = want_k ? 1 : 0;
+  RysDevice &R = rys_device();
+  if (!ex) {
+    build_rowptrs(P, tau);
+    // heavy classes first so the long launches start early
+    for (auto it = P.cached_cps.rbegin(); it != P.cached_cps.rend(); ++it) {
+      if (!it->total) continue;
+      QuartetSet qs = implicit_set(P, *it);
+      const ClassRange &X = P.cls[it->x], &Y = P.cls[it->y];
+      unsigned long long *ctr = next_launch(P, X, Y, s);
+      check(launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, ctr, s),
+            "jk launch");
+      end_launch(P, s);
+    }
+  } else {
+    size_t total = 0;
+    for (auto &l : ex->lists) total += l.size();
+    std::vector<int2> flat;
+    flat.reserve(total);
+    for (auto &l : ex->lists) flat.insert(flat.end(), l.begin(), l.end());
+    P.s_list.upload(flat);
+    size_t off = 0;
+    for (size_t c = 0; c < ex->cls.size(); ++c) {
+      const ClassRange &X = P.cls[ex->cls[c].first], &Y = P.cls[ex->cls[c].second];
+      QuartetSet qs{};
+      qs.x_off = X.off;
+      qs.x_n = X.n;
+      qs.y_off = Y.off;
+      qs.y_n = Y.n;
+      qs.same = ex->cls[c].first == ex->cls[c].second;
+      qs.list = P.s_list.as<int2>() + off;
+      qs.total = (long long)ex->lists[c].size();
+      off += ex->lists[c].size();
+      unsigned long long *ctr = next_launch(P, X, Y, s);
+      check(launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, ctr, s),
+            "jk launch");
+      end_launch(P, s);
+    }
+  }
+  check(launch_jk_finish(P.bdev, dd, d_J, d_K, s), "jk finish");
+  P.stats_stream = s;
+  P.stats_pending = true;
+}
+
+bool host_symmetric(const double *d, int n) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j)
+      if (d[size_t(i) * n + j] != d[size_t(j) * n + i]) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" int ig_direct_create(const ig_basis *basis, int engine, ig_direct **out) {
+  if (!basis || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (int rc = check_engine(engine)) return rc;
+  return guarded([&] {
+    auto d = std::make_unique<ig_direct>();
+    d->engine = engine;
+    d->plan = build_plan(*basis, nullptr);
+    *out = d.release();
+    return int(IG_OK);
+  });
+}
+
+extern "C" void ig_direct_destroy(ig_direct *d) { delete d; }
+extern "C" size_t ig_direct_nao(const ig_direct *d) { return d ? size_t(d->plan->nao) : 0; }
+extern "C" size_t ig_direct_n_bra_pairs(const ig_direct *d) {
+  return d ? size_t(d->plan->npairs) : 0;
+}
+extern "C" int ig_direct_bra_pairs(const ig_direct *d, int32_t *pairs, size_t n) {
+  if (!d || (!pairs && n)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (n != size_t(d->plan->npairs)) return fail(IG_ERR_LENGTH_MISMATCH, "pairs length");
+  for (size_t c = 0; c < n; ++c) {
+    int i, j;
+    pair_of(int64_t(c), i, j);
+    pairs[2 * c] = i;
+    pairs[2 * c + 1] = j;
+  }
+  return IG_OK;
+}
+
+extern "C" int ig_direct_density_shell_bounds(const ig_direct *d, const double *const *dens,
+                                              size_t ndens, double *bounds, size_t len) {
+  if (!d || !bounds || (ndens && !dens)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  const Plan &P = *d->plan;
+  if (len != ndens * size_t(P.nsh) * P.nsh) return fail(IG_ERR_LENGTH_MISMATCH, "bounds length");
+  // direct.rs:103-125 (ordered blocks, host computation)
+  for (size_t k = 0; k < ndens; ++k) {
+    if (!dens[k]) return fail(IG_ERR_INVALID_ARGUMENT, "null density");
+    for (int i = 0; i < P.nsh; ++i)
+      for (int j = 0; j < P.nsh; ++j) {
+        int ni = P.sh_kind[i] ? 2 * P.sh_l[i] + 1 : ncart(P.sh_l[i]);
+        int nj = P.sh_kind[j] ? 2 * P.sh_l[j] + 1 : ncart(P.sh_l[j]);
+        double m = 0.0;
+        for (int a = 0; a < ni; ++a)
+          for (int c = 0; c < nj; ++c)
+            m = std::max(m, std::fabs(dens[k][size_t(P.sh_off_func[i] + a) * P.nao +
+                                               P.sh_off_func[j] + c]));
+        bounds[(k * P.nsh + i) * P.nsh + j] = m;
+      }
+  }
+  return IG_OK;
+}
+
+static int jk_host(const ig_direct *d, const double *const *dens, size_t ndens, double tau,
+                   int want_k, double *const *J, double *const *K, const ExplicitSets *ex,
+                   void *stream) {
+  Plan &P = *d->plan;
+  std::lock_guard<std::mutex> g(P.mu);
+  const size_t n2 = size_t(P.nao) * P.nao;
+  hipStream_t s = as_stream(stream);
+  std::vector<double> host(n2 * ndens);
+  bool sym = true;
+  for (size_t k = 0; k < ndens; ++k) {
+    std::memcpy(host.data() + k * n2, dens[k], n2 * sizeof(double));
+    sym = sym && host_symmetric(dens[k], P.nao);
+  }
+  P.s_dfunc.upload(host);
+  P.s_J.reserve(n2 * ndens * sizeof(double));
+  P.s_K.reserve(n2 * ndens * sizeof(double));
+  check(launch_fill(P.s_J.as<double>(), n2 * ndens, 0.0, s), "zero J");
+  if (want_k) check(launch_fill(P.s_K.as<double>(), n2 * ndens, 0.0, s), "zero K");
+  run_jk(P, P.s_dfunc.as<double>(), int(ndens), !sym, tau, want_k != 0, P.s_J.as<double>(),
+         want_k ? P.s_K.as<double>() : nullptr, ex, s);
+  std::vector<double> hj(n2 * ndens), hk(want_k ? n2 * ndens : 0);
+  check(hipMemcpyAsync(hj.data(), P.s_J.p, hj.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+        "copy J");
+  if (want_k)
+    check(hipMemcpyAsync(hk.data(), P.s_K.p, hk.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+          "copy K");
+  finalize_stats(P);  // synchronises the stream
+  for (size_t k = 0; k < ndens; ++k) {
+    for (size_t x = 0; x < n2; ++x) J[k][x] += hj[k * n2 + x];
+    if (want_k)
+      for (size_t x = 0; x < n2; ++x) K[k][x] += hk[k * n2 + x];
+  }
+  return int(IG_OK);
+}
+
+static int check_dens_args(const ig_direct *d, const double *const *dens, size_t ndens,
+                           int want_k, double *const *J, double *const *K) {
+  if (!d) return fail(IG_ERR_INVALID_ARGUMENT, "null contractor");
+  if (ndens == 0) return fail(IG_ERR_NO_DENSITY, "direct contraction needs at least one density");
+  if (ndens > size_t(kMaxDens))
+    return fail(IG_ERR_TOO_MANY_DENSITIES, "at most 64 simultaneous densities are supported");
+  if (!dens || !J || (want_k && !K)) return fail(IG_ERR_INVALID_ARGUMENT, "null buffer list");
+  for (size_t k = 0; k < ndens; ++k)
+    if (!dens[k] || !J[k] || (want_k && !K[k]))
+      return fail(IG_ERR_INVALID_ARGUMENT, "null buffer");
+  return IG_OK;
+}
+
+extern "C" int ig_direct_accumulate_bra_pair(const ig_direct *d, int32_t i, int32_t j,
+                                             const double *const *dens, const double *bounds,
+                                             size_t ndens, double threshold, int want_k,
+                                             double *const *J, double *const *K, void *stream) {
+  if (int rc = check_dens_args(d, dens, ndens, want_k, J, K)) return rc;
+  const Plan &P = *d->plan;
+  if (!(j >= 0 && j <= i && i < P.nsh))
+    return fail(IG_ERR_NON_CANONICAL_BRA, "bra pair is not canonical (i >= j)");
+  (void)bounds;  // the device recomputes the same bounds from the densities
+  return guarded([&] {
+    int64_t bc = int64_t(i) * (i + 1) / 2 + j;
+    ExplicitSets ex = explicit_sets(P, bc, bc + 1, threshold);
+    return jk_host(d, dens, ndens, threshold, want_k, J, K, &ex, stream);
+  });
+}
+
+extern "C" int ig_direct_jk(const ig_direct *d, const double *const *dens, size_t ndens,
+                            double threshold, int want_k, double *const *J, double *const *K,
+                            int64_t b0, int64_t b1, void *stream) {
+  if (int rc = check_dens_args(d, dens, ndens, want_k, J, K)) return rc;
+  const Plan &P = *d->plan;
+  const int64_t np = P.npairs;
+  if (b1 < 0 || b1 > np) b1 = np;
+  if (b0 < 0 || b0 > b1) return fail(IG_ERR_INVALID_ARGUMENT, "bad bra range");
+  return guarded([&] {
+    if (b0 == 0 && b1 == np) return jk_host(d, dens, ndens, threshold, want_k, J, K, nullptr, stream);
+    ExplicitSets ex = explicit_sets(P, b0, b1, threshold);
+    return jk_host(d, dens, ndens, threshold, want_k, J, K, &ex, stream);
+  });
+}
+
+extern "C" int ig_direct_jk_device(const ig_direct *d, const double *d_dens, size_t ndens,
+                                   double threshold, int want_k, double *d_J, double *d_K,
+                                   void *stream) {
+  if (!d || !d_dens || !d_J || (want_k && !d_K)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (ndens == 0) return fail(IG_ERR_NO_DENSITY, "direct contraction needs at least one density");
+  if (ndens > size_t(kMaxDens))
+    return fail(IG_ERR_TOO_MANY_DENSITIES, "at most 64 simultaneous densities are supported");
+  return guarded([&] {
+    Plan &P = *d->plan;
+    std::lock_guard<std::mutex> g(P.mu);
+    hipStream_t s = as_stream(stream);
+    // exact symmetry test on the device; only a 4-byte flag comes back
+    P.s_flag.reserve(sizeof(int));
+    check(launch_asym_flag(d_dens, P.nao, int(ndens), P.s_flag.as<int>(), s), "symmetry check");
+    int asym = 0;
+    check(hipMemcpyAsync(&asym, P.s_flag.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flag");
+    check(hipStreamSynchronize(s), "sync");
+    run_jk(P, d_dens, int(ndens), asym != 0, threshold, want_k != 0, d_J, want_k ? d_K : nullptr,
+           nullptr, s);
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_direct_set_profiling(ig_direct *d, int enable) {
+  if (!d) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> g(d->plan->mu);
+  d->plan->profiling = enable != 0;
+  return IG_OK;
+}
+
+extern "C" int ig_direct_profile_read(const ig_direct *d, ig_class_profile *out, size_t cap,
+                                      size_t *n) {
+  if (!d || !n || (cap && !out)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  return guarded([&] {
+    Plan &P = *d->plan;
+    std::lock_guard<std::mutex> g(P.mu);
+    finalize_stats(P);
+    *n = P.launches.size();
+    for (size_t k = 0; k < P.launches.size() && k < cap; ++k) {
+      out[k].la = P.launches[k].la;
+      out[k].lb = P.launches[k].lb;
+      out[k].lc = P.launches[k].lc;
+      out[k].ld = P.launches[k].ld;
+      out[k].quartets = int64_t(P.launch_ctr[2 * k]);
+      out[k].prim_quartets = int64_t(P.launch_ctr[2 * k + 1]);
+      out[k].ms = P.profiling && k < P.launch_ms.size() ? double(P.launch_ms[k]) : 0.0;
+    }
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_direct_last_stats(const ig_direct *d, int64_t *st) {
+  if (!d || !st) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = guarded([&] {
+        std::lock_guard<std::mutex> g(d->plan->mu);
+        finalize_stats(*d->plan);
+        return int(IG_OK);
+      }))
+    return rc;
+  st[0] = d->plan->last_stats[0];
+  st[1] = d->plan->last_stats[1];
+  return IG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// dense tensor and Schwarz
+
+extern "C" int ig_schwarz_bounds(const ig_basis *b, int engine, double *out, size_t len,
+                                 void *stream) {
+  if (!b || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  if (len != b->shells.size() * b->shells.size())
+    return fail(IG_ERR_LENGTH_MISMATCH, "schwarz length");
+  return guarded([&] {
+    auto P = build_plan(*b, as_stream(stream));
+    std::copy(P->schwarz.begin(), P->schwarz.end(), out);
+    return int(IG_OK);
+  });
+}
+
+static int eri_impl(const ig_basis *b, double tau, double *out, size_t len, int64_t *stats,
+                    void *stream) {
+  const size_t nao = size_t(b->nao);
+  if (len != nao * nao * nao * nao) return fail(IG_ERR_LENGTH_MISMATCH, "eri length");
+  return guarded([&] {
+    hipStream_t s = as_stream(stream);
+    auto P = build_plan(*b, s);
+    DBuf dout;
+    dout.reserve(len * sizeof(double));
+    check(launch_fill(dout.as<double>(), len, 0.0, s), "zero eri");
+    build_rowptrs(*P, tau < 0 ? 0.0 : tau);
+    RysDevice &R = rys_device();
+    for (auto &cp : P->cached_cps) {
+      if (!cp.total) continue;
+      QuartetSet qs = implicit_set(*P, cp);
+      const ClassRange &X = P->cls[cp.x], &Y = P->cls[cp.y];
+      check(launch_eri(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, dout.as<double>(), s),
+            "eri launch");
+    }
+    check(hipMemcpyAsync(out, dout.p, len * sizeof(double), hipMemcpyDeviceToHost, s), "copy eri");
+    check(hipStreamSynchronize(s), "eri sync");
+    if (stats) {  // integrals.rs:985-1005: canonical quartets and Schwarz skips
+      int64_t np = P->npairs, total = np * (np + 1) / 2, kept = 0;
+      std::vector<double> q = P->p_q;
+      std::sort(q.begin(), q.end(), std::greater<double>());
+      for (int64_t r = 0; r < np; ++r) {
+        int c = count_kets(q.data(), int(np), q[r], tau);
+        kept += std::min<int64_t>(c, r + 1);
+      }
+      stats[0] = total;
+      stats[1] = total - kept;
+    }
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_eri_dense(const ig_basis *b, int engine, double *out, size_t len, void *stream) {
+  if (!b || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  return eri_impl(b, 0.0, out, len, nullptr, stream);
+}
+
+extern "C" int ig_eri_screened(const ig_basis *b, int engine, double tau, double *out, size_t len,
+                               int64_t *stats, void *stream) {
+  if (!b || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  return eri_impl(b, tau, out, len, stats, stream);
+}
+
+// ---------------------------------------------------------------------------
+// gradient (plan objects; contraction kernels in ig_kernels.hip)
+
+extern "C" int ig_grad_create(const ig_basis *basis, int engine, ig_grad **out) {
+  if (!basis || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (int rc = check_engine(engine)) return rc;
+  for (size_t s = 0; s < basis->shells.size(); ++s)
+    if (basis->shells[s].l > kMaxGradL)
+      return fail(IG_ERR_L_TOO_HIGH_FOR_GRADIENT,
+                  "shell " + std::to_string(s) + ": l=" + std::to_string(basis->shells[s].l) +
+                      " exceeds the gradient maximum 5");
+  return guarded([&] {
+    auto g = std::make_unique<ig_grad>();
+    g->engine = engine;
+    g->plan = build_plan(*basis, nullptr);
+    *out = g.release();
+    return int(IG_OK);
+  });
+}
+extern "C" void ig_grad_destroy(ig_grad *g) { delete g; }
+extern "C" size_t ig_grad_natom(const ig_grad *g) { return g ? size_t(g->plan->natom) : 0; }
+extern "C" size_t ig_grad_nao(const ig_grad *g) { return g ? size_t(g->plan->nao) : 0; }
+
+extern "C" int ig_grad_density_create(const ig_grad *g, size_t nc, const double *cf,
+                                      const double *const *cl, const double *const *cr,
+                                      size_t nx, const double *xf, const double *const *xl,
+                                      const double *const *xr, ig_grad_density **out) {
+  if (!g || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if ((nc && (!cf || !cl || !cr)) || (nx && (!xf || !xl || !xr)))
+    return fail(IG_ERR_INVALID_ARGUMENT, "null term list");
+  for (size_t t = 0; t < nc; ++t)
+    if (!cl[t] || !cr[t]) return fail(IG_ERR_INVALID_ARGUMENT, "null matrix");
+  for (size_t t = 0; t < nx; ++t)
+    if (!xl[t] || !xr[t]) return fail(IG_ERR_INVALID_ARGUMENT, "null matrix");
+  return guarded([&] {
+    Plan &P = *g->plan;
+    auto d = std::make_unique<ig_grad_density>();
+    d->owner = g;
+    d->ncoul = int(nc);
+    d->nexch = int(nx);
+    const size_t n2 = size_t(P.nao) * P.nao, nc2 = size_t(P.nao_cart) * P.nao_cart;
+    const size_t nterm = nc + nx, nm = 2 * nterm;
+    std::vector<double> host(std::max<size_t>(nm, 1) * n2, 0.0);
+    for (size_t t = 0; t < nterm; ++t) {
+      const double *L = t < nc ? cl[t] : xl[t - nc];
+      const double *R = t < nc ? cr[t] : xr[t - nc];
+      std::memcpy(host.data() + (2 * t) * n2, L, n2 * sizeof(double));
+      std::memcpy(host.data() + (2 * t + 1) * n2, R, n2 * sizeof(double));
+      d->fac.push_back(t < nc ? cf[t] : xf[t - nc]);
+    }
+    if (d->fac.empty()) d->fac.push_back(0.0);
+    DBuf func;
+    func.upload(host);
+    d->d_mats.reserve(std::max<size_t>(nm, 1) * nc2 * sizeof(double));
+    d->d_bmax.reserve(std::max<size_t>(nm, 1) * size_t(P.nsh) * P.nsh * sizeof(double));
+    if (nm) {
+      check(launch_to_cart(P.bdev, func.as<double>(), int(nm), d->d_mats.as<double>(), nullptr),
+            "density to Cartesian");
+      check(launch_block_max(P.bdev, func.as<double>(), int(nm), d->d_bmax.as<double>(), nullptr),
+            "density block maxima");
+    }
+    d->d_fac.upload(d->fac);
+    check(hipDeviceSynchronize(), "density sync");
+    *out = d.release();
+    return int(IG_OK);
+  });
+}
+
+extern "C" void ig_grad_density_destroy(ig_grad_density *d) { delete d; }
+
+static int grad_run(const ig_grad *g, const ig_grad_density *dens, int kernel, double omega,
+                    double tau, double *out, size_t len, int64_t b0, int64_t b1, void *stream) {
+  if (!g || !dens || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  Plan &P = *g->plan;
+  if (dens->owner != g)
+    return fail(IG_ERR_FOREIGN_DENSITY,
+                "density was prepared by a plan with a different shell layout");
+  if (len != size_t(P.natom) * 3) return fail(IG_ERR_LENGTH_MISMATCH, "out must hold natom*3");
+  if (kernel != IG_KERNEL_COULOMB && kernel != IG_KERNEL_ERF)
+    return fail(IG_ERR_INVALID_ARGUMENT, "unknown kernel");
+  if (kernel == IG_KERNEL_ERF && !(std::isfinite(omega) && omega > 0.0))
+    return fail(IG_ERR_INVALID_OMEGA, "EriKernel::Erf requires a finite omega > 0");
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipStream_t s = as_stream(stream);
+    DBuf dout;
+    dout.reserve(len * sizeof(double));
+    check(launch_fill(dout.as<double>(), len, 0.0, s), "zero grad");
+    P.s_counters.reserve(2 * sizeof(unsigned long long));
+    check(hipMemsetAsync(P.s_counters.p, 0, 2 * sizeof(unsigned long long), s), "zero counters");
+    GradDev gd{};
+    gd.ncoul = dens->ncoul;
+    gd.nexch = dens->nexch;
+    gd.fac = dens->d_fac.as<double>();
+    gd.mats = dens->d_mats.as<double>();
+    gd.bmax = dens->d_bmax.as<double>();
+    gd.out = dout.as<double>();
+    gd.tau = tau;
+    gd.omega = kernel == IG_KERNEL_ERF ? omega : 0.0;
+    RysDevice &R = rys_device();
+    auto *ctr = P.s_counters.as<unsigned long long>();
+    const int64_t np = P.npairs;
+    if (gd.ncoul + gd.nexch > 0) {
+      if (b0 == 0 && b1 == np) {
+        build_rowptrs(P, tau);
+        for (auto it = P.cached_cps.rbegin(); it != P.cached_cps.rend(); ++it) {
+          if (!it->total) continue;
+          QuartetSet qs = implicit_set(P, *it);
+          const ClassRange &X = P.cls[it->x], &Y = P.cls[it->y];
+          check(launch_grad(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd, ctr, s),
+                "grad launch");
+        }
+      } else {
+        ExplicitSets ex = explicit_sets(P, b0, b1, tau);
+        std::vector<int2> flat;
+        for (auto &l : ex.lists) flat.insert(flat.end(), l.begin(), l.end());
+        P.s_list.upload(flat);
+        size_t off = 0;
+        for (size_t c = 0; c < ex.cls.size(); ++c) {
+          const ClassRange &X = P.cls[ex.cls[c].first], &Y = P.cls[ex.cls[c].second];
+          QuartetSet qs{};
+          qs.x_off = X.off;
+          qs.x_n = X.n;
+          qs.y_off = Y.off;
+          qs.y_n = Y.n;
+          qs.same = ex.cls[c].first == ex.cls[c].second;
+          qs.list = P.s_list.as<int2>() + off;
+          qs.total = (long long)ex.lists[c].size();
+          off += ex.lists[c].size();
+          check(launch_grad(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd, ctr, s),
+                "grad launch");
+        }
+      }
+    }
+    std::vector<double> h(len);
+    unsigned long long c2[2];
+    check(hipMemcpyAsync(h.data(), dout.p, len * sizeof(double), hipMemcpyDeviceToHost, s),
+          "copy grad");
+    check(hipMemcpyAsync(c2, P.s_counters.p, sizeof(c2), hipMemcpyDeviceToHost, s), "copy ctr");
+    check(hipStreamSynchronize(s), "grad sync");
+    P.last_stats[0] = int64_t(c2[0]);
+    P.last_stats[1] = int64_t(c2[1]);
+    for (size_t x = 0; x < len; ++x) out[x] += h[x];
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_grad_accumulate_bra_pair(const ig_grad *g, const ig_grad_density *dens,
+                                           int32_t i, int32_t j, int kernel, double omega,
+                                           double threshold, double *out, size_t len,
+                                           void *stream) {
+  if (!g) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (!(j >= 0 && j <= i && i < g->plan->nsh))
+    return fail(IG_ERR_NON_CANONICAL_BRA, "bra pair is not canonical (i >= j)");
+  int64_t bc = int64_t(i) * (i + 1) / 2 + j;
+  return grad_run(g, dens, kernel, omega, threshold, out, len, bc, bc + 1, stream);
+}
+
+extern "C" int ig_grad_contract(const ig_grad *g, const ig_grad_density *dens, int kernel,
+                                double omega, double threshold, double *out, size_t len,
+                                void *stream) {
+  if (!g) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  return grad_run(g, dens, kernel, omega, threshold, out, len, 0, g->plan->npairs, stream);
+}
+
+extern "C" int ig_grad_last_stats(const ig_grad *g, int64_t *st) {
+  if (!g || !st) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  st[0] = g->plan->last_stats[0];
+  st[1] = g->plan->last_stats[1];
+  return IG_OK;
+}
