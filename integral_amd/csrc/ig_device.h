@@ -104,6 +104,52 @@ __device__ __forceinline__ void rys_axis_table(double c00, double cp0, double b1
   }
 }
 
+// Incremental decoder for a lane that walks t upward in steps: one binary
+// search at the first index, then rows advance by comparing against the cached
+// end of the current row (implicit sets), or a direct list read (explicit).
+struct RowCursor {
+  int P = 0;
+  long long row_end = -1;  // rowptr[P + 1]
+  long long row_begin = 0;
+  __device__ __forceinline__ void next(const QuartetSet &qs, long long t, int &Pout, int &Qout) {
+    if (qs.list) {
+      int2 v = qs.list[t];
+      Pout = v.x;
+      Qout = v.y;
+      return;
+    }
+    if (row_end < 0) {
+      int lo = 0, hi = qs.x_n - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (qs.rowptr[mid] <= t)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      P = lo;
+      row_begin = qs.rowptr[P];
+      row_end = qs.rowptr[P + 1];
+    }
+    while (t >= row_end) {
+      ++P;
+      row_begin = row_end;
+      row_end = qs.rowptr[P + 1];
+    }
+    Pout = P;
+    Qout = int(t - row_begin);
+  }
+};
+
+// Contiguous range of flat quartet indices owned by one wavefront.
+__device__ __forceinline__ void wave_range(long long total, long long &begin, long long &end) {
+  const long long waves = (long long)gridDim.x * (blockDim.x >> 6);
+  const long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  long long chunk = (total + waves - 1) / waves;
+  begin = w * chunk;
+  end = begin + chunk < total ? begin + chunk : total;
+}
+
 // Decode flat quartet index t of a QuartetSet into (P, Q) (relative indices).
 __device__ __forceinline__ void decode_quartet(const QuartetSet &qs, long long t, int &P, int &Q) {
   if (qs.list) {

@@ -18,6 +18,9 @@ __host__ __device__ constexpr int pick_team(int n4) {
                                                                    : n4 <= 1024 ? 32 : 64;
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// classes with at most this many Cartesian block elements run lane-per-quartet
+// with the block in VGPRs
+constexpr int kRegBlockMax = 81;
 
 template <int LA_, int LB_, int LC_, int LD_>
 struct Cls {
@@ -156,6 +159,37 @@ __device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R
     }
   }
 }
+
+// Compile-time element -> per-axis 2D-table offsets for the register path.
+template <class C>
+struct ElemTab {
+  int x[C::N4], y[C::N4], z[C::N4];
+  constexpr ElemTab() : x(), y(), z() {
+    for (int e = 0; e < C::N4; ++e) {
+      int id = e % C::nd, r = e / C::nd, ic = r % C::nc;
+      r /= C::nc;
+      int ib = r % C::nb, ia = r / C::nb;
+      int p[4][3] = {};
+      const int ls[4] = {C::LA, C::LB, C::LC, C::LD};
+      const int is[4] = {ia, ib, ic, id};
+      for (int q = 0; q < 4; ++q) {  // lx descending, ly descending (am.rs:23-31)
+        int d = 0;
+        while ((d + 1) * (d + 2) / 2 <= is[q]) ++d;
+        p[q][0] = ls[q] - d;
+        p[q][2] = is[q] - d * (d + 1) / 2;
+        p[q][1] = d - p[q][2];
+      }
+      int o[3] = {};
+      for (int ax = 0; ax < 3; ++ax)
+        o[ax] = ((p[0][ax] * (C::LB + 1) + p[1][ax]) * (C::LC + 1) + p[2][ax]) * (C::LD + 1) + p[3][ax];
+      x[e] = o[0];
+      y[e] = o[1];
+      z[e] = o[2];
+    }
+  }
+};
+
+#include "ig_reg.h"
 
 template <class C>
 __device__ __forceinline__ void init_offsets(int tl, int *offs) {
@@ -729,8 +763,16 @@ hipError_t jk_one(const BasisDev &b, const PairsDev &p, const RysDev &r, const Q
                   const DensDev &d, double omega, unsigned long long *ctr, hipStream_t s) {
   using C = Cls<LA, LB, LC, LD>;
   if (qs.total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(jk_kernel<C>, dim3(grid_for<LA, LB, LC, LD>(qs.total)), dim3(64), 0, s, b,
-                     p, r, qs, d, omega, ctr);
+  if constexpr (C::N4 <= kRegBlockMax) {
+    long long blocks = (qs.total + 63) / 64;
+    if (blocks > 8192) blocks = 8192;
+    const size_t lds = size_t(2) * r.nsub[C::NR] * C::NR * kChebTerms * sizeof(double);
+    hipLaunchKernelGGL(jk_reg_kernel<C>, dim3(int(blocks)), dim3(64), lds, s, b, p, r, qs, d,
+                       omega, ctr);
+  } else {
+    hipLaunchKernelGGL(jk_kernel<C>, dim3(grid_for<LA, LB, LC, LD>(qs.total)), dim3(64), 0, s,
+                       b, p, r, qs, d, omega, ctr);
+  }
   return hipGetLastError();
 }
 

@@ -1,0 +1,295 @@
+// ig_reg.h -- lane-per-quartet J/K path (included inside namespace ig by
+// ig_kernels.hip).  For classes whose Cartesian block fits one lane's VGPRs:
+// Rys roots and weights from Chebyshev tables staged in LDS once per
+// workgroup, per-root 2D tables and the contracted block in registers with
+// compile-time indices, J/K digestion from registers.  Each wavefront walks a
+// contiguous range of screened quartets; lanes of a wavefront usually share
+// the bra pair, so the bra-side Coulomb updates are reduced across the
+// wavefront before one atomic per element.
+
+// Rys coefficient tables of NR roots copied to LDS: roots then weights,
+// nsub * NR * 17 doubles each.
+template <int NR>
+__device__ __forceinline__ void stage_rys(const RysDev &R, double *lds) {
+  const int n = R.nsub[NR] * NR * kChebTerms;
+  const double *r = R.buf + R.root_off[NR], *w = R.buf + R.wt_off[NR];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    lds[i] = r[i];
+    lds[n + i] = w[i];
+  }
+  __syncthreads();
+}
+
+template <int NR>
+__device__ __forceinline__ void rys_all_lds(const RysDev &R, const double *lds, double T,
+                                            double *x, double *w) {
+  const double hi = 33.0 + 2.5 * double(2 * NR - 1);
+  if (T >= hi) {
+    const double *L = R.buf + R.lag_off[NR];
+    const double it = 1.0 / T, sc = 0.5 / sqrt(T);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      x[i] = L[i] * it;
+      w[i] = L[NR + i] * sc;
+    }
+    return;
+  }
+  const int nsub = R.nsub[NR];
+  const double width = hi / double(nsub);
+  int s = int(T / width);
+  s = s < nsub ? s : nsub - 1;
+  const double a = double(s) * width, b = a + width;
+  const double xi = (2.0 * T - (a + b)) / (b - a);
+  const double *cr = lds + s * NR * kChebTerms;
+  const double *cw = cr + nsub * NR * kChebTerms;
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    x[i] = cheb17(cr + i * kChebTerms, xi);
+    w[i] = cheb17(cw + i * kChebTerms, xi);
+  }
+}
+
+// Contracted Cartesian block of (pb | pk) into acc[N4].  Ket primitives in
+// the outer loop (per-lane data), bra primitives inside (usually uniform
+// across the wavefront).
+template <class C>
+__device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDev &R,
+                                                  const double *rys_lds, int pb, int pk,
+                                                  const QuartetGeom &g, double omega,
+                                                  double *acc) {
+  constexpr int NR = C::NR, T = C::T;
+  constexpr ElemTab<C> O{};
+  double AB[3], CD[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    AB[x] = g.A[x] - g.B[x];
+    CD[x] = g.C[x] - g.D[x];
+  }
+  const int ob = p.pp_off[pb], nbp = p.pp_n[pb];
+  const int ok = p.pp_off[pk], nkp = p.pp_n[pk];
+  for (int ik = 0; ik < nkp; ++ik) {
+    const double qz = p.zeta[ok + ik];
+    const double Q[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
+    const double kcd = p.kc[ok + ik];
+    double QC[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) QC[x] = Q[x] - g.C[x];
+    for (int ib = 0; ib < nbp; ++ib) {
+      const double pz = p.zeta[ob + ib];
+      const double P[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
+      const double kab = p.kc[ob + ib];
+      const double pq = pz + qz;
+      const double ipq = 1.0 / pq;
+      const double rho = pz * qz * ipq;
+      double PQ[3], r2 = 0.0;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        PQ[x] = P[x] - Q[x];
+        r2 += PQ[x] * PQ[x];
+      }
+      double Tq = rho * r2;
+      double pref = kTwoPi52 / (pz * qz * sqrt(pq)) * kab * kcd;
+      double s_att = 1.0;
+      if (omega > 0.0) {  // erf(w r)/r: roots/weights at sT, x -> s x, w -> sqrt(s) w
+        s_att = omega * omega / (rho + omega * omega);
+        Tq *= s_att;
+        pref *= sqrt(s_att);
+      }
+      double rx[NR], rwt[NR];
+      rys_all_lds<NR>(R, rys_lds, Tq, rx, rwt);
+      const double i2p = 0.5 / pz, i2q = 0.5 / qz;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const double x = rx[r] * s_att;
+        const double fq = x * qz * ipq, fp = x * pz * ipq;
+        const double b00 = 0.5 * x * ipq;
+        const double b10 = (1.0 - fq) * i2p;
+        const double b01 = (1.0 - fp) * i2q;
+        double I[3][T];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+          rys_axis_table<C::LA, C::LB, C::LC, C::LD>((P[ax] - g.A[ax]) - fq * PQ[ax],
+                                                     QC[ax] + fp * PQ[ax], b10, b01, b00, AB[ax],
+                                                     CD[ax], I[ax]);
+        const double pw = pref * rwt[r];
+#pragma unroll
+        for (int e = 0; e < C::N4; ++e) acc[e] += pw * (I[0][O.x[e]] * I[1][O.y[e]] * I[2][O.z[e]]);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// J/K digestion of a register block.  Called by every lane of the wavefront
+// (`on` false for lanes without a quartet).  When all lanes with work share
+// the bra shell pair, the Jp[a,b] sums are reduced across the wavefront first.
+template <class C>
+__device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
+                                           const QuartetGeom &g, bool same_pair,
+                                           unsigned long long mask, bool on, bool bra_uniform,
+                                           const double *blk) {
+  constexpr int na = C::na, nb = C::nb, nc = C::nc, nd = C::nd;
+  const double f = (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (same_pair ? 0.5 : 1.0);
+  const size_t n = (size_t)b.nao_cart, n2 = n * n;
+  const size_t a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
+               d0 = b.off_cart[g.sD];
+  const int lane = threadIdx.x & 63;
+  for (int e = 0; e < dd.neff; ++e) {
+    const bool use = on && ((mask >> dd.eff_src[e]) & 1ull);
+    const bool anti = dd.eff_anti[e];  // uniform
+    const double *D = dd.dc + (size_t)e * n2;
+    if (!anti) {
+      double *J = dd.jc + (size_t)e * n2;
+      double jcd[nc * nd];
+#pragma unroll
+      for (int q = 0; q < nc * nd; ++q) jcd[q] = 0.0;
+#pragma unroll
+      for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int bb = 0; bb < nb; ++bb) {
+          double v = 0.0;
+          if (use) {
+            const double dab = D[(a0 + a) * n + b0 + bb];
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+#pragma unroll
+              for (int d = 0; d < nd; ++d) {
+                const double gv = blk[((a * nb + bb) * nc + c) * nd + d];
+                v += gv * D[(c0 + c) * n + d0 + d];
+                jcd[c * nd + d] += gv * dab;
+              }
+          }
+          if (bra_uniform) {  // f depends on the lane's ket: scale before summing
+            v = wave_sum(2.0 * f * v);
+            if (lane == 0 && v != 0.0) atomicAdd(J + (a0 + a) * n + b0 + bb, v);
+          } else if (use) {
+            atomicAdd(J + (a0 + a) * n + b0 + bb, 2.0 * f * v);
+          }
+        }
+      if (use) {
+#pragma unroll
+        for (int c = 0; c < nc; ++c)
+#pragma unroll
+          for (int d = 0; d < nd; ++d)
+            atomicAdd(J + (c0 + c) * n + d0 + d, 2.0 * f * jcd[c * nd + d]);
+      }
+    }
+    if (use && dd.kc && dd.want_k) {
+      double *K = dd.kc + (size_t)e * n2;
+#pragma unroll
+      for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int c = 0; c < nc; ++c) {  // Kp[a,c] += g D[b,d]
+          double v = 0.0;
+#pragma unroll
+          for (int bb = 0; bb < nb; ++bb)
+#pragma unroll
+            for (int d = 0; d < nd; ++d)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * n + d0 + d];
+          atomicAdd(K + (a0 + a) * n + c0 + c, f * v);
+        }
+#pragma unroll
+      for (int bb = 0; bb < nb; ++bb)
+#pragma unroll
+        for (int c = 0; c < nc; ++c) {  // Kp[b,c] += g D[a,d]
+          double v = 0.0;
+#pragma unroll
+          for (int a = 0; a < na; ++a)
+#pragma unroll
+            for (int d = 0; d < nd; ++d)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * n + d0 + d];
+          atomicAdd(K + (b0 + bb) * n + c0 + c, f * v);
+        }
+#pragma unroll
+      for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {  // Kp[a,d] += g D[b,c]
+          double v = 0.0;
+#pragma unroll
+          for (int bb = 0; bb < nb; ++bb)
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * n + c0 + c];
+          atomicAdd(K + (a0 + a) * n + d0 + d, f * v);
+        }
+#pragma unroll
+      for (int bb = 0; bb < nb; ++bb)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {  // Kp[b,d] += g D[a,c]
+          double v = 0.0;
+#pragma unroll
+          for (int a = 0; a < na; ++a)
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * n + c0 + c];
+          atomicAdd(K + (b0 + bb) * n + d0 + d, f * v);
+        }
+    }
+  }
+}
+
+// Lane-per-quartet J/K kernel.  Every lane of a wavefront runs the same number
+// of iterations (lanes past the range or with an empty density mask compute
+// nothing) so the wavefront reductions see all 64 lanes.
+template <class C>
+__global__ void __launch_bounds__(64) jk_reg_kernel(BasisDev b, PairsDev p, RysDev R,
+                                                    QuartetSet qs, DensDev dd, double omega,
+                                                    unsigned long long *counters) {
+  extern __shared__ double rys_lds[];
+  stage_rys<C::NR>(R, rys_lds);
+  unsigned long long my_quartets = 0, my_prims = 0;
+  const int nsh = b.nsh;
+  const int lane = threadIdx.x & 63;
+  long long begin, end;
+  wave_range(qs.total, begin, end);
+  RowCursor cur;
+  for (long long base = begin; base < end; base += 64) {
+    const long long t = base + lane;
+    bool on = t < end;
+    int P = 0, Q = 0;
+    if (on) cur.next(qs, t, P, Q);
+    const int pb = qs.x_off + P, pk = qs.y_off + Q;
+    QuartetGeom g;
+    load_geom(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk], g);
+    unsigned long long mask = 0;
+    if (on) {
+      const double pair_bound = p.q[pb] * p.q[pk];
+      for (int d = 0; d < dd.ndens; ++d) {
+        const double *bd = dd.bound + (size_t)d * nsh * nsh;
+        double m = bd[g.sA * nsh + g.sB];
+        m = fmax(m, bd[g.sC * nsh + g.sD]);
+        m = fmax(m, bd[g.sA * nsh + g.sC]);
+        m = fmax(m, bd[g.sA * nsh + g.sD]);
+        m = fmax(m, bd[g.sB * nsh + g.sC]);
+        m = fmax(m, bd[g.sB * nsh + g.sD]);
+        if (pair_bound * m >= dd.tau) mask |= 1ull << d;
+      }
+      on = mask != 0;
+    }
+    double acc[C::N4];
+#pragma unroll
+    for (int e = 0; e < C::N4; ++e) acc[e] = 0.0;
+    if (on) {
+      quartet_block_reg<C>(p, R, rys_lds, pb, pk, g, omega, acc);
+      ++my_quartets;
+      my_prims += (unsigned long long)p.pp_n[pb] * p.pp_n[pk];
+    }
+    // bra pair shared by every lane with work?
+    const int pb_first = __shfl(pb, __ffsll((unsigned long long)__ballot(on)) - 1);
+    const bool bra_uniform = __all(!on || pb == pb_first);
+    digest_reg<C>(b, dd, g, pb == pk, mask, on, bra_uniform, acc);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    my_quartets += __shfl_down(my_quartets, off);
+    my_prims += __shfl_down(my_prims, off);
+  }
+  if (counters && lane == 0) {
+    atomicAdd(counters, my_quartets);
+    atomicAdd(counters + 1, my_prims);
+  }
+}
