@@ -43,6 +43,19 @@ def class_flops(rec, spherical=True, ndens=2):
     return per * q
 
 
+def kernel_name(cls, variant):
+    """Kernel(s) of one class launch: the team kernel, or the register kernel
+    in as many element slices as the class needs (launched back to back)."""
+    ncart = [(l + 1) * (l + 2) // 2 for l in cls]
+    n4 = ncart[0] * ncart[1] * ncart[2] * ncart[3]
+    c = "%d,%d,%d,%d" % tuple(cls)
+    if variant == 0:
+        return "jk_kernel<Cls<%s>>" % c
+    slices = (n4 + 80) // 81
+    return "jk_reg_kernel<Cls<%s>, LO, HI> (%d element slice%s)" % (c, slices,
+                                                                     "s" if slices > 1 else "")
+
+
 def cpu_baseline(shells, dens, tau, seconds):
     """Oracle on the host cores over a contiguous sample of canonical bra pairs
     from the middle of the list, grown until it takes about `seconds`."""
@@ -76,6 +89,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the built-in class-to-kernel schedule instead of tuning it")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device once and exit: the reference "
+                         "kernel of known traffic that scales the profiler's FETCH_SIZE and "
+                         "WRITE_SIZE readings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -84,6 +103,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d MiB" % args.copy_probe)
+        return
     import integral_amd as ig
     from integral_amd import synthetic
 
@@ -104,7 +132,15 @@ def main():
         K.zero_()
         dc.jk_device(D, J, K, threshold=tau, stream=stream)
 
-    log("plan built in %.2f s (nao=%d); warmup" % (plan_s, nao))
+    log("plan built in %.2f s (nao=%d)" % (plan_s, nao))
+    tune_s = 0.0
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        dc.autotune(D, threshold=tau, reps=2, stream=stream)
+        tune_s = time.perf_counter() - t0
+        log("autotuned class-to-kernel schedule in %.2f s" % tune_s)
+    sched = {e["cls"]: e["variant"] for e in dc.schedule()}
+    log("warmup")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize(dev)
@@ -137,7 +173,7 @@ def main():
     achieved = dom_flops / (dom["ms"] * 1e-3) / 1e12
     roofline = {
         "bound": "mfma", "pipe": "FP64 vector ALU (78.6 TFLOP/s, same as dense FP64 matrix)",
-        "kernel": "jk_kernel<%d,%d,%d,%d>" % tuple(dom["cls"]),
+        "kernel": kernel_name(dom["cls"], sched.get(tuple(dom["cls"]), 0)),
         "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
         "launch_ms": round(dom["ms"], 4), "launch_flops": dom_flops,
@@ -167,10 +203,12 @@ def main():
                    "nao": nao, "nshell": len(shells), "ndens": 2, "tau": tau,
                    "quartets": quartets, "prim_quartets": prim_quartets,
                    "jk_build_s": ms * 1e-3, "plan_build_s": plan_s,
-                   "parallelism": "1 GPU"},
+                   "schedule": "autotuned" if not args.no_autotune else "built-in",
+                   "autotune_s": tune_s, "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "classes": [{"cls": "".join("spdfghi"[l] for l in r["cls"]), "quartets": r["quartets"],
+                     "variant": ["team", "register"][sched.get(tuple(r["cls"]), 0)],
                      "ms": round(r["ms"], 4)} for r in recs],
     }
     print(json.dumps(out))

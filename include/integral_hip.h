@@ -160,6 +160,31 @@ typedef struct ig_class_profile {
 int ig_direct_set_profiling(ig_direct *direct, int enable);
 int ig_direct_profile_read(const ig_direct *direct, ig_class_profile *out, size_t cap, size_t *n);
 
+/* Class-to-kernel schedule of a plan.  Every compiled quartet class has two
+ * J/K kernels: variant 0 runs one quartet per team of lanes with its 2D
+ * tables in LDS, variant 1 one quartet per lane in registers.  The reference
+ * has no counterpart (its CPU engines pick by angular momentum only,
+ * integrals.rs:323-329); this replaces a fixed dispatch on the GPU.
+ *
+ * ig_direct_autotune times both variants of every class pair with work, on
+ * the given device densities and threshold (min over `reps` launches each),
+ * and keeps the faster in the plan.  Trial results go to plan scratch; no
+ * caller buffer is written.  Synchronises `stream`. */
+typedef struct ig_schedule_entry {
+  int32_t la, lb, lc, ld;  /* kernel orientation, pair class (la,lb) >= (lc,ld) */
+  int32_t variant;         /* 0 team/LDS, 1 lane/registers; -1 on write = default */
+  int32_t tuned;           /* read: 1 if set by the autotuner or a write */
+  double ms;               /* read: the autotuner's time for the chosen variant */
+} ig_schedule_entry;
+int ig_direct_autotune(ig_direct *direct, const double *d_dens, size_t ndens, double threshold,
+                       int want_exchange, int reps, void *stream);
+/* All compiled class pairs present in the plan's basis, with the variant in
+ * use; *n = the number available, up to `cap` written. */
+int ig_direct_schedule_read(const ig_direct *direct, ig_schedule_entry *out, size_t cap,
+                            size_t *n);
+/* Load a schedule (e.g. saved from an earlier autotune of the same basis). */
+int ig_direct_schedule_write(ig_direct *direct, const ig_schedule_entry *in, size_t n);
+
 /* ---- ERI-derivative contraction: DirectGradContractor (direct_grad.rs) --- */
 typedef struct ig_grad ig_grad;
 typedef struct ig_grad_density ig_grad_density;

@@ -134,6 +134,12 @@ class _ClassProfile(ctypes.Structure):
 _lib = None
 
 
+class _ScheduleEntry(ctypes.Structure):
+    _fields_ = [("la", ctypes.c_int32), ("lb", ctypes.c_int32), ("lc", ctypes.c_int32),
+                ("ld", ctypes.c_int32), ("variant", ctypes.c_int32), ("tuned", ctypes.c_int32),
+                ("ms", ctypes.c_double)]
+
+
 def library():
     """Load libintegral_hip.so (built by __graft_entry__.build or `make -C integral_amd/csrc`)."""
     global _lib
@@ -174,6 +180,11 @@ def library():
     lib.ig_direct_set_profiling.argtypes = [_vp, ctypes.c_int]
     lib.ig_direct_profile_read.argtypes = [_vp, ctypes.POINTER(_ClassProfile), sz,
                                            ctypes.POINTER(sz)]
+    lib.ig_direct_autotune.argtypes = [_vp, _vp, sz, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                       _vp]
+    lib.ig_direct_schedule_read.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz,
+                                            ctypes.POINTER(sz)]
+    lib.ig_direct_schedule_write.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz]
     lib.ig_grad_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
     lib.ig_grad_destroy.argtypes = [_vp]
     lib.ig_grad_natom.argtypes = [_vp]
@@ -444,6 +455,40 @@ class DirectContractor:
 
     def set_profiling(self, enable):
         _raise(library().ig_direct_set_profiling(self._h, 1 if enable else 0))
+
+    # J/K kernel variants of a compiled quartet class
+    TEAM, REGISTER = 0, 1
+
+    def autotune(self, dens, threshold=0.0, want_exchange=True, reps=2, stream=None):
+        """Time both J/K kernel variants of every class pair on the device
+        densities `dens` (float64 CUDA tensor, ndens x nao x nao) and keep the
+        faster per class pair; returns the resulting schedule."""
+        if not (dens.is_cuda and dens.element_size() == 8 and dens.is_contiguous()):
+            raise UsageError("densities must be a contiguous float64 CUDA tensor")
+        _raise(library().ig_direct_autotune(self._h, _vp(dens.data_ptr()), dens.shape[0],
+                                            float(threshold), 1 if want_exchange else 0,
+                                            int(reps), _stream_ptr(stream)))
+        return self.schedule()
+
+    def schedule(self):
+        """[{"cls": (la, lb, lc, ld), "variant": v, "tuned": bool, "ms": t}] for
+        every compiled class pair of this basis."""
+        lib = library()
+        n = ctypes.c_size_t()
+        _raise(lib.ig_direct_schedule_read(self._h, None, 0, ctypes.byref(n)))
+        buf = (_ScheduleEntry * max(n.value, 1))()
+        _raise(lib.ig_direct_schedule_read(self._h, buf, n.value, ctypes.byref(n)))
+        return [{"cls": (e.la, e.lb, e.lc, e.ld), "variant": e.variant, "tuned": bool(e.tuned),
+                 "ms": e.ms} for e in buf[:n.value]]
+
+    def set_schedule(self, entries):
+        """Load entries as returned by schedule() (variant -1 = default)."""
+        buf = (_ScheduleEntry * max(len(entries), 1))()
+        for b, e in zip(buf, entries):
+            b.la, b.lb, b.lc, b.ld = e["cls"]
+            b.variant = int(e["variant"])
+            b.ms = float(e.get("ms", 0.0))
+        _raise(library().ig_direct_schedule_write(self._h, buf, len(entries)))
 
     def profile(self):
         """Per-class records of the last call: dicts with la..ld, quartets,

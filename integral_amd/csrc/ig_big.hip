@@ -1,0 +1,700 @@
+// ig_big.hip -- runtime-angular-momentum quartet kernels for the classes the
+// compiled (template) kernels do not cover: shells up to l = 6, and gradient
+// tables raised by one on three centres for shells up to l = 5.
+//
+// One 256-thread workgroup owns one shell quartet at a time:
+//   * lanes evaluate the n Rys roots/weights (n <= 13) into LDS;
+//   * per batch of RB roots (RB chosen on the host so the batch's 2D tables
+//     and their recurrence work areas fit the dynamic LDS budget), 3*RB lanes
+//     build the x, y, z tables (runtime extents) in LDS;
+//   * all lanes accumulate their strided slice of the Cartesian block into the
+//     workgroup's global scratch slot (blocks reach 28^4 elements, beyond LDS);
+//   * the epilogue (J/K digestion, dense scatter, Schwarz diagonal or the
+//     adjoint gradient contraction) reads the block back from the slot.
+// The grid is persistent (at most kBigMaxWG workgroups, one scratch slot each)
+// and walks the quartet set with a grid stride; every workgroup leaves the
+// loop once its stride passes the set's total.
+#include "ig_device.h"
+
+namespace ig {
+
+namespace {
+
+constexpr int kBigThreads = 256;
+constexpr int kBigWaves = kBigThreads / 64;
+constexpr int kBigMaxWG = 1024;
+constexpr size_t kBigLdsBudget = 64 * 1024;         // dynamic LDS per workgroup
+constexpr size_t kBigScratchBytes = size_t(1) << 30;  // all slots of one launch
+
+struct BigStatic {
+  double rw[2 * kMaxRys];
+  double red[9 * kBigWaves];
+};
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Work-area size (doubles) of one (root, axis) table task.
+__host__ __device__ inline int table_work(int la, int lb, int lc, int ld) {
+  const int NB = la + lb, NK = lc + ld;
+  return (NB + 1) * (NK + 1) + (NK + 1);
+}
+__host__ __device__ inline int table_size(int la, int lb, int lc, int ld) {
+  return (la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
+}
+
+// Same recurrences as rys_axis_table (engine/rys.rs:234-308), runtime extents:
+// vertical recurrence on (bra, ket) combined powers into g, then the bra and
+// ket horizontal transfers; out[((i*(lb+1)+j)*(lc+1)+k)*(ld+1)+l].
+__device__ void table_rt(int la, int lb, int lc, int ld, double c00, double cp0, double b10,
+                         double b01, double b00, double ab, double cd, double *g, double *t,
+                         double *out) {
+  const int NB = la + lb, NK = lc + ld, W = NK + 1;
+  g[0] = 1.0;
+  if (NB >= 1) g[W] = c00;
+  for (int n = 1; n < NB; ++n) g[(n + 1) * W] = c00 * g[n * W] + double(n) * b10 * g[(n - 1) * W];
+  for (int m = 0; m < NK; ++m)
+    for (int n = 0; n <= NB; ++n) {
+      double v = cp0 * g[n * W + m];
+      if (m >= 1) v += double(m) * b01 * g[n * W + m - 1];
+      if (n >= 1) v += double(n) * b00 * g[(n - 1) * W + m];
+      g[n * W + m + 1] = v;
+    }
+  for (int j = 0; j <= lb; ++j) {
+    if (j > 0)  // bra transfer: I(i, j) = I(i+1, j-1) + AB I(i, j-1), in place over i
+      for (int i = 0; i <= NB - j; ++i)
+        for (int m = 0; m <= NK; ++m) g[i * W + m] = g[(i + 1) * W + m] + ab * g[i * W + m];
+    for (int i = 0; i <= la; ++i) {
+      for (int k = 0; k <= NK; ++k) t[k] = g[i * W + k];
+      for (int l = 0; l <= ld; ++l) {
+        if (l > 0)
+          for (int k = 0; k <= NK - l; ++k) t[k] = t[k + 1] + cd * t[k];
+        for (int k = 0; k <= lc; ++k) out[((i * (lb + 1) + j) * (lc + 1) + k) * (ld + 1) + l] = t[k];
+      }
+    }
+  }
+}
+
+struct BigClass {
+  int la, lb, lc, ld, na, nb, nc, nd, N4, NR;
+  __device__ BigClass(int a, int b, int c, int d, int raise)
+      : la(a), lb(b), lc(c), ld(d), na(ncart(a)), nb(ncart(b)), nc(ncart(c)), nd(ncart(d)) {
+    N4 = na * nb * nc * nd;
+    NR = (a + b + c + d + raise) / 2 + 1;
+  }
+  // element e -> per-axis powers of the four shells
+  __device__ void powers(int e, int pa[3], int pb[3], int pc[3], int pd[3]) const {
+    const int id = e % nd;
+    int r = e / nd;
+    const int ic = r % nc;
+    r /= nc;
+    const int ib = r % nb, ia = r / nb;
+    cart_powers(la, ia, pa[0], pa[1], pa[2]);
+    cart_powers(lb, ib, pb[0], pb[1], pb[2]);
+    cart_powers(lc, ic, pc[0], pc[1], pc[2]);
+    cart_powers(ld, id, pd[0], pd[1], pd[2]);
+  }
+};
+
+// Primitive-quartet scalars (same expressions as quartet_block in ig_kernels.hip).
+struct PrimQ {
+  double pz, qz, ipq, i2p, i2q, P[3], Q[3], PQ[3], pref, T, s_att, al, be, ga;
+};
+
+__device__ __forceinline__ void prim_quartet(const PairsDev &p, int ob, int ib, int ok, int ik,
+                                             double omega, PrimQ &q) {
+  q.pz = p.zeta[ob + ib];
+  q.qz = p.zeta[ok + ik];
+  q.P[0] = p.px[ob + ib];
+  q.P[1] = p.py[ob + ib];
+  q.P[2] = p.pz[ob + ib];
+  q.Q[0] = p.px[ok + ik];
+  q.Q[1] = p.py[ok + ik];
+  q.Q[2] = p.pz[ok + ik];
+  q.al = p.alpha[ob + ib];
+  q.be = p.beta[ob + ib];
+  q.ga = p.alpha[ok + ik];
+  const double rs = rsqrt(q.pz + q.qz);
+  q.ipq = rs * rs;
+  q.i2p = p.i2z[ob + ib];
+  q.i2q = p.i2z[ok + ik];
+  const double rho = q.pz * q.qz * q.ipq;
+  double r2 = 0.0;
+  for (int x = 0; x < 3; ++x) {
+    q.PQ[x] = q.P[x] - q.Q[x];
+    r2 += q.PQ[x] * q.PQ[x];
+  }
+  q.T = rho * r2;
+  q.pref = kTwoPi52 * p.kz[ob + ib] * p.kz[ok + ik] * rs;
+  q.s_att = 1.0;
+  if (omega > 0.0) {  // erf(w r)/r: roots/weights at sT, x -> s x, w -> sqrt(s) w
+    q.s_att = omega * omega / (rho + omega * omega);
+    q.T *= q.s_att;
+    q.pref *= sqrt(q.s_att);
+  }
+}
+
+struct BigGeom {
+  int sA, sB, sC, sD;
+  double A[3], C[3], AB[3], CD[3];
+  __device__ void load(const BasisDev &b, int a, int bb, int c, int d) {
+    sA = a;
+    sB = bb;
+    sC = c;
+    sD = d;
+    for (int x = 0; x < 3; ++x) {
+      A[x] = b.xyz[3 * a + x];
+      C[x] = b.xyz[3 * c + x];
+      AB[x] = A[x] - b.xyz[3 * bb + x];
+      CD[x] = C[x] - b.xyz[3 * d + x];
+    }
+  }
+};
+
+// Roots/weights of one primitive quartet into S.rw (ends with a barrier).
+__device__ __forceinline__ void roots_rt(const RysDev &R, int NR, const PrimQ &q, BigStatic &S) {
+  for (int idx = threadIdx.x; idx < 2 * NR; idx += blockDim.x) {
+    const double v = rys_series(R, NR, q.T, idx);
+    S.rw[idx] = (idx < NR) ? v * q.s_att : v;
+  }
+  __syncthreads();
+}
+
+// Tables of roots [r0, r0+nr) for extents (la..ld): task t = 3*(r-r0)+axis
+// writes tab + t*T (ends with a barrier).
+__device__ __forceinline__ void tables_rt(int la, int lb, int lc, int ld, int r0, int nr,
+                                          const PrimQ &q, const BigGeom &g, const BigStatic &S,
+                                          double *tab, double *work) {
+  const int T = table_size(la, lb, lc, ld), GW = table_work(la, lb, lc, ld);
+  const int NW = (la + lb + 1) * (lc + ld + 1);
+  for (int task = threadIdx.x; task < 3 * nr; task += blockDim.x) {
+    const int r = r0 + task / 3, ax = task % 3;
+    const double x = S.rw[r];
+    const double fq = x * q.qz * q.ipq, fp = x * q.pz * q.ipq;
+    const double b00 = 0.5 * x * q.ipq;
+    const double b10 = (1.0 - fq) * q.i2p;
+    const double b01 = (1.0 - fp) * q.i2q;
+    const double c00 = (q.P[ax] - g.A[ax]) - fq * q.PQ[ax];
+    const double cp0 = (q.Q[ax] - g.C[ax]) + fp * q.PQ[ax];
+    double *w = work + task * GW;
+    table_rt(la, lb, lc, ld, c00, cp0, b10, b01, b00, g.AB[ax], g.CD[ax], w, w + NW,
+             tab + task * T);
+  }
+  __syncthreads();
+}
+
+// Contracted Cartesian block of quartet (pb | pk) into blk (N4 doubles).
+__device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const RysDev &R, int pb,
+                          int pk, const BigGeom &g, double omega, BigStatic &S, double *dyn,
+                          double *__restrict__ blk) {
+  const int T = table_size(K.la, K.lb, K.lc, K.ld);
+  double *tab = dyn, *work = dyn + 3 * rb * T;
+  for (int e = threadIdx.x; e < K.N4; e += blockDim.x) blk[e] = 0.0;
+  const int ob = p.pp_off[pb], nbp = p.pp_n[pb], ok = p.pp_off[pk], nkp = p.pp_n[pk];
+  const int SA = (K.lb + 1) * (K.lc + 1) * (K.ld + 1), SB = (K.lc + 1) * (K.ld + 1),
+            SC = K.ld + 1;
+  for (int ib = 0; ib < nbp; ++ib)
+    for (int ik = 0; ik < nkp; ++ik) {
+      PrimQ q;
+      prim_quartet(p, ob, ib, ok, ik, omega, q);
+      roots_rt(R, K.NR, q, S);
+      for (int r0 = 0; r0 < K.NR; r0 += rb) {
+        const int nr = K.NR - r0 < rb ? K.NR - r0 : rb;
+        tables_rt(K.la, K.lb, K.lc, K.ld, r0, nr, q, g, S, tab, work);
+        for (int e = threadIdx.x; e < K.N4; e += blockDim.x) {
+          int pa[3], pbp[3], pc[3], pd[3];
+          K.powers(e, pa, pbp, pc, pd);
+          const int ox = pa[0] * SA + pbp[0] * SB + pc[0] * SC + pd[0];
+          const int oy = pa[1] * SA + pbp[1] * SB + pc[1] * SC + pd[1];
+          const int oz = pa[2] * SA + pbp[2] * SB + pc[2] * SC + pd[2];
+          double v = 0.0;
+          for (int k = 0; k < nr; ++k) {
+            const double *tx = tab + 3 * k * T;
+            v += S.rw[K.NR + r0 + k] * tx[ox] * tx[T + oy] * tx[2 * T + oz];
+          }
+          blk[e] += q.pref * v;
+        }
+        __syncthreads();  // tables are rebuilt for the next batch
+      }
+    }
+}
+
+__device__ __forceinline__ void next_quartet(const QuartetSet &qs, long long t, int &pb, int &pk) {
+  int P, Q;
+  decode_quartet(qs, t, P, Q);
+  pb = qs.x_off + P;
+  pk = qs.y_off + Q;
+}
+
+// ---------------------------------------------------------------------------
+// Direct J/K: same screening and digestion as jk_reg_kernel (direct.rs:131-218
+// contract, Cartesian digestion of the canonical quartet with weight f).
+__global__ void __launch_bounds__(kBigThreads)
+    jk_big_kernel(int la, int lb, int lc, int ld, int rb, BasisDev b, PairsDev p, RysDev R,
+                  QuartetSet qs, DensDev dd, double omega, double *scratch, size_t slot,
+                  unsigned long long *counters) {
+  __shared__ BigStatic S;
+  extern __shared__ double dyn[];
+  const BigClass K(la, lb, lc, ld, 0);
+  double *blk = scratch + (size_t)blockIdx.x * slot;
+  const int nsh = b.nsh, ncrt = b.nao_cart;
+  const size_t n2 = (size_t)ncrt * ncrt;
+  unsigned long long my_q = 0, my_p = 0;
+  for (long long t = blockIdx.x; t < qs.total; t += gridDim.x) {
+    int pb, pk;
+    next_quartet(qs, t, pb, pk);
+    BigGeom g;
+    g.load(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk]);
+    const double pair_bound = p.q[pb] * p.q[pk];
+    unsigned long long mask = 0;
+    for (int d = 0; d < dd.ndens; ++d) {
+      const double *bd = dd.bound + (size_t)d * nsh * nsh;
+      double m = bd[g.sA * nsh + g.sB];
+      m = fmax(m, bd[g.sC * nsh + g.sD]);
+      m = fmax(m, bd[g.sA * nsh + g.sC]);
+      m = fmax(m, bd[g.sA * nsh + g.sD]);
+      m = fmax(m, bd[g.sB * nsh + g.sC]);
+      m = fmax(m, bd[g.sB * nsh + g.sD]);
+      if (pair_bound * m >= dd.tau) mask |= 1ull << d;
+    }
+    if (!mask) continue;  // uniform over the workgroup
+    big_block(K, rb, p, R, pb, pk, g, omega, S, dyn, blk);
+    __syncthreads();  // the epilogue reads elements written by other lanes
+    if (threadIdx.x == 0) {
+      ++my_q;
+      my_p += (unsigned long long)p.pp_n[pb] * p.pp_n[pk];
+    }
+    const double f =
+        (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (pb == pk ? 0.5 : 1.0);
+    const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
+              d0 = b.off_cart[g.sD];
+    const int na = K.na, nb = K.nb, nc = K.nc, nd = K.nd;
+    const int NJ = na * nb + nc * nd;
+    const int NO = NJ + ((dd.kc && dd.want_k) ? na * nc + nb * nc + na * nd + nb * nd : 0);
+    for (int e = 0; e < dd.neff; ++e) {
+      if (!((mask >> dd.eff_src[e]) & 1ull)) continue;
+      const bool anti = dd.eff_anti[e];
+      const double *D = dd.dc + (size_t)e * n2;
+      double *J = dd.jc + (size_t)e * n2;
+      double *Kp = dd.kc ? dd.kc + (size_t)e * n2 : nullptr;
+      for (int o = threadIdx.x; o < NO; o += blockDim.x) {
+        if (anti && o < NJ) continue;  // antisymmetric parts feed K only
+        double v = 0.0;
+        int row, col;
+        double *dst;
+        if (o < na * nb) {  // J[a,b] += 2f sum_cd g D[c,d]
+          const int a = o / nb, bb = o % nb;
+          for (int c = 0; c < nc; ++c)
+            for (int d = 0; d < nd; ++d)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(c0 + c) * ncrt + d0 + d];
+          v *= 2.0 * f;
+          row = a0 + a;
+          col = b0 + bb;
+          dst = J;
+        } else if (o < NJ) {  // J[c,d] += 2f sum_ab g D[a,b]
+          const int q = o - na * nb, c = q / nd, d = q % nd;
+          for (int a = 0; a < na; ++a)
+            for (int bb = 0; bb < nb; ++bb)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(a0 + a) * ncrt + b0 + bb];
+          v *= 2.0 * f;
+          row = c0 + c;
+          col = d0 + d;
+          dst = J;
+        } else {
+          int q = o - NJ;
+          if (q < na * nc) {  // Kp[a,c] += f g D[b,d]
+            const int a = q / nc, c = q % nc;
+            for (int bb = 0; bb < nb; ++bb)
+              for (int d = 0; d < nd; ++d)
+                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(b0 + bb) * ncrt + d0 + d];
+            row = a0 + a;
+            col = c0 + c;
+          } else if ((q -= na * nc) < nb * nc) {  // Kp[b,c] += f g D[a,d]
+            const int bb = q / nc, c = q % nc;
+            for (int a = 0; a < na; ++a)
+              for (int d = 0; d < nd; ++d)
+                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(a0 + a) * ncrt + d0 + d];
+            row = b0 + bb;
+            col = c0 + c;
+          } else if ((q -= nb * nc) < na * nd) {  // Kp[a,d] += f g D[b,c]
+            const int a = q / nd, d = q % nd;
+            for (int bb = 0; bb < nb; ++bb)
+              for (int c = 0; c < nc; ++c)
+                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(b0 + bb) * ncrt + c0 + c];
+            row = a0 + a;
+            col = d0 + d;
+          } else {  // Kp[b,d] += f g D[a,c]
+            q -= na * nd;
+            const int bb = q / nd, d = q % nd;
+            for (int a = 0; a < na; ++a)
+              for (int c = 0; c < nc; ++c)
+                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(a0 + a) * ncrt + c0 + c];
+            row = b0 + bb;
+            col = d0 + d;
+          }
+          v *= f;
+          dst = Kp;
+        }
+        if (v != 0.0) atomicAdd(dst + (size_t)row * ncrt + col, v);
+      }
+    }
+    __syncthreads();  // the slot is overwritten by the next quartet
+  }
+  if (counters && threadIdx.x == 0) {
+    atomicAdd(counters, my_q);
+    atomicAdd(counters + 1, my_p);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense ERI: c2s per spherical shell in the slot (two halves, ping-pong), then
+// scatter to every distinct permutation of the canonical quartet.
+__device__ __constant__ int kBigPerms8[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2},
+                                                {1, 0, 3, 2}, {2, 3, 0, 1}, {2, 3, 1, 0},
+                                                {3, 2, 0, 1}, {3, 2, 1, 0}};
+
+__global__ void __launch_bounds__(kBigThreads)
+    eri_big_kernel(int la, int lb, int lc, int ld, int rb, BasisDev b, PairsDev p, RysDev R,
+                   QuartetSet qs, double *out, double *scratch, size_t slot) {
+  __shared__ BigStatic S;
+  extern __shared__ double dyn[];
+  const BigClass K(la, lb, lc, ld, 0);
+  double *half0 = scratch + (size_t)blockIdx.x * slot, *half1 = half0 + K.N4;
+  const size_t nao = (size_t)b.nao;
+  for (long long t = blockIdx.x; t < qs.total; t += gridDim.x) {
+    int pb, pk;
+    next_quartet(qs, t, pb, pk);
+    BigGeom g;
+    g.load(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk]);
+    big_block(K, rb, p, R, pb, pk, g, 0.0, S, dyn, half0);
+    __syncthreads();
+    double *c0 = half0, *c1 = half1;
+    const int sh[4] = {g.sA, g.sB, g.sC, g.sD};
+    const int ls[4] = {la, lb, lc, ld};
+    int dims[4] = {K.na, K.nb, K.nc, K.nd};
+    for (int ax = 0; ax < 4; ++ax) {
+      if (!b.kind[sh[ax]]) continue;
+      const int nin = dims[ax], nout = 2 * ls[ax] + 1;
+      const double *M = b.c2s + b.c2s_off[ls[ax]];
+      int outer = 1, inner = 1;
+      for (int x = 0; x < ax; ++x) outer *= dims[x];
+      for (int x = ax + 1; x < 4; ++x) inner *= dims[x];
+      for (int o = threadIdx.x; o < outer * nout * inner; o += blockDim.x) {
+        const int i = o % inner, rq = o / inner, m = rq % nout, oo = rq / nout;
+        double v = 0.0;
+        for (int j = 0; j < nin; ++j) v += M[m * nin + j] * c0[(oo * nin + j) * inner + i];
+        c1[o] = v;
+      }
+      dims[ax] = nout;
+      double *tmp = c0;
+      c0 = c1;
+      c1 = tmp;
+      __syncthreads();
+    }
+    const int nf = dims[0] * dims[1] * dims[2] * dims[3];
+    int seen[8][4];
+    int nseen = 0;
+    for (int pi = 0; pi < 8; ++pi) {
+      int tup[4];
+      for (int m = 0; m < 4; ++m) tup[m] = sh[kBigPerms8[pi][m]];
+      bool dup = false;
+      for (int s = 0; s < nseen; ++s)
+        dup |= (seen[s][0] == tup[0] && seen[s][1] == tup[1] && seen[s][2] == tup[2] &&
+                seen[s][3] == tup[3]);
+      if (dup) continue;
+      for (int m = 0; m < 4; ++m) seen[nseen][m] = tup[m];
+      ++nseen;
+      // position m of the permuted index takes source axis kBigPerms8[pi][m]
+      size_t stride[4], base = 0;
+      for (int m = 0; m < 4; ++m) {
+        const int src = kBigPerms8[pi][m];
+        size_t st = 1;
+        for (int x = 0; x < 3 - m; ++x) st *= nao;
+        stride[src] = st;
+        base += (size_t)b.off_func[sh[src]] * st;
+      }
+      for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+        const int id = e % dims[3], r1 = e / dims[3], ic = r1 % dims[2], r2 = r1 / dims[2],
+                  ibb = r2 % dims[1], ia = r2 / dims[1];
+        out[base + ia * stride[0] + ibb * stride[1] + ic * stride[2] + id * stride[3]] = c0[e];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Schwarz diagonal: Q[ab] = sqrt(max |(ab|ab)|) over the pair's functions
+// (screening.rs:17-52), spherical shells transformed on the fly.
+__global__ void __launch_bounds__(kBigThreads)
+    schwarz_big_kernel(int la, int lb, int rb, BasisDev b, PairsDev p, RysDev R, int first,
+                       int count, double *qout, double *scratch, size_t slot) {
+  __shared__ BigStatic S;
+  extern __shared__ double dyn[];
+  const BigClass K(la, lb, la, lb, 0);
+  double *blk = scratch + (size_t)blockIdx.x * slot;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = blockIdx.x; t < count; t += gridDim.x) {
+    const int pr = first + t;
+    BigGeom g;
+    g.load(b, p.sh1[pr], p.sh2[pr], p.sh1[pr], p.sh2[pr]);
+    big_block(K, rb, p, R, pr, pr, g, 0.0, S, dyn, blk);
+    __syncthreads();
+    const bool sa = b.kind[g.sA], sb = b.kind[g.sB];
+    const int na = K.na, nb = K.nb;
+    const int nfa = sa ? 2 * la + 1 : na, nfb = sb ? 2 * lb + 1 : nb;
+    const double *Ma = b.c2s + b.c2s_off[la], *Mb = b.c2s + b.c2s_off[lb];
+    double mx = 0.0;
+    for (int o = threadIdx.x; o < nfa * nfb; o += blockDim.x) {
+      const int mu = o / nfb, nu = o % nfb;
+      double v = 0.0;
+      for (int a = 0; a < na; ++a) {
+        const double ma = sa ? Ma[mu * na + a] : (a == mu ? 1.0 : 0.0);
+        if (ma == 0.0) continue;
+        for (int bb = 0; bb < nb; ++bb) {
+          const double mb = sb ? Mb[nu * nb + bb] : (bb == nu ? 1.0 : 0.0);
+          if (mb == 0.0) continue;
+          for (int c = 0; c < na; ++c) {
+            const double mc = sa ? Ma[mu * na + c] : (c == mu ? 1.0 : 0.0);
+            if (mc == 0.0) continue;
+            for (int d = 0; d < nb; ++d) {
+              const double md = sb ? Mb[nu * nb + d] : (d == nu ? 1.0 : 0.0);
+              if (md == 0.0) continue;
+              v += ma * mb * mc * md * blk[((a * nb + bb) * na + c) * nb + d];
+            }
+          }
+        }
+      }
+      mx = fmax(mx, fabs(v));
+    }
+    mx = wave_max_d(mx);
+    if (lane == 0) S.red[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < kBigWaves; ++w) mx = fmax(mx, S.red[w]);
+      qout[pr] = sqrt(mx);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint gradient: the same contraction as grad_kernel in ig_kernels.hip
+// (grad.rs direct gradient): per element the density weight W (Coulomb and
+// exchange terms), then per root the derivative tables from the raised 2D
+// integrals, D's derivative by translational invariance.
+__global__ void __launch_bounds__(kBigThreads)
+    grad_big_kernel(int la, int lb, int lc, int ld, int rb, BasisDev b, PairsDev p, RysDev R,
+                    QuartetSet qs, GradDev gd, double *scratch, size_t slot,
+                    unsigned long long *counters) {
+  __shared__ BigStatic S;
+  extern __shared__ double dyn[];
+  const BigClass K(la, lb, lc, ld, 1);
+  double *W = scratch + (size_t)blockIdx.x * slot;
+  const int nsh = b.nsh, ncrt = b.nao_cart;
+  const size_t nc2 = (size_t)ncrt * ncrt;
+  const int nterm = gd.ncoul + gd.nexch;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // raised extents: A, B, C up by one, D unchanged
+  const int RA = la + 1, RB = lb + 1, RC = lc + 1, RD = ld;
+  const int TR = table_size(RA, RB, RC, RD);
+  const int SA = (RB + 1) * (RC + 1) * (RD + 1), SB = (RC + 1) * (RD + 1), SC = RD + 1;
+  double *tab = dyn, *work = dyn + 3 * rb * TR;
+  unsigned long long my_q = 0, my_p = 0;
+  for (long long t = blockIdx.x; t < qs.total; t += gridDim.x) {
+    int pb, pk;
+    next_quartet(qs, t, pb, pk);
+    BigGeom g;
+    g.load(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk]);
+    const int atA = b.atom[g.sA], atB = b.atom[g.sB], atC = b.atom[g.sC], atD = b.atom[g.sD];
+    if (atA == atB && atB == atC && atC == atD) continue;  // one-centre: zero by invariance
+    const double deg =
+        (g.sA == g.sB ? 1.0 : 2.0) * (g.sC == g.sD ? 1.0 : 2.0) * (pb == pk ? 1.0 : 2.0);
+    double bound = 0.0;
+    {
+      const int i = g.sA, j = g.sB, k = g.sC, l = g.sD;
+      for (int tm = 0; tm < nterm; ++tm) {
+        const double *Lm = gd.bmax + (size_t)(2 * tm) * nsh * nsh;
+        const double *Rm = Lm + (size_t)nsh * nsh;
+        if (tm < gd.ncoul)
+          bound += 0.5 * fabs(gd.fac[tm]) *
+                   (Lm[i * nsh + j] * Rm[k * nsh + l] + Lm[k * nsh + l] * Rm[i * nsh + j]);
+        else
+          bound += 0.25 * fabs(gd.fac[tm]) *
+                   (Lm[i * nsh + k] * Rm[j * nsh + l] + Lm[j * nsh + k] * Rm[i * nsh + l] +
+                    Lm[i * nsh + l] * Rm[j * nsh + k] + Lm[j * nsh + l] * Rm[i * nsh + k]);
+      }
+    }
+    if (p.q[pb] * p.q[pk] * deg * bound < gd.tau) continue;
+    const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
+              d0 = b.off_cart[g.sD];
+    for (int e = threadIdx.x; e < K.N4; e += blockDim.x) {
+      const int id = e % K.nd, r1 = e / K.nd, ic = r1 % K.nc, r2 = r1 / K.nc, ib = r2 % K.nb,
+                ia = r2 / K.nb;
+      const size_t A = a0 + ia, B = b0 + ib, Cc = c0 + ic, D = d0 + id;
+      double w = 0.0;
+      for (int tm = 0; tm < nterm; ++tm) {
+        const double *Lc = gd.mats + (size_t)(2 * tm) * nc2, *Rc = Lc + nc2;
+        if (tm < gd.ncoul)
+          w += 0.5 * deg * gd.fac[tm] *
+               (Lc[A * ncrt + B] * Rc[Cc * ncrt + D] + Rc[A * ncrt + B] * Lc[Cc * ncrt + D]);
+        else
+          w += 0.25 * deg * gd.fac[tm] *
+               (Lc[A * ncrt + Cc] * Rc[B * ncrt + D] + Lc[B * ncrt + Cc] * Rc[A * ncrt + D] +
+                Lc[A * ncrt + D] * Rc[B * ncrt + Cc] + Lc[B * ncrt + D] * Rc[A * ncrt + Cc]);
+      }
+      W[e] = w;  // each lane reads back only its own elements
+    }
+    double gs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int ob = p.pp_off[pb], nbp = p.pp_n[pb], ok = p.pp_off[pk], nkp = p.pp_n[pk];
+    for (int ib = 0; ib < nbp; ++ib)
+      for (int ik = 0; ik < nkp; ++ik) {
+        PrimQ q;
+        prim_quartet(p, ob, ib, ok, ik, gd.omega, q);
+        roots_rt(R, K.NR, q, S);
+        const double ta = 2.0 * q.al, tb = 2.0 * q.be, tc = 2.0 * q.ga;
+        for (int r0 = 0; r0 < K.NR; r0 += rb) {
+          const int nr = K.NR - r0 < rb ? K.NR - r0 : rb;
+          tables_rt(RA, RB, RC, RD, r0, nr, q, g, S, tab, work);
+          for (int e = threadIdx.x; e < K.N4; e += blockDim.x) {
+            int pa[3], pbp[3], pc[3], pd[3];
+            K.powers(e, pa, pbp, pc, pd);
+            const double we = q.pref * W[e];
+            for (int k = 0; k < nr; ++k) {
+              const double s = we * S.rw[K.NR + r0 + k];
+              double v0[3], dA[3], dB[3], dC[3];
+              for (int ax = 0; ax < 3; ++ax) {
+                const double *tx = tab + (3 * k + ax) * TR;
+                const int oo = pa[ax] * SA + pbp[ax] * SB + pc[ax] * SC + pd[ax];
+                v0[ax] = tx[oo];
+                // d/dA x^a = 2 alpha x^{a+1} - a x^{a-1}; the lowered read stays
+                // in bounds (offset 0 when a = 0, weighted by a = 0)
+                dA[ax] = ta * tx[oo + SA] - double(pa[ax]) * tx[oo - (pa[ax] ? SA : 0)];
+                dB[ax] = tb * tx[oo + SB] - double(pbp[ax]) * tx[oo - (pbp[ax] ? SB : 0)];
+                dC[ax] = tc * tx[oo + SC] - double(pc[ax]) * tx[oo - (pc[ax] ? SC : 0)];
+              }
+              const double yz = v0[1] * v0[2], xz = v0[0] * v0[2], xy = v0[0] * v0[1];
+              gs[0] += s * dA[0] * yz;
+              gs[1] += s * dA[1] * xz;
+              gs[2] += s * dA[2] * xy;
+              gs[3] += s * dB[0] * yz;
+              gs[4] += s * dB[1] * xz;
+              gs[5] += s * dB[2] * xy;
+              gs[6] += s * dC[0] * yz;
+              gs[7] += s * dC[1] * xz;
+              gs[8] += s * dC[2] * xy;
+            }
+          }
+          __syncthreads();
+        }
+      }
+    if (threadIdx.x == 0) {
+      ++my_q;
+      my_p += (unsigned long long)nbp * nkp;
+    }
+    for (int x = 0; x < 9; ++x) {
+      const double v = wave_sum_d(gs[x]);
+      if (lane == 0) S.red[x * kBigWaves + wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+      double s = 0.0;
+      for (int w = 0; w < kBigWaves; ++w) s += S.red[threadIdx.x * kBigWaves + w];
+      const int which = threadIdx.x / 3, ax = threadIdx.x % 3;
+      const int at = which == 0 ? atA : (which == 1 ? atB : atC);
+      atomicAdd(gd.out + 3 * at + ax, s);
+      atomicAdd(gd.out + 3 * atD + ax, -s);
+    }
+    __syncthreads();
+  }
+  if (counters && threadIdx.x == 0) {
+    atomicAdd(counters, my_q);
+    atomicAdd(counters + 1, my_p);
+  }
+}
+
+// Workgroups of one launch: one per quartet up to kBigMaxWG, fewer when the
+// slots (slot doubles each) would exceed kBigScratchBytes.
+int big_grid(long long total, size_t slot) {
+  long long cap = (long long)(kBigScratchBytes / (slot * sizeof(double)));
+  cap = cap < 1 ? 1 : (cap > kBigMaxWG ? kBigMaxWG : cap);
+  const long long g = total < cap ? total : cap;
+  return int(g < 1 ? 1 : g);
+}
+
+// Roots per table batch and the dynamic LDS bytes for extents (la..ld) and
+// nr roots: as many roots as fit the budget, at least one.
+void root_batch(int la, int lb, int lc, int ld, int nr, int &rb, size_t &bytes) {
+  const size_t per = size_t(3) * (table_size(la, lb, lc, ld) + table_work(la, lb, lc, ld)) *
+                     sizeof(double);
+  rb = int(kBigLdsBudget / per);
+  rb = rb < 1 ? 1 : (rb > nr ? nr : rb);
+  bytes = per * rb;
+}
+
+}  // namespace
+
+size_t big_scratch_elems(int la, int lb, int lc, int ld, long long total, int two_halves) {
+  const size_t slot = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld) * (two_halves ? 2 : 1);
+  return size_t(big_grid(total, slot)) * slot;
+}
+
+hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                         const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
+                         double *scratch, unsigned long long *ctr, hipStream_t s) {
+  if (qs.total <= 0) return hipSuccess;
+  const size_t slot = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
+  int rb;
+  size_t lds;
+  root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
+  hipLaunchKernelGGL(jk_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la, lb,
+                     lc, ld, rb, b, p, r, qs, d, omega, scratch, slot, ctr);
+  return hipGetLastError();
+}
+
+hipError_t launch_eri_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                          const RysDev &r, const QuartetSet &qs, double *out, double *scratch,
+                          hipStream_t s) {
+  if (qs.total <= 0) return hipSuccess;
+  const size_t slot = 2 * size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
+  int rb;
+  size_t lds;
+  root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
+  hipLaunchKernelGGL(eri_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la, lb,
+                     lc, ld, rb, b, p, r, qs, out, scratch, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_schwarz_big(int la, int lb, const BasisDev &b, const PairsDev &p,
+                              const RysDev &r, int first, int count, double *q, double *scratch,
+                              hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const size_t slot = size_t(ncart(la)) * ncart(lb) * ncart(la) * ncart(lb);
+  int rb;
+  size_t lds;
+  root_batch(la, lb, la, lb, la + lb + 1, rb, lds);
+  hipLaunchKernelGGL(schwarz_big_kernel, dim3(big_grid(count, slot)), dim3(kBigThreads), lds, s, la, lb,
+                     rb, b, p, r, first, count, q, scratch, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_grad_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                           const RysDev &r, const QuartetSet &qs, const GradDev &g,
+                           double *scratch, unsigned long long *ctr, hipStream_t s) {
+  if (qs.total <= 0) return hipSuccess;
+  const size_t slot = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
+  int rb;
+  size_t lds;
+  root_batch(la + 1, lb + 1, lc + 1, ld, (la + lb + lc + ld + 1) / 2 + 1, rb, lds);
+  hipLaunchKernelGGL(grad_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la,
+                     lb, lc, ld, rb, b, p, r, qs, g, scratch, slot, ctr);
+  return hipGetLastError();
+}
+
+}  // namespace ig

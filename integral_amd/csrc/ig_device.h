@@ -17,20 +17,28 @@ __device__ __forceinline__ void team_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Half-c0 Clenshaw sum of a 17-term Chebyshev series (math/rys.rs:259-269).
-__device__ __forceinline__ double cheb17(const double *__restrict__ c, double xi) {
-  double b1 = 0.0, b2 = 0.0, two = 2.0 * xi;
+// Degree-12 polynomial of one Rys series on one cell, power basis in the
+// cell variable xi in [-1, 1], by Horner.
+__device__ __forceinline__ double rys_poly(const double *__restrict__ c, double xi) {
+  double r = c[kRysTerms - 1];
 #pragma unroll
-  for (int k = kChebTerms - 1; k >= 1; --k) {
-    double t = two * b1 - b2 + c[k];
-    b2 = b1;
-    b1 = t;
-  }
-  return xi * b1 - b2 + 0.5 * c[0];
+  for (int k = kRysTerms - 2; k >= 0; --k) r = fma(r, xi, c[k]);
+  return r;
+}
+
+// Cell index and cell variable of T on the n-point table: v = T * nsub / hi,
+// cell s = floor(v) (clamped to the last cell), xi = 2 (v - s) - 1.
+__device__ __forceinline__ int rys_cell(const RysDev &R, int n, double T, double &xi) {
+  const int nsub = R.nsub[n];
+  const double v = T * (double(nsub) / (33.0 + 2.5 * double(2 * n - 1)));
+  int s = int(v);
+  s = s < nsub ? s : nsub - 1;
+  xi = 2.0 * (v - double(s)) - 1.0;
+  return s;
 }
 
 // Series `idx` of the n-point Rys rule at T: idx < n is squared root idx,
-// otherwise weight idx - n.  Finite branch: Chebyshev cell lookup; T at or
+// otherwise weight idx - n.  Finite branch: polynomial cell lookup; T at or
 // beyond 33 + 2.5(2n-1): scaled Gauss-Laguerre(-1/2) (math/rys.rs:174-185).
 __device__ __forceinline__ double rys_series(const RysDev &R, int n, double T, int idx) {
   const double hi = 33.0 + 2.5 * double(2 * n - 1);
@@ -38,16 +46,13 @@ __device__ __forceinline__ double rys_series(const RysDev &R, int n, double T, i
   const int i = is_w ? idx - n : idx;
   if (T >= hi) {
     const double *L = R.buf + R.lag_off[n];
-    return is_w ? L[n + i] * (0.5 / sqrt(T)) : L[i] * (1.0 / T);
+    const double rs = rsqrt(T);
+    return is_w ? L[n + i] * (0.5 * rs) : L[i] * (rs * rs);
   }
-  const int nsub = R.nsub[n];
-  const double width = hi / double(nsub);
-  int s = int(T / width);
-  s = s < nsub ? s : nsub - 1;
-  const double a = double(s) * width, b = a + width;
-  const double xi = (2.0 * T - (a + b)) / (b - a);
-  const double *c = R.buf + (is_w ? R.wt_off[n] : R.root_off[n]) + (s * n + i) * kChebTerms;
-  return cheb17(c, xi);
+  double xi;
+  const int s = rys_cell(R, n, T, xi);
+  const double *c = R.buf + (is_w ? R.wt_off[n] : R.root_off[n]) + (s * n + i) * kRysTerms;
+  return rys_poly(c, xi);
 }
 
 // Rys 2D integrals of one axis for one root: I[i][j][k][l], i<=LA, j<=LB,

@@ -12,7 +12,7 @@ namespace ig {
 constexpr int kMaxL = 6;        // engine/os.rs:34
 constexpr int kMaxGradL = 5;    // grad.rs:50
 constexpr int kMaxRys = 13;     // math/rys.rs:104
-constexpr int kChebTerms = 17;  // degree-16 Chebyshev series
+constexpr int kRysTerms = 13;  // Rys table polynomials: degree 12 per cell
 constexpr int kMaxDens = 64;    // direct.rs:145-148
 constexpr int kMaxKernelL = 2;  // shells up to d have compiled quartet kernels in this build
 
@@ -70,8 +70,9 @@ struct PairsDev {
   const int *canon;      // canonical pair index i(i+1)/2+j (i >= j)
   const double *q;       // Schwarz bound Q[i,j]
   const int *pp_off, *pp_n;  // primitive-pair range
-  // primitive pairs, structure of arrays
-  const double *zeta, *px, *py, *pz, *kc, *alpha, *beta;
+  // primitive pairs, structure of arrays; kz = kc / zeta and i2z = 1 / (2 zeta)
+  // spare the kernels their divisions
+  const double *zeta, *px, *py, *pz, *kc, *alpha, *beta, *kz, *i2z;
 };
 
 // One launch covers the quartets {P, Q} with P in bra class X and Q in ket
@@ -113,14 +114,35 @@ struct GradDev {
 bool class_supported(int la, int lb, int lc, int ld);
 hipError_t launch_schwarz(int lcls1, int lcls2, const BasisDev &b, const PairsDev &p,
                           const RysDev &r, int first, int count, double *qout, hipStream_t s);
+// J/K kernel variants of a compiled class: one quartet per team of lanes
+// with LDS tables, or one quartet per lane in registers (sliced blocks)
+enum JkVariant { kJkTeam = 0, kJkReg = 1, kJkNumVariants = 2 };
+int jk_default_variant(int la, int lb, int lc, int ld);
 hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                      const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
-                     unsigned long long *counters, hipStream_t s);
+                     int variant, unsigned long long *counters, hipStream_t s);
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                       const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s);
 hipError_t launch_grad(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                        const RysDev &r, const QuartetSet &qs, const GradDev &g,
                        unsigned long long *counters, hipStream_t s);
+
+// runtime-l launchers (ig_big.hip) for classes class_supported() rejects;
+// `scratch` holds big_scratch_elems(...) doubles (one block slot per
+// workgroup; two per workgroup for the dense ERI, which transforms in place)
+size_t big_scratch_elems(int la, int lb, int lc, int ld, long long total, int two_halves);
+hipError_t launch_schwarz_big(int la, int lb, const BasisDev &b, const PairsDev &p,
+                              const RysDev &r, int first, int count, double *qout,
+                              double *scratch, hipStream_t s);
+hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                         const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
+                         double *scratch, unsigned long long *counters, hipStream_t s);
+hipError_t launch_eri_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                          const RysDev &r, const QuartetSet &qs, double *out, double *scratch,
+                          hipStream_t s);
+hipError_t launch_grad_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                           const RysDev &r, const QuartetSet &qs, const GradDev &g,
+                           double *scratch, unsigned long long *counters, hipStream_t s);
 
 // small dense helpers
 // ds/da: ndens x nao^2 symmetric/antisymmetric parts (da used only when

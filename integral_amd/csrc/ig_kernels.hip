@@ -99,13 +99,14 @@ __device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R
   for (int ib = 0; ib < nbp; ++ib) {
     const double pz = p.zeta[ob + ib];
     const double P[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
-    const double kab = p.kc[ob + ib];
+    const double kzp = kTwoPi52 * p.kz[ob + ib], i2p = p.i2z[ob + ib];
     for (int ik = 0; ik < nkp; ++ik) {
       const double qz = p.zeta[ok + ik];
       const double Q[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
-      const double kcd = p.kc[ok + ik];
+      const double kzq = p.kz[ok + ik], i2q = p.i2z[ok + ik];
       const double pq = pz + qz;
-      const double rho = pz * qz / pq;
+      const double rs = rsqrt(pq), ipq = rs * rs;
+      const double rho = pz * qz * ipq;
       double PQ[3], r2 = 0.0;
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
@@ -113,7 +114,7 @@ __device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R
         r2 += PQ[x] * PQ[x];
       }
       double Tq = rho * r2;
-      double pref = kTwoPi52 / (pz * qz * sqrt(pq)) * kab * kcd;
+      double pref = kzp * kzq * rs;
       double s_att = 1.0;
       if (omega > 0.0) {  // erf(w r)/r: roots/weights at sT, x -> s x, w -> sqrt(s) w
         s_att = omega * omega / (rho + omega * omega);
@@ -132,12 +133,13 @@ __device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R
         // per-axis scalars by select, keeping the small arrays in registers
         auto pick = [ax](const double *v) { return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]); };
         const double x = rw[r];
-        const double b00 = x / (2.0 * pq);
-        const double b10 = (1.0 - x * qz / pq) / (2.0 * pz);
-        const double b01 = (1.0 - x * pz / pq) / (2.0 * qz);
+        const double fq = x * qz * ipq, fp = x * pz * ipq;
+        const double b00 = 0.5 * x * ipq;
+        const double b10 = (1.0 - fq) * i2p;
+        const double b01 = (1.0 - fp) * i2q;
         const double pqa = pick(PQ);
-        const double c00 = (pick(P) - pick(g.A)) - (x * qz / pq) * pqa;
-        const double cp0 = (pick(Q) - pick(g.C)) + (x * pz / pq) * pqa;
+        const double c00 = (pick(P) - pick(g.A)) - fq * pqa;
+        const double cp0 = (pick(Q) - pick(g.C)) + fp * pqa;
         rys_axis_table<C::LA, C::LB, C::LC, C::LD>(c00, cp0, b10, b01, b00, pick(AB), pick(CD),
                                                    tab + task * T);
       }
@@ -628,12 +630,12 @@ __global__ void __launch_bounds__(64) grad_kernel(BasisDev b, PairsDev p, RysDev
     for (int ib = 0; ib < nbp; ++ib) {
       const double pz = p.zeta[ob + ib], al = p.alpha[ob + ib], be = p.beta[ob + ib];
       const double Pv[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
-      const double kab = p.kc[ob + ib];
+      const double kzp = kTwoPi52 * p.kz[ob + ib], i2p = p.i2z[ob + ib];
       for (int ik = 0; ik < nkp; ++ik) {
         const double qz = p.zeta[ok + ik], ga_ = p.alpha[ok + ik];
         const double Qv[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
-        const double kcd = p.kc[ok + ik];
-        const double pq = pz + qz, rho = pz * qz / pq;
+        const double kzq = p.kz[ok + ik], i2q = p.i2z[ok + ik];
+        const double pq = pz + qz, rs = rsqrt(pq), ipq = rs * rs, rho = pz * qz * ipq;
         double PQ[3], r2 = 0.0;
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
@@ -641,7 +643,7 @@ __global__ void __launch_bounds__(64) grad_kernel(BasisDev b, PairsDev p, RysDev
           r2 += PQ[x] * PQ[x];
         }
         double Tq = rho * r2;
-        double pref = kTwoPi52 / (pz * qz * sqrt(pq)) * kab * kcd;
+        double pref = kzp * kzq * rs;
         double s_att = 1.0;
         if (gd.omega > 0.0) {
           s_att = gd.omega * gd.omega / (rho + gd.omega * gd.omega);
@@ -657,12 +659,13 @@ __global__ void __launch_bounds__(64) grad_kernel(BasisDev b, PairsDev p, RysDev
           const int r = task / 3, ax = task - 3 * r;
           auto pick = [ax](const double *v) { return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]); };
           const double x = rw[r];
-          const double b00 = x / (2.0 * pq);
-          const double b10 = (1.0 - x * qz / pq) / (2.0 * pz);
-          const double b01 = (1.0 - x * pz / pq) / (2.0 * qz);
+          const double fq = x * qz * ipq, fp = x * pz * ipq;
+          const double b00 = 0.5 * x * ipq;
+          const double b10 = (1.0 - fq) * i2p;
+          const double b01 = (1.0 - fp) * i2q;
           const double pqa = pick(PQ);
-          const double c00 = (pick(Pv) - pick(g.A)) - (x * qz / pq) * pqa;
-          const double cp0 = (pick(Qv) - pick(g.C)) + (x * pz / pq) * pqa;
+          const double c00 = (pick(Pv) - pick(g.A)) - fq * pqa;
+          const double cp0 = (pick(Qv) - pick(g.C)) + fp * pqa;
           rys_axis_table<C::LA + 1, C::LB + 1, C::LC + 1, C::LD>(c00, cp0, b10, b01, b00,
                                                                  pick(AB), pick(CD), tab + task * T);
         }
@@ -760,20 +763,19 @@ inline int grid_for(long long total) {
 
 template <int LA, int LB, int LC, int LD>
 hipError_t jk_one(const BasisDev &b, const PairsDev &p, const RysDev &r, const QuartetSet &qs,
-                  const DensDev &d, double omega, unsigned long long *ctr, hipStream_t s) {
+                  const DensDev &d, double omega, int variant, unsigned long long *ctr,
+                  hipStream_t s) {
   using C = Cls<LA, LB, LC, LD>;
   if (qs.total <= 0) return hipSuccess;
-  if constexpr (C::N4 <= kRegBlockMax) {
-    long long blocks = (qs.total + 63) / 64;
-    if (blocks > 8192) blocks = 8192;
-    const size_t lds = size_t(2) * r.nsub[C::NR] * C::NR * kChebTerms * sizeof(double);
-    hipLaunchKernelGGL(jk_reg_kernel<C>, dim3(int(blocks)), dim3(64), lds, s, b, p, r, qs, d,
-                       omega, ctr);
-  } else {
-    hipLaunchKernelGGL(jk_kernel<C>, dim3(grid_for<LA, LB, LC, LD>(qs.total)), dim3(64), 0, s,
-                       b, p, r, qs, d, omega, ctr);
+  if (variant == kJkTeam) {
+    hipLaunchKernelGGL(jk_kernel<C>, dim3(grid_for<LA, LB, LC, LD>(qs.total)), dim3(64), 0, s, b,
+                       p, r, qs, d, omega, ctr);
+    return hipGetLastError();
   }
-  return hipGetLastError();
+  long long blocks = (qs.total + 63) / 64;
+  if (blocks > 8192) blocks = 8192;
+  const size_t lds = size_t(2) * r.nsub[C::NR] * C::NR * kRysTerms * sizeof(double);
+  return launch_jk_reg_slices<C, 0>(b, p, r, qs, d, omega, ctr, s, int(blocks), lds);
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -813,11 +815,19 @@ bool class_supported(int la, int lb, int lc, int ld) {
          pair_class(la, lb) >= pair_class(lc, ld);
 }
 
+int jk_default_variant(int la, int lb, int lc, int ld) {
+  // register slices while a quartet needs at most two of them (each slice
+  // rebuilds the quartet's 2D tables), team kernels for larger blocks
+  const int n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+  return n4 <= 2 * kRegBlockMax ? kJkReg : kJkTeam;
+}
+
 hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                      const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
-                     unsigned long long *ctr, hipStream_t s) {
-#define IG_CASE(A, B, Cc, D) \
-  if (la == A && lb == B && lc == Cc && ld == D) return jk_one<A, B, Cc, D>(b, p, r, qs, d, omega, ctr, s);
+                     int variant, unsigned long long *ctr, hipStream_t s) {
+#define IG_CASE(A, B, Cc, D)                       \
+  if (la == A && lb == B && lc == Cc && ld == D) \
+    return jk_one<A, B, Cc, D>(b, p, r, qs, d, omega, variant, ctr, s);
   IG_FOR_CLASSES(IG_CASE)
 #undef IG_CASE
   return hipErrorInvalidValue;

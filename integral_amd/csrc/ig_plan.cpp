@@ -221,12 +221,17 @@ struct Plan {
   std::vector<int> canon_to_pos;  // canonical pair index -> sorted position
   ClassRange cls[kNumPairClass];
   DBuf d_sh1, d_sh2, d_canon, d_q, d_ppoff, d_ppn;
-  DBuf d_zeta, d_px, d_py, d_pz, d_kc, d_alpha, d_beta;
+  DBuf d_zeta, d_px, d_py, d_pz, d_kc, d_alpha, d_beta, d_kz, d_i2z;
   PairsDev pdev{};
   std::vector<double> schwarz;  // nsh x nsh (function space)
   std::mutex mu;                // serialises calls that use the scratch below
   DBuf s_dfunc, s_ds, s_da, s_dc, s_bound, s_jc, s_kc, s_J, s_K, s_eff, s_rowptr, s_list,
       s_counters, s_flag;
+  DBuf s_big;  // block slots of the runtime-l kernels (classes beyond the compiled ones)
+  // J/K kernel variant per (bra class, ket class): -1 = the built-in default,
+  // else set by the autotuner or a loaded schedule (with its measured time)
+  std::vector<int8_t> variant = std::vector<int8_t>(kNumPairClass * kNumPairClass, -1);
+  std::vector<float> variant_ms = std::vector<float>(kNumPairClass * kNumPairClass, 0.0f);
   double cached_tau = NAN;
   std::vector<long long> cached_rowptr_host;
   struct CP {
@@ -365,7 +370,7 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
   const int64_t np = pair_count(nsh);
   P->npairs = int(np);
   // primitive pairs (os_eri.rs:221-249 quantities; K c1 c2 folded together)
-  std::vector<double> zeta, px, py, pz, kc, al, be;
+  std::vector<double> zeta, px, py, pz, kc, al, be, kz, i2z;
   struct Tmp {
     int sh1, sh2, canon, off, n, cls;
   };
@@ -393,6 +398,8 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
         py.push_back((e1 * s1->center[1] + e2 * s2->center[1]) / z);
         pz.push_back((e1 * s1->center[2] + e2 * s2->center[2]) / z);
         kc.push_back(w);
+        kz.push_back(w / z);
+        i2z.push_back(0.5 / z);
         al.push_back(e1);
         be.push_back(e2);
         ++t.n;
@@ -406,6 +413,10 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
   P->d_kc.upload(kc);
   P->d_alpha.upload(al);
   P->d_beta.upload(be);
+  P->d_kz.upload(kz);
+  P->d_i2z.upload(i2z);
+  P->pdev.kz = P->d_kz.as<double>();
+  P->pdev.i2z = P->d_i2z.as<double>();
   P->pdev.zeta = P->d_zeta.as<double>();
   P->pdev.px = P->d_px.as<double>();
   P->pdev.py = P->d_py.as<double>();
@@ -445,11 +456,6 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
       ClassRange &r = P->cls[pair_class(l1, l2)];
       r.l1 = l1;
       r.l2 = l2;
-      if (r.n && !class_supported(l1, l2, l1, l2))
-        throw DeviceError{IG_ERR_UNSUPPORTED,
-                          "shell pairs with l=(" + std::to_string(l1) + "," + std::to_string(l2) +
-                              ") have no compiled quartet kernel in this build (max l = " +
-                              std::to_string(kMaxKernelL) + ")"};
     }
   P->p_q.assign(np, 0.0);
   upload_pairs(*P);
@@ -457,11 +463,23 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
   DBuf dq;
   dq.reserve(np * sizeof(double));
   RysDevice &R = rys_device();
+  size_t big = 0;
+  for (int c = 0; c < kNumPairClass; ++c) {
+    const ClassRange &r = P->cls[c];
+    if (r.n && !class_supported(r.l1, r.l2, r.l1, r.l2))
+      big = std::max(big, big_scratch_elems(r.l1, r.l2, r.l1, r.l2, r.n, 0));
+  }
+  P->s_big.reserve(big * sizeof(double));
   for (int c = 0; c < kNumPairClass; ++c) {
     const ClassRange &r = P->cls[c];
     if (!r.n) continue;
-    check(launch_schwarz(r.l1, r.l2, P->bdev, P->pdev, R.view, r.off, r.n, dq.as<double>(), s),
-          "schwarz launch");
+    if (class_supported(r.l1, r.l2, r.l1, r.l2))
+      check(launch_schwarz(r.l1, r.l2, P->bdev, P->pdev, R.view, r.off, r.n, dq.as<double>(), s),
+            "schwarz launch");
+    else
+      check(launch_schwarz_big(r.l1, r.l2, P->bdev, P->pdev, R.view, r.off, r.n, dq.as<double>(),
+                               P->s_big.as<double>(), s),
+            "schwarz launch");
   }
   std::vector<double> q(np);
   check(hipMemcpyAsync(q.data(), dq.p, np * sizeof(double), hipMemcpyDeviceToHost, s), "copy Q");
@@ -633,9 +651,42 @@ int check_engine(int engine) {
 
 hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
 
-// Run J/K for ndens densities already on the device (d_dens, function space).
-void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau, bool want_k,
-            double *d_J, double *d_K, const ExplicitSets *ex, hipStream_t s) {
+// Classes beyond the compiled kernels run on the runtime-l kernels, whose
+// block slots live in P.s_big; every launch of a call is sized before the
+// first one is queued, so the buffer never moves under a running kernel.
+size_t big_need(const ClassRange &X, const ClassRange &Y, long long total, int two_halves) {
+  if (total <= 0 || class_supported(X.l1, X.l2, Y.l1, Y.l2)) return 0;
+  return big_scratch_elems(X.l1, X.l2, Y.l1, Y.l2, total, two_halves);
+}
+
+int jk_variant(const Plan &P, const ClassRange &X, const ClassRange &Y) {
+  const int v = P.variant[pair_class(X.l1, X.l2) * kNumPairClass + pair_class(Y.l1, Y.l2)];
+  return v >= 0 ? v : jk_default_variant(X.l1, X.l2, Y.l1, Y.l2);
+}
+
+hipError_t jk_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const QuartetSet &qs,
+                     const DensDev &dd, unsigned long long *ctr, hipStream_t s, int variant = -1) {
+  RysDevice &R = rys_device();
+  if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
+    return launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0,
+                     variant >= 0 ? variant : jk_variant(P, X, Y), ctr, s);
+  return launch_jk_big(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0,
+                       P.s_big.as<double>(), ctr, s);
+}
+
+hipError_t grad_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const QuartetSet &qs,
+                       const GradDev &gd, unsigned long long *ctr, hipStream_t s) {
+  RysDevice &R = rys_device();
+  if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
+    return launch_grad(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd, ctr, s);
+  return launch_grad_big(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd,
+                         P.s_big.as<double>(), ctr, s);
+}
+
+// Density preparation of a J/K call: symmetric/antisymmetric split, Cartesian
+// densities, shell-block bounds and zeroed Cartesian J'/K' accumulators.
+DensDev prep_densities(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
+                       bool want_k, hipStream_t s) {
   const size_t n2 = size_t(P.nao) * P.nao, nc2 = size_t(P.nao_cart) * P.nao_cart;
   const int neff = ndens * (has_anti ? 2 : 1);
   P.s_ds.reserve(n2 * ndens * sizeof(double));
@@ -653,7 +704,6 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
     P.s_eff.upload(eff);
     P.eff_key = neff * 128 + ndens;
   }
-  begin_launches(P, s);
   check(launch_density_prep(P.bdev, d_dens, ndens, has_anti, P.s_ds.as<double>(),
                             P.s_da.as<double>(), P.s_dc.as<double>(), P.s_bound.as<double>(), s),
         "density prep");
@@ -670,17 +720,26 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
   dd.bound = P.s_bound.as<double>();
   dd.tau = tau;
   dd.want_k = want_k ? 1 : 0;
-  RysDevice &R = rys_device();
+  return dd;
+}
+
+// Run J/K for ndens densities already on the device (d_dens, function space).
+void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau, bool want_k,
+            double *d_J, double *d_K, const ExplicitSets *ex, hipStream_t s) {
+  begin_launches(P, s);
+  const DensDev dd = prep_densities(P, d_dens, ndens, has_anti, tau, want_k, s);
   if (!ex) {
     build_rowptrs(P, tau);
+    size_t big = 0;
+    for (auto &cp : P.cached_cps) big = std::max(big, big_need(P.cls[cp.x], P.cls[cp.y], cp.total, 0));
+    P.s_big.reserve(big * sizeof(double));
     // heavy classes first so the long launches start early
     for (auto it = P.cached_cps.rbegin(); it != P.cached_cps.rend(); ++it) {
       if (!it->total) continue;
       QuartetSet qs = implicit_set(P, *it);
       const ClassRange &X = P.cls[it->x], &Y = P.cls[it->y];
       unsigned long long *ctr = next_launch(P, X, Y, s);
-      check(launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, ctr, s),
-            "jk launch");
+      check(jk_launch(P, X, Y, qs, dd, ctr, s), "jk launch");
       end_launch(P, s);
     }
   } else {
@@ -690,6 +749,11 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
     flat.reserve(total);
     for (auto &l : ex->lists) flat.insert(flat.end(), l.begin(), l.end());
     P.s_list.upload(flat);
+    size_t big = 0;
+    for (size_t c = 0; c < ex->cls.size(); ++c)
+      big = std::max(big, big_need(P.cls[ex->cls[c].first], P.cls[ex->cls[c].second],
+                                   (long long)ex->lists[c].size(), 0));
+    P.s_big.reserve(big * sizeof(double));
     size_t off = 0;
     for (size_t c = 0; c < ex->cls.size(); ++c) {
       const ClassRange &X = P.cls[ex->cls[c].first], &Y = P.cls[ex->cls[c].second];
@@ -703,8 +767,7 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
       qs.total = (long long)ex->lists[c].size();
       off += ex->lists[c].size();
       unsigned long long *ctr = next_launch(P, X, Y, s);
-      check(launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, ctr, s),
-            "jk launch");
+      check(jk_launch(P, X, Y, qs, dd, ctr, s), "jk launch");
       end_launch(P, s);
     }
   }
@@ -877,6 +940,125 @@ extern "C" int ig_direct_jk_device(const ig_direct *d, const double *d_dens, siz
   });
 }
 
+extern "C" int ig_direct_autotune(ig_direct *d, const double *d_dens, size_t ndens,
+                                  double threshold, int want_k, int reps, void *stream) {
+  if (!d || !d_dens) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (ndens == 0) return fail(IG_ERR_NO_DENSITY, "direct contraction needs at least one density");
+  if (ndens > size_t(kMaxDens))
+    return fail(IG_ERR_TOO_MANY_DENSITIES, "at most 64 simultaneous densities are supported");
+  if (reps < 1) return fail(IG_ERR_INVALID_ARGUMENT, "reps must be at least 1");
+  return guarded([&] {
+    Plan &P = *d->plan;
+    std::lock_guard<std::mutex> g(P.mu);
+    hipStream_t s = as_stream(stream);
+    finalize_stats(P);
+    P.s_flag.reserve(sizeof(int));
+    check(launch_asym_flag(d_dens, P.nao, int(ndens), P.s_flag.as<int>(), s), "symmetry check");
+    int asym = 0;
+    check(hipMemcpyAsync(&asym, P.s_flag.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flag");
+    check(hipStreamSynchronize(s), "sync");
+    // the accumulators of the trial launches are scratch: the caller's J/K
+    // are not touched
+    const DensDev dd = prep_densities(P, d_dens, int(ndens), asym != 0, threshold, want_k != 0, s);
+    build_rowptrs(P, threshold);
+    P.s_counters.reserve(2 * sizeof(unsigned long long));
+    auto *ctr = P.s_counters.as<unsigned long long>();
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    try {
+      for (auto &cp : P.cached_cps) {
+        const ClassRange &X = P.cls[cp.x], &Y = P.cls[cp.y];
+        if (!cp.total || !class_supported(X.l1, X.l2, Y.l1, Y.l2)) continue;
+        const QuartetSet qs = implicit_set(P, cp);
+        int best = -1;
+        float best_ms = 0.0f;
+        for (int v = 0; v < kJkNumVariants; ++v) {
+          float vmin = -1.0f;
+          for (int r = 0; r < reps; ++r) {
+            check(hipEventRecord(e0, s), "event");
+            check(jk_launch(P, X, Y, qs, dd, ctr, s, v), "jk launch");
+            check(hipEventRecord(e1, s), "event");
+            check(hipEventSynchronize(e1), "event sync");
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, e0, e1), "event time");
+            if (vmin < 0.0f || ms < vmin) vmin = ms;
+          }
+          if (best < 0 || vmin < best_ms) {
+            best = v;
+            best_ms = vmin;
+          }
+        }
+        const int key = cp.x * kNumPairClass + cp.y;
+        P.variant[key] = int8_t(best);
+        P.variant_ms[key] = best_ms;
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P.launches.clear();
+    P.stats_pending = false;
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_direct_schedule_read(const ig_direct *d, ig_schedule_entry *out, size_t cap,
+                                       size_t *n) {
+  if (!d || !n || (cap && !out)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  return guarded([&] {
+    Plan &P = *d->plan;
+    std::lock_guard<std::mutex> g(P.mu);
+    size_t k = 0;
+    for (int x = 0; x < kNumPairClass; ++x)
+      for (int y = 0; y <= x; ++y) {
+        const ClassRange &X = P.cls[x], &Y = P.cls[y];
+        if (!X.n || !Y.n || !class_supported(X.l1, X.l2, Y.l1, Y.l2)) continue;
+        if (k < cap) {
+          const int key = x * kNumPairClass + y;
+          out[k].la = X.l1;
+          out[k].lb = X.l2;
+          out[k].lc = Y.l1;
+          out[k].ld = Y.l2;
+          out[k].variant = jk_variant(P, X, Y);
+          out[k].tuned = P.variant[key] >= 0 ? 1 : 0;
+          out[k].ms = P.variant_ms[key];
+        }
+        ++k;
+      }
+    *n = k;
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_direct_schedule_write(ig_direct *d, const ig_schedule_entry *in, size_t n) {
+  if (!d || (n && !in)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  for (size_t k = 0; k < n; ++k) {
+    const ig_schedule_entry &e = in[k];
+    const bool ok_l = e.la >= 0 && e.lb >= 0 && e.lc >= 0 && e.ld >= 0 && e.la <= kMaxL &&
+                      e.lc <= kMaxL && e.lb <= e.la && e.ld <= e.lc;
+    if (!ok_l || !class_supported(e.la, e.lb, e.lc, e.ld))
+      return fail(IG_ERR_INVALID_ARGUMENT,
+                  "schedule entry " + std::to_string(k) + " names a class without compiled kernels");
+    if (e.variant < -1 || e.variant >= kJkNumVariants)
+      return fail(IG_ERR_INVALID_ARGUMENT,
+                  "schedule entry " + std::to_string(k) + " has an unknown variant");
+  }
+  return guarded([&] {
+    Plan &P = *d->plan;
+    std::lock_guard<std::mutex> g(P.mu);
+    for (size_t k = 0; k < n; ++k) {
+      const int key = pair_class(in[k].la, in[k].lb) * kNumPairClass + pair_class(in[k].lc, in[k].ld);
+      P.variant[key] = int8_t(in[k].variant);
+      P.variant_ms[key] = float(in[k].variant >= 0 ? in[k].ms : 0.0);
+    }
+    return int(IG_OK);
+  });
+}
+
 extern "C" int ig_direct_set_profiling(ig_direct *d, int enable) {
   if (!d) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> g(d->plan->mu);
@@ -946,12 +1128,22 @@ static int eri_impl(const ig_basis *b, double tau, double *out, size_t len, int6
     check(launch_fill(dout.as<double>(), len, 0.0, s), "zero eri");
     build_rowptrs(*P, tau < 0 ? 0.0 : tau);
     RysDevice &R = rys_device();
+    size_t big = 0;
+    for (auto &cp : P->cached_cps)
+      big = std::max(big, big_need(P->cls[cp.x], P->cls[cp.y], cp.total, 1));
+    P->s_big.reserve(big * sizeof(double));
     for (auto &cp : P->cached_cps) {
       if (!cp.total) continue;
       QuartetSet qs = implicit_set(*P, cp);
       const ClassRange &X = P->cls[cp.x], &Y = P->cls[cp.y];
-      check(launch_eri(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, dout.as<double>(), s),
-            "eri launch");
+      if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
+        check(launch_eri(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, dout.as<double>(),
+                         s),
+              "eri launch");
+      else
+        check(launch_eri_big(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs,
+                             dout.as<double>(), P->s_big.as<double>(), s),
+              "eri launch");
     }
     check(hipMemcpyAsync(out, dout.p, len * sizeof(double), hipMemcpyDeviceToHost, s), "copy eri");
     check(hipStreamSynchronize(s), "eri sync");
@@ -1084,24 +1276,31 @@ static int grad_run(const ig_grad *g, const ig_grad_density *dens, int kernel, d
     gd.out = dout.as<double>();
     gd.tau = tau;
     gd.omega = kernel == IG_KERNEL_ERF ? omega : 0.0;
-    RysDevice &R = rys_device();
     auto *ctr = P.s_counters.as<unsigned long long>();
     const int64_t np = P.npairs;
     if (gd.ncoul + gd.nexch > 0) {
       if (b0 == 0 && b1 == np) {
         build_rowptrs(P, tau);
+        size_t big = 0;
+        for (auto &cp : P.cached_cps)
+          big = std::max(big, big_need(P.cls[cp.x], P.cls[cp.y], cp.total, 0));
+        P.s_big.reserve(big * sizeof(double));
         for (auto it = P.cached_cps.rbegin(); it != P.cached_cps.rend(); ++it) {
           if (!it->total) continue;
           QuartetSet qs = implicit_set(P, *it);
           const ClassRange &X = P.cls[it->x], &Y = P.cls[it->y];
-          check(launch_grad(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd, ctr, s),
-                "grad launch");
+          check(grad_launch(P, X, Y, qs, gd, ctr, s), "grad launch");
         }
       } else {
         ExplicitSets ex = explicit_sets(P, b0, b1, tau);
         std::vector<int2> flat;
         for (auto &l : ex.lists) flat.insert(flat.end(), l.begin(), l.end());
         P.s_list.upload(flat);
+        size_t big = 0;
+        for (size_t c = 0; c < ex.cls.size(); ++c)
+          big = std::max(big, big_need(P.cls[ex.cls[c].first], P.cls[ex.cls[c].second],
+                                       (long long)ex.lists[c].size(), 0));
+        P.s_big.reserve(big * sizeof(double));
         size_t off = 0;
         for (size_t c = 0; c < ex.cls.size(); ++c) {
           const ClassRange &X = P.cls[ex.cls[c].first], &Y = P.cls[ex.cls[c].second];
@@ -1114,8 +1313,7 @@ static int grad_run(const ig_grad *g, const ig_grad_density *dens, int kernel, d
           qs.list = P.s_list.as<int2>() + off;
           qs.total = (long long)ex.lists[c].size();
           off += ex.lists[c].size();
-          check(launch_grad(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd, ctr, s),
-                "grad launch");
+          check(grad_launch(P, X, Y, qs, gd, ctr, s), "grad launch");
         }
       }
     }

@@ -1,17 +1,24 @@
 // ig_reg.h -- lane-per-quartet J/K path (included inside namespace ig by
-// ig_kernels.hip).  For classes whose Cartesian block fits one lane's VGPRs:
-// Rys roots and weights from Chebyshev tables staged in LDS once per
-// workgroup, per-root 2D tables and the contracted block in registers with
-// compile-time indices, J/K digestion from registers.  Each wavefront walks a
-// contiguous range of screened quartets; lanes of a wavefront usually share
-// the bra pair, so the bra-side Coulomb updates are reduced across the
-// wavefront before one atomic per element.
+// ig_kernels.hip).  Rys roots and weights come from Chebyshev tables staged
+// in LDS once per workgroup; per-root 2D tables and the contracted block live
+// in registers with compile-time indices; J/K are digested from registers.
+//
+// Blocks larger than one lane's register budget are cut into NS element
+// slices [LO, HI) of at most kRegBlockMax elements.  Each slice is a separate
+// launch: every wavefront of a slice launch walks the same quartet range,
+// rebuilds the (cheap) 2D tables in registers and accumulates and digests only
+// its slice, so slices never communicate and the slice index is uniform per
+// wavefront (no divergence).
+//
+// Each wavefront walks a contiguous range of screened quartets; lanes of a
+// wavefront usually share the bra pair, so the bra-side Coulomb updates are
+// reduced across the wavefront before one atomic per element.
 
 // Rys coefficient tables of NR roots copied to LDS: roots then weights,
 // nsub * NR * 17 doubles each.
 template <int NR>
 __device__ __forceinline__ void stage_rys(const RysDev &R, double *lds) {
-  const int n = R.nsub[NR] * NR * kChebTerms;
+  const int n = R.nsub[NR] * NR * kRysTerms;
   const double *r = R.buf + R.root_off[NR], *w = R.buf + R.wt_off[NR];
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     lds[i] = r[i];
@@ -26,7 +33,7 @@ __device__ __forceinline__ void rys_all_lds(const RysDev &R, const double *lds, 
   const double hi = 33.0 + 2.5 * double(2 * NR - 1);
   if (T >= hi) {
     const double *L = R.buf + R.lag_off[NR];
-    const double it = 1.0 / T, sc = 0.5 / sqrt(T);
+    const double rs = rsqrt(T), it = rs * rs, sc = 0.5 * rs;
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
       x[i] = L[i] * it;
@@ -34,25 +41,21 @@ __device__ __forceinline__ void rys_all_lds(const RysDev &R, const double *lds, 
     }
     return;
   }
-  const int nsub = R.nsub[NR];
-  const double width = hi / double(nsub);
-  int s = int(T / width);
-  s = s < nsub ? s : nsub - 1;
-  const double a = double(s) * width, b = a + width;
-  const double xi = (2.0 * T - (a + b)) / (b - a);
-  const double *cr = lds + s * NR * kChebTerms;
-  const double *cw = cr + nsub * NR * kChebTerms;
+  double xi;
+  const int s = rys_cell(R, NR, T, xi);
+  const double *cr = lds + s * NR * kRysTerms;
+  const double *cw = cr + R.nsub[NR] * NR * kRysTerms;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
-    x[i] = cheb17(cr + i * kChebTerms, xi);
-    w[i] = cheb17(cw + i * kChebTerms, xi);
+    x[i] = rys_poly(cr + i * kRysTerms, xi);
+    w[i] = rys_poly(cw + i * kRysTerms, xi);
   }
 }
 
-// Contracted Cartesian block of (pb | pk) into acc[N4].  Ket primitives in
-// the outer loop (per-lane data), bra primitives inside (usually uniform
-// across the wavefront).
-template <class C>
+// Elements [LO, HI) of the contracted Cartesian block of (pb | pk) into
+// acc[e - LO].  Ket primitives in the outer loop (per-lane data), bra
+// primitives inside (usually uniform across the wavefront).
+template <class C, int LO, int HI>
 __device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDev &R,
                                                   const double *rys_lds, int pb, int pk,
                                                   const QuartetGeom &g, double omega,
@@ -70,16 +73,18 @@ __device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDe
   for (int ik = 0; ik < nkp; ++ik) {
     const double qz = p.zeta[ok + ik];
     const double Q[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
-    const double kcd = p.kc[ok + ik];
+    const double kzq = kTwoPi52 * p.kz[ok + ik], i2q = p.i2z[ok + ik];
     double QC[3];
 #pragma unroll
     for (int x = 0; x < 3; ++x) QC[x] = Q[x] - g.C[x];
     for (int ib = 0; ib < nbp; ++ib) {
       const double pz = p.zeta[ob + ib];
       const double P[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
-      const double kab = p.kc[ob + ib];
+      const double kzp = p.kz[ob + ib], i2p = p.i2z[ob + ib];
       const double pq = pz + qz;
-      const double ipq = 1.0 / pq;
+      // 2 pi^{5/2} K_ab K_cd / (zeta_p zeta_q sqrt(zeta_p + zeta_q)) from one rsqrt
+      const double rs = rsqrt(pq);
+      const double ipq = rs * rs;
       const double rho = pz * qz * ipq;
       double PQ[3], r2 = 0.0;
 #pragma unroll
@@ -88,7 +93,7 @@ __device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDe
         r2 += PQ[x] * PQ[x];
       }
       double Tq = rho * r2;
-      double pref = kTwoPi52 / (pz * qz * sqrt(pq)) * kab * kcd;
+      double pref = kzp * kzq * rs;
       double s_att = 1.0;
       if (omega > 0.0) {  // erf(w r)/r: roots/weights at sT, x -> s x, w -> sqrt(s) w
         s_att = omega * omega / (rho + omega * omega);
@@ -97,7 +102,6 @@ __device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDe
       }
       double rx[NR], rwt[NR];
       rys_all_lds<NR>(R, rys_lds, Tq, rx, rwt);
-      const double i2p = 0.5 / pz, i2q = 0.5 / qz;
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         const double x = rx[r] * s_att;
@@ -113,7 +117,8 @@ __device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDe
                                                      CD[ax], I[ax]);
         const double pw = pref * rwt[r];
 #pragma unroll
-        for (int e = 0; e < C::N4; ++e) acc[e] += pw * (I[0][O.x[e]] * I[1][O.y[e]] * I[2][O.z[e]]);
+        for (int e = LO; e < HI; ++e)
+          acc[e - LO] += pw * (I[0][O.x[e]] * I[1][O.y[e]] * I[2][O.z[e]]);
       }
     }
   }
@@ -125,15 +130,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// J/K digestion of a register block.  Called by every lane of the wavefront
-// (`on` false for lanes without a quartet).  When all lanes with work share
-// the bra shell pair, the Jp[a,b] sums are reduced across the wavefront first.
-template <class C>
+// J/K digestion of the block elements [LO, HI) held in registers.  Called by
+// every lane of the wavefront (`on` false for lanes without a quartet).  When
+// all lanes with work share the bra shell pair, the Jp[a,b] sums are reduced
+// across the wavefront first and written by `leader`, the first lane with
+// work (its geometry names that shared bra pair; a lane without work may hold
+// another one).
+template <class C, int LO, int HI>
 __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
                                            const QuartetGeom &g, bool same_pair,
                                            unsigned long long mask, bool on, bool bra_uniform,
-                                           const double *blk) {
-  constexpr int na = C::na, nb = C::nb, nc = C::nc, nd = C::nd;
+                                           int leader, const double *acc) {
+  constexpr int nb = C::nb, nc = C::nc, nd = C::nd;
+  constexpr int A0 = LO / (nb * nc * nd), A1 = (HI - 1) / (nb * nc * nd) + 1;  // a rows touched
+#define IG_E(a, bb, c, d) (((a) * nb + (bb)) * nc + (c)) * nd + (d)
+#define IG_IN(e) ((e) >= LO && (e) < HI)
   const double f = (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (same_pair ? 0.5 : 1.0);
   const size_t n = (size_t)b.nao_cart, n2 = n * n;
   const size_t a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
@@ -149,9 +160,11 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
 #pragma unroll
       for (int q = 0; q < nc * nd; ++q) jcd[q] = 0.0;
 #pragma unroll
-      for (int a = 0; a < na; ++a)
+      for (int a = A0; a < A1; ++a)
 #pragma unroll
         for (int bb = 0; bb < nb; ++bb) {
+          // skip (a,b) rows that do not overlap the slice
+          if (IG_E(a, bb, 0, 0) >= HI || IG_E(a, bb, nc - 1, nd - 1) < LO) continue;
           double v = 0.0;
           if (use) {
             const double dab = D[(a0 + a) * n + b0 + bb];
@@ -159,14 +172,17 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
             for (int c = 0; c < nc; ++c)
 #pragma unroll
               for (int d = 0; d < nd; ++d) {
-                const double gv = blk[((a * nb + bb) * nc + c) * nd + d];
-                v += gv * D[(c0 + c) * n + d0 + d];
-                jcd[c * nd + d] += gv * dab;
+                const int ee = IG_E(a, bb, c, d);
+                if (IG_IN(ee)) {
+                  const double gv = acc[ee - LO];
+                  v += gv * D[(c0 + c) * n + d0 + d];
+                  jcd[c * nd + d] += gv * dab;
+                }
               }
           }
           if (bra_uniform) {  // f depends on the lane's ket: scale before summing
             v = wave_sum(2.0 * f * v);
-            if (lane == 0 && v != 0.0) atomicAdd(J + (a0 + a) * n + b0 + bb, v);
+            if (lane == leader && v != 0.0) atomicAdd(J + (a0 + a) * n + b0 + bb, v);
           } else if (use) {
             atomicAdd(J + (a0 + a) * n + b0 + bb, 2.0 * f * v);
           }
@@ -182,61 +198,80 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
     if (use && dd.kc && dd.want_k) {
       double *K = dd.kc + (size_t)e * n2;
 #pragma unroll
-      for (int a = 0; a < na; ++a)
+      for (int a = A0; a < A1; ++a)
 #pragma unroll
         for (int c = 0; c < nc; ++c) {  // Kp[a,c] += g D[b,d]
           double v = 0.0;
+          bool any = false;
 #pragma unroll
           for (int bb = 0; bb < nb; ++bb)
 #pragma unroll
             for (int d = 0; d < nd; ++d)
-              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * n + d0 + d];
-          atomicAdd(K + (a0 + a) * n + c0 + c, f * v);
+              if (IG_IN(IG_E(a, bb, c, d))) {
+                v += acc[IG_E(a, bb, c, d) - LO] * D[(b0 + bb) * n + d0 + d];
+                any = true;
+              }
+          if (any) atomicAdd(K + (a0 + a) * n + c0 + c, f * v);
         }
 #pragma unroll
       for (int bb = 0; bb < nb; ++bb)
 #pragma unroll
         for (int c = 0; c < nc; ++c) {  // Kp[b,c] += g D[a,d]
           double v = 0.0;
+          bool any = false;
 #pragma unroll
-          for (int a = 0; a < na; ++a)
+          for (int a = A0; a < A1; ++a)
 #pragma unroll
             for (int d = 0; d < nd; ++d)
-              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * n + d0 + d];
-          atomicAdd(K + (b0 + bb) * n + c0 + c, f * v);
+              if (IG_IN(IG_E(a, bb, c, d))) {
+                v += acc[IG_E(a, bb, c, d) - LO] * D[(a0 + a) * n + d0 + d];
+                any = true;
+              }
+          if (any) atomicAdd(K + (b0 + bb) * n + c0 + c, f * v);
         }
 #pragma unroll
-      for (int a = 0; a < na; ++a)
+      for (int a = A0; a < A1; ++a)
 #pragma unroll
         for (int d = 0; d < nd; ++d) {  // Kp[a,d] += g D[b,c]
           double v = 0.0;
+          bool any = false;
 #pragma unroll
           for (int bb = 0; bb < nb; ++bb)
 #pragma unroll
             for (int c = 0; c < nc; ++c)
-              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * n + c0 + c];
-          atomicAdd(K + (a0 + a) * n + d0 + d, f * v);
+              if (IG_IN(IG_E(a, bb, c, d))) {
+                v += acc[IG_E(a, bb, c, d) - LO] * D[(b0 + bb) * n + c0 + c];
+                any = true;
+              }
+          if (any) atomicAdd(K + (a0 + a) * n + d0 + d, f * v);
         }
 #pragma unroll
       for (int bb = 0; bb < nb; ++bb)
 #pragma unroll
         for (int d = 0; d < nd; ++d) {  // Kp[b,d] += g D[a,c]
           double v = 0.0;
+          bool any = false;
 #pragma unroll
-          for (int a = 0; a < na; ++a)
+          for (int a = A0; a < A1; ++a)
 #pragma unroll
             for (int c = 0; c < nc; ++c)
-              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * n + c0 + c];
-          atomicAdd(K + (b0 + bb) * n + d0 + d, f * v);
+              if (IG_IN(IG_E(a, bb, c, d))) {
+                v += acc[IG_E(a, bb, c, d) - LO] * D[(a0 + a) * n + c0 + c];
+                any = true;
+              }
+          if (any) atomicAdd(K + (b0 + bb) * n + d0 + d, f * v);
         }
     }
   }
+#undef IG_E
+#undef IG_IN
 }
 
-// Lane-per-quartet J/K kernel.  Every lane of a wavefront runs the same number
-// of iterations (lanes past the range or with an empty density mask compute
-// nothing) so the wavefront reductions see all 64 lanes.
-template <class C>
+// Lane-per-quartet J/K kernel for block slice [LO, HI).  Every lane of a
+// wavefront runs the same number of iterations (lanes past the range or with
+// an empty density mask compute nothing) so wavefront reductions see all 64
+// lanes.  Only slice 0 counts quartets.
+template <class C, int LO, int HI>
 __global__ void __launch_bounds__(64) jk_reg_kernel(BasisDev b, PairsDev p, RysDev R,
                                                     QuartetSet qs, DensDev dd, double omega,
                                                     unsigned long long *counters) {
@@ -271,25 +306,55 @@ __global__ void __launch_bounds__(64) jk_reg_kernel(BasisDev b, PairsDev p, RysD
       }
       on = mask != 0;
     }
-    double acc[C::N4];
+    double acc[HI - LO];
 #pragma unroll
-    for (int e = 0; e < C::N4; ++e) acc[e] = 0.0;
+    for (int e = 0; e < HI - LO; ++e) acc[e] = 0.0;
     if (on) {
-      quartet_block_reg<C>(p, R, rys_lds, pb, pk, g, omega, acc);
-      ++my_quartets;
-      my_prims += (unsigned long long)p.pp_n[pb] * p.pp_n[pk];
+      quartet_block_reg<C, LO, HI>(p, R, rys_lds, pb, pk, g, omega, acc);
+      if (LO == 0) {
+        ++my_quartets;
+        my_prims += (unsigned long long)p.pp_n[pb] * p.pp_n[pk];
+      }
     }
-    // bra pair shared by every lane with work?
-    const int pb_first = __shfl(pb, __ffsll((unsigned long long)__ballot(on)) - 1);
+    // bra pair shared by every lane with work?  (no lane with work: leader -1,
+    // nothing is written)
+    const int leader = __ffsll((unsigned long long)__ballot(on)) - 1;
+    const int pb_first = __shfl(pb, leader < 0 ? 0 : leader);
     const bool bra_uniform = __all(!on || pb == pb_first);
-    digest_reg<C>(b, dd, g, pb == pk, mask, on, bra_uniform, acc);
+    digest_reg<C, LO, HI>(b, dd, g, pb == pk, mask, on, bra_uniform, leader, acc);
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    my_quartets += __shfl_down(my_quartets, off);
-    my_prims += __shfl_down(my_prims, off);
+  if (LO == 0) {
+    for (int off = 32; off > 0; off >>= 1) {
+      my_quartets += __shfl_down(my_quartets, off);
+      my_prims += __shfl_down(my_prims, off);
+    }
+    if (counters && lane == 0) {
+      atomicAdd(counters, my_quartets);
+      atomicAdd(counters + 1, my_prims);
+    }
   }
-  if (counters && lane == 0) {
-    atomicAdd(counters, my_quartets);
-    atomicAdd(counters + 1, my_prims);
+}
+
+// number of register slices for a class and the launch of all of them
+template <class C>
+struct RegSlices {
+  static constexpr int NS = (C::N4 + kRegBlockMax - 1) / kRegBlockMax;
+  static constexpr int E = (C::N4 + NS - 1) / NS;
+};
+
+template <class C, int S>
+hipError_t launch_jk_reg_slices(const BasisDev &b, const PairsDev &p, const RysDev &r,
+                                const QuartetSet &qs, const DensDev &d, double omega,
+                                unsigned long long *ctr, hipStream_t s, int blocks, size_t lds) {
+  constexpr int NS = RegSlices<C>::NS, E = RegSlices<C>::E;
+  if constexpr (S < NS) {
+    constexpr int LO = S * E, HI = (S + 1) * E < C::N4 ? (S + 1) * E : C::N4;
+    hipLaunchKernelGGL((jk_reg_kernel<C, LO, HI>), dim3(blocks), dim3(64), lds, s, b, p, r, qs,
+                       d, omega, ctr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_jk_reg_slices<C, S + 1>(b, p, r, qs, d, omega, ctr, s, blocks, lds);
+  } else {
+    return hipSuccess;
   }
 }

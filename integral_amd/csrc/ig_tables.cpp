@@ -1,10 +1,14 @@
 // ig_tables.cpp -- host-side constant tables for the device kernels:
-//   * Rys roots/weights as piecewise degree-16 Chebyshev series in T on
+//   * Rys roots/weights as piecewise degree-12 polynomials in T on
 //     equal-width cells of [0, 33 + 2.5(2n-1)), fit to a discretized-Stieltjes
-//     Gauss rule (128-point Gauss-Legendre in u = sqrt(x)) and Golub-Welsch;
-//     the number of cells per n is the smallest that meets 2e-14 absolute on 33
-//     samples per cell.  Method and seam follow crates/integral/src/math/rys.rs
-//     (:104-185, :230-269, :275-569) and xtask/src/gen_rys_tables.rs:1-190.
+//     Gauss rule (128-point Gauss-Legendre in u = sqrt(x)) and Golub-Welsch.
+//     Each cell is fit as a Chebyshev series, then stored in the power basis of
+//     the cell variable xi in [-1, 1] (converted in long double) so the device
+//     evaluates it by Horner: one FMA per term instead of Clenshaw's two.  The
+//     number of cells per n is the smallest whose stored polynomials meet 3e-15
+//     absolute on 33 samples per cell.  Fit method and seam follow
+//     crates/integral/src/math/rys.rs (:104-185, :230-269, :275-569) and
+//     xtask/src/gen_rys_tables.rs:1-190.
 //   * generalized Gauss-Laguerre(-1/2) nodes for the large-T branch (rys.rs:519-569).
 //   * Cartesian->spherical matrices scaled by sqrt(4pi/(2l+1)) for l >= 2
 //     (math/solid_harmonics.rs:87-250, spherical.rs:183-194).
@@ -22,7 +26,7 @@ namespace ig {
 namespace {
 
 constexpr int kGL = 128;
-constexpr int kNC = kChebTerms;  // 17 coefficients (degree 16)
+constexpr int kNC = kRysTerms;  // 13 coefficients (degree 12)
 
 double seam(int n) { return 33.0 + 2.5 * double(2 * n - 1); }
 
@@ -153,16 +157,6 @@ void stieltjes_rule(int n, double T, double *x, double *w) {
   gauss_from_jacobi(n, alpha, beta, x, w);
 }
 
-double clenshaw(const double *c, double xi) {
-  double b1 = 0.0, b2 = 0.0;
-  for (int k = kNC - 1; k >= 1; --k) {
-    double t = 2.0 * xi * b1 - b2 + c[k];
-    b2 = b1;
-    b1 = t;
-  }
-  return xi * b1 - b2 + 0.5 * c[0];
-}
-
 // Chebyshev coefficients of every root and weight of the n-point rule on [a,b].
 void fit(int n, double a, double b, double *rc, double *wc) {
   double xs[kNC][kMaxRys], ws[kNC][kMaxRys];
@@ -183,6 +177,34 @@ void fit(int n, double a, double b, double *rc, double *wc) {
     }
 }
 
+// Half-c0 Chebyshev series -> power basis in xi, in place (long double).
+void cheb_to_power(double *c) {
+  long double Tkm1[kNC] = {}, Tk[kNC] = {}, out[kNC] = {};
+  Tkm1[0] = 1.0L;  // T_0
+  Tk[1] = 1.0L;    // T_1
+  out[0] = 0.5L * c[0];
+  for (int j = 0; j < kNC; ++j) out[j] += (long double)c[1] * Tk[j];
+  for (int k = 2; k < kNC; ++k) {
+    long double Tn[kNC] = {};
+    for (int j = 0; j < kNC; ++j) {  // T_k = 2 xi T_{k-1} - T_{k-2}
+      if (j >= 1) Tn[j] += 2.0L * Tk[j - 1];
+      Tn[j] -= Tkm1[j];
+    }
+    for (int j = 0; j < kNC; ++j) {
+      Tkm1[j] = Tk[j];
+      Tk[j] = Tn[j];
+      out[j] += (long double)c[k] * Tn[j];
+    }
+  }
+  for (int j = 0; j < kNC; ++j) c[j] = double(out[j]);
+}
+
+double horner(const double *m, double xi) {
+  double r = m[kNC - 1];
+  for (int k = kNC - 2; k >= 0; --k) r = std::fma(r, xi, m[k]);
+  return r;
+}
+
 double cells_error(int n, int cells, std::vector<double> &rc, std::vector<double> &wc) {
   double width = seam(n) / cells, worst = 0.0;
   rc.assign(size_t(cells) * n * kNC, 0.0);
@@ -191,14 +213,18 @@ double cells_error(int n, int cells, std::vector<double> &rc, std::vector<double
     double a = s * width, b = a + width;
     double *r = rc.data() + size_t(s) * n * kNC, *w = wc.data() + size_t(s) * n * kNC;
     fit(n, a, b, r, w);
+    for (int i = 0; i < n; ++i) {
+      cheb_to_power(r + i * kNC);
+      cheb_to_power(w + i * kNC);
+    }
     for (int j = 0; j < 33; ++j) {
       double T = a + (b - a) * j / 32.0;
       double xr[kMaxRys], wr[kMaxRys];
       stieltjes_rule(n, T, xr, wr);
       double xi = (2.0 * T - (a + b)) / (b - a);
       for (int i = 0; i < n; ++i) {
-        worst = std::max(worst, std::fabs(clenshaw(r + i * kNC, xi) - xr[i]));
-        worst = std::max(worst, std::fabs(clenshaw(w + i * kNC, xi) - wr[i]));
+        worst = std::max(worst, std::fabs(horner(r + i * kNC, xi) - xr[i]));
+        worst = std::max(worst, std::fabs(horner(w + i * kNC, xi) - wr[i]));
       }
     }
   }
@@ -228,11 +254,12 @@ const RysHostTables &rys_host_tables() {
     std::vector<double> rc, wc;
     t.data.clear();
     for (int n = 1; n <= kMaxRys; ++n) {
-      int cells = 1;
+      // counts grow slowly with n: start the search just below the last one
+      int cells = n == 1 ? 1 : std::max(1, t.nsub[n - 1] - 3);
       double err;
       for (;; ++cells) {
         err = cells_error(n, cells, rc, wc);
-        if (err <= 2e-14 || cells >= 256) break;
+        if (err <= 3e-15 || cells >= 256) break;
       }
       t.nsub[n] = cells;
       t.worst[n] = err;
@@ -261,12 +288,12 @@ void rys_host_eval(int n, double T, double *x, double *w) {
     }
     return;
   }
-  double width = hi / t.nsub[n];
-  int s = std::min(int(T / width), t.nsub[n] - 1);
-  double a = s * width, b = a + width, xi = (2.0 * T - (a + b)) / (b - a);
+  const double v = T * (double(t.nsub[n]) / hi);  // cell coordinate, as on the device
+  const int s = std::min(int(v), t.nsub[n] - 1);
+  const double xi = 2.0 * (v - double(s)) - 1.0;
   for (int i = 0; i < n; ++i) {
-    x[i] = clenshaw(t.data.data() + t.root_off[n] + (size_t(s) * n + i) * kNC, xi);
-    w[i] = clenshaw(t.data.data() + t.wt_off[n] + (size_t(s) * n + i) * kNC, xi);
+    x[i] = horner(t.data.data() + t.root_off[n] + (size_t(s) * n + i) * kNC, xi);
+    w[i] = horner(t.data.data() + t.wt_off[n] + (size_t(s) * n + i) * kNC, xi);
   }
 }
 

@@ -162,3 +162,95 @@ def test_grad_bra_pairs_sum_in_any_order():
     for k, bra in enumerate(reversed(gc.bra_pairs())):
         gc.accumulate_bra_pair(None, bra, dens, ig.EriKernel.coulomb(), 0.0, parts[k % 2])
     assert np.max(np.abs(parts[0] + parts[1] - serial)) < 1e-12
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_direct_jk_every_kernel_variant(variant):
+    """Both J/K kernels of every compiled class (team/LDS and lane/register)
+    against the oracle, forced through a loaded schedule."""
+    shells = (B.grad_basis(True, max_l=2) + B.fd_basis()
+              + [(2, (1.3, -0.8, 0.9), [0.45], [1.0], True)])
+    n = B.nfunc(shells)
+    dens = B.direct_contractor_densities(n)
+    note("jk variant %d nsh=%d" % (variant, len(shells)))
+    dc = gpu_basis(shells).direct_contractor()
+    sched = dc.schedule()
+    dc.set_schedule([dict(e, variant=variant) for e in sched])
+    assert all(e["variant"] == variant and e["tuned"] for e in dc.schedule())
+    J, K = dc.jk(dens, 0.0)
+    oJ, oK = O.Basis(shells).direct_jk(dens, 0.0)
+    for x in range(2):
+        assert np.max(np.abs(J[x] - oJ[x])) < TOL
+        assert np.max(np.abs(K[x] - oK[x])) < TOL
+
+
+def test_autotune_keeps_results():
+    import torch
+    shells = B.grad_basis(True, max_l=2) + B.shared_centers_basis()
+    n = B.nfunc(shells)
+    p = B.symmetric(n, 9)
+    note("autotune")
+    dc = gpu_basis(shells).direct_contractor()
+    D = torch.tensor(p[None], dtype=torch.float64, device="cuda:0").contiguous()
+    sched = dc.autotune(D, threshold=1e-12, reps=1)
+    assert sched and all(e["tuned"] and e["variant"] in (0, 1) for e in sched)
+    J, K = dc.jk([p], 1e-12)
+    oJ, oK = O.Basis(shells).direct_jk([p], 1e-12)
+    assert np.max(np.abs(J[0] - oJ[0])) < TOL and np.max(np.abs(K[0] - oK[0])) < TOL
+    with pytest.raises(ig.IntegralError):
+        dc.set_schedule([{"cls": (3, 0, 0, 0), "variant": 0}])
+
+
+def _support_density(shells, keep, seed):
+    """Symmetric density whose nonzero blocks only touch the shells in `keep`:
+    most quartets then fail the density mask, interleaved with ones that pass
+    inside every wavefront's range."""
+    n = B.nfunc(shells)
+    off, o = [], 0
+    for s in shells:
+        off.append(o)
+        o += (2 * s[0] + 1) if s[4] else (s[0] + 1) * (s[0] + 2) // 2
+    rows = np.zeros(n, dtype=bool)
+    for k in keep:
+        nf = (2 * shells[k][0] + 1) if shells[k][4] else (shells[k][0] + 1) * (shells[k][0] + 2) // 2
+        rows[off[k]:off[k] + nf] = True
+    p = B.symmetric(n, seed)
+    return p * np.outer(rows, rows)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_direct_jk_density_screened_lanes(variant):
+    """Quartets dropped by the density mask next to kept ones in the same
+    wavefront, across bra rows (regression: the wavefront-reduced bra J sum
+    must be written at the kept lanes' bra pair)."""
+    shells = B.grad_basis(True, max_l=2) + B.shared_centers_basis() + B.fd_basis()
+    p = _support_density(shells, [0, 2, 7], 5)
+    note("jk density-screened lanes variant %d" % variant)
+    dc = gpu_basis(shells).direct_contractor()
+    dc.set_schedule([dict(e, variant=variant) for e in dc.schedule()])
+    J, K = dc.jk([p], 1e-12)
+    ob = O.Basis(shells)
+    oJ, oK = ob.direct_jk([p], 1e-12)
+    assert dc.last_stats()[0] == ob.last_stats[0]
+    assert 0 < ob.last_stats[0]
+    assert np.max(np.abs(J[0] - oJ[0])) < TOL
+    assert np.max(np.abs(K[0] - oK[0])) < TOL
+
+
+def test_direct_jk_twelve_atom_molecule():
+    """The benchmark's molecule recipe at 12 atoms (spherical s,s,p,p,d,
+    3/3/3/3/1 primitives, nao = 156), two densities, tau = 1e-12: every
+    kernel at a size where each wavefront walks many quartets."""
+    shells, rng = synthetic.molecule(20260007, 12, 8.0, [(0, 3), (0, 3), (1, 3), (1, 3), (2, 1)],
+                                     (0.08, 40.0), True)
+    n = B.nfunc(shells)
+    dens = [synthetic.density(rng, n, 6), synthetic.density(rng, n, 6)]
+    note("jk 12-atom molecule nao=%d" % n)
+    dc = gpu_basis(shells).direct_contractor()
+    J, K = dc.jk(dens, 1e-12)
+    ob = O.Basis(shells)
+    oJ, oK = ob.direct_jk(dens, 1e-12)
+    assert dc.last_stats()[0] == ob.last_stats[0]
+    for x in range(2):
+        assert np.max(np.abs(J[x] - oJ[x])) < TOL
+        assert np.max(np.abs(K[x] - oK[x])) < TOL
