@@ -43,6 +43,9 @@ def class_flops(rec, spherical=True, ndens=2):
     return per * q
 
 
+VARIANTS = ["team", "register", "register+row sums"]
+
+
 def kernel_name(cls, variant):
     """Kernel(s) of one class launch: the team kernel, or the register kernel
     in as many element slices as the class needs (launched back to back)."""
@@ -52,8 +55,9 @@ def kernel_name(cls, variant):
     if variant == 0:
         return "jk_kernel<Cls<%s>>" % c
     slices = (n4 + 80) // 81
-    return "jk_reg_kernel<Cls<%s>, LO, HI> (%d element slice%s)" % (c, slices,
-                                                                     "s" if slices > 1 else "")
+    name = "jk_reg_kernel" if variant == 1 else "jk_rowacc_kernel"
+    return "%s<Cls<%s>, LO, HI, double> (%d element slice%s)" % (name, c, slices,
+                                                                 "s" if slices > 1 else "")
 
 
 def cpu_baseline(shells, dens, tau, seconds):
@@ -82,6 +86,97 @@ def cpu_baseline(shells, dens, tau, seconds):
                       "J and K, tau=1e-12" % (b0, b0 + m, npairs, nq, dt)}
 
 
+def bench_grad(args):
+    """Seconds per geometry of forward J/K energy + backward forces on the
+    30-atom s,s,p,p,d molecule (nao 390): plan build at the geometry, one J/K
+    build from the device density, the energy, the gradient density and the
+    adjoint gradient contraction.  Geometries are the starting one displaced by
+    0.01 bohr Gaussian noise per step, as in a geometry optimisation."""
+    import torch
+    import integral_amd as ig
+    from integral_amd import energy as EN, synthetic
+    cj, ck, tau = 1.0, -0.5, 1e-12
+    shells, D = synthetic.gradient_config()
+    tmpl, pos0 = EN.templates_from_shells(shells)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    Dd = torch.tensor(D[None], dtype=torch.float64, device=dev).contiguous()
+    rng = np.random.default_rng(20260004)
+
+    def geometry(p):
+        t0 = time.perf_counter()
+        basis = EN.basis_at(tmpl, p)
+        dc = basis.direct_contractor()
+        gc = basis.direct_grad_contractor()
+        t1 = time.perf_counter()
+        J = torch.zeros_like(Dd)
+        K = torch.zeros_like(Dd)
+        dc.jk_device(Dd, J, K, threshold=tau, stream=stream)
+        e = float((0.5 * cj * (Dd * J).sum() + 0.5 * ck * (Dd * K).sum()).item())
+        t2 = time.perf_counter()
+        dens = gc.density([ig.GradDensityTerm(0.5 * cj, D, D)], [ig.GradDensityTerm(0.5 * ck, D, D)])
+        f = gc.contract(dens, threshold=tau)
+        t3 = time.perf_counter()
+        return e, f, (t1 - t0, t2 - t1, t3 - t2), gc.last_stats()
+
+    for _ in range(args.warmup):
+        geometry(pos0)
+    torch.cuda.synchronize(dev)
+    parts = np.zeros(3)
+    t0 = time.perf_counter()
+    p = pos0.copy()
+    for _ in range(args.steps):
+        e, f, dt, gstats = geometry(p)
+        parts += dt
+        p = p + 0.01 * rng.standard_normal(p.shape)
+    torch.cuda.synchronize(dev)
+    per = (time.perf_counter() - t0) / args.steps
+    parts /= args.steps
+    log("grad: %.1f ms/geometry (plans %.1f, J/K energy %.1f, forces %.1f)"
+        % (per * 1e3, parts[0] * 1e3, parts[1] * 1e3, parts[2] * 1e3))
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_grad(shells, D, cj, ck, tau, args.cpu_seconds)
+        # the whole geometry at the sampled rate (kernel work only)
+        cpu["est_s_per_geometry"] = gstats[0] / cpu["value"]
+    out = {
+        "metric": METRIC, "value": per, "unit": "s/geometry", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": per * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded molecule and density, no checkpoint)",
+        "config": {"workload": "J/K energy forward + adjoint forces backward, 30 atoms in a 12 "
+                               "bohr cube, spherical s,s,p,p,d (3,3,2,2,1 primitives), nao=390, "
+                               "Coulomb 1.0 / exchange -0.5, tau=1e-12, FP64",
+                   "plan_ms": parts[0] * 1e3, "forward_ms": parts[1] * 1e3,
+                   "backward_ms": parts[2] * 1e3, "grad_quartets": gstats[0],
+                   "parallelism": "1 GPU"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline_grad(shells, D, cj, ck, tau, seconds):
+    """Oracle adjoint gradient on the host cores over a contiguous bra-pair
+    sample, reported as gradient quartets per second."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    ob = O.Basis(shells)
+    npairs = ob.nsh * (ob.nsh + 1) // 2
+    b0, m = npairs // 2, 8
+    while True:
+        t0 = time.perf_counter()
+        ob.direct_grad([(0.5 * cj, D, D)], [(0.5 * ck, D, D)], tau, bra_range=(b0, b0 + m),
+                       threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= seconds or b0 + m >= npairs:
+            break
+        m = min(npairs - b0, int(m * max(2.0, min(8.0, seconds / max(dt, 1e-3)))) + 1)
+    nq = ob.last_stats[0]
+    return {"value": nq / dt, "unit": "gradient quartets/s", "cores": threads, "kind": "port",
+            "sample": "canonical bra pairs [%d, %d) of %d (%d quartets, %.1f s)"
+                      % (b0, b0 + m, npairs, nq, dt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +186,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the built-in class-to-kernel schedule instead of tuning it")
+    ap.add_argument("--mixed-cut", type=float, default=1e-6,
+                    help="also time the build with FP32 quartets below this Schwarz product "
+                         "(0: skip)")
+    ap.add_argument("--workload", choices=["jk", "grad"], default="jk",
+                    help="jk: the direct J/K build (default); grad: seconds per geometry of "
+                         "the J/K energy and its adjoint forces")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once and exit: the reference "
                          "kernel of known traffic that scales the profiler's FETCH_SIZE and "
@@ -112,6 +213,8 @@ def main():
         torch.cuda.synchronize()
         log("copied %d MiB" % args.copy_probe)
         return
+    if args.workload == "grad":
+        return bench_grad(args)
     import integral_amd as ig
     from integral_amd import synthetic
 
@@ -152,6 +255,27 @@ def main():
     ms = (t1 - t0) * 1e3 / args.steps
     quartets, prim_quartets = dc.last_stats()
     log("timed %d steps: %.3f ms/step, %d quartets" % (args.steps, ms, quartets))
+
+    # the same build with FP32 tables for quartets whose Schwarz product is
+    # below the cut, against the FP64 result just computed
+    mixed = []
+    J64, K64 = J.clone(), K.clone()
+    for cut in (args.mixed_cut, args.mixed_cut * 1e-2) if args.mixed_cut > 0 else ():
+        dc.set_mixed_precision(cut)
+        step()
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize(dev)
+        mms = (time.perf_counter() - t0) * 1e3 / args.steps
+        m = {"fp32_below": cut, "ms_per_step": mms, "speedup_vs_fp64": ms / mms,
+             "max_abs_dJ": float((J - J64).abs().max().item()),
+             "max_abs_dK": float((K - K64).abs().max().item())}
+        mixed.append(m)
+        log("mixed (FP32 below %.0e): %.3f ms/step, max |dJ| %.2e, max |dK| %.2e"
+            % (cut, mms, m["max_abs_dJ"], m["max_abs_dK"]))
+    dc.set_mixed_precision(0.0)
 
     # per-class attribution with HIP events on the launch stream
     dc.set_profiling(True)
@@ -207,8 +331,9 @@ def main():
                    "autotune_s": tune_s, "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": mixed,
         "classes": [{"cls": "".join("spdfghi"[l] for l in r["cls"]), "quartets": r["quartets"],
-                     "variant": ["team", "register"][sched.get(tuple(r["cls"]), 0)],
+                     "variant": VARIANTS[sched.get(tuple(r["cls"]), 0)],
                      "ms": round(r["ms"], 4)} for r in recs],
     }
     print(json.dumps(out))

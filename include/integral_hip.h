@@ -153,26 +153,40 @@ int ig_direct_last_stats(const ig_direct *direct, int64_t *stats2);
  * records and sets *n to the number available. */
 typedef struct ig_class_profile {
   int32_t la, lb, lc, ld;   /* kernel orientation, pair class (la,lb) >= (lc,ld) */
+  int32_t fp32;             /* 1: the FP32 part of a mixed-precision call */
+  int32_t reserved;
   int64_t quartets;         /* canonical quartets accepted (density mask != 0) */
   int64_t prim_quartets;    /* primitive quartets evaluated */
   double ms;                /* kernel time from HIP events (0 when not profiling) */
 } ig_class_profile;
 int ig_direct_set_profiling(ig_direct *direct, int enable);
+
+/* Precision policy of the whole-list J/K calls (ig_direct_jk over all bra
+ * pairs, ig_direct_jk_device).  Quartets whose Schwarz product Q_ij*Q_kl is
+ * below fp32_below (and at or above the threshold) run with FP32 2D tables and
+ * contracted block; roots and weights, prefactors, the screening and the J/K
+ * accumulation stay FP64.  0 (the default) = FP64 only.  Per-bra-pair and
+ * bra-range calls are always FP64.  No counterpart in the reference. */
+int ig_direct_set_mixed_precision(ig_direct *direct, double fp32_below);
 int ig_direct_profile_read(const ig_direct *direct, ig_class_profile *out, size_t cap, size_t *n);
 
-/* Class-to-kernel schedule of a plan.  Every compiled quartet class has two
+/* Class-to-kernel schedule of a plan.  Every compiled quartet class has three
  * J/K kernels: variant 0 runs one quartet per team of lanes with its 2D
- * tables in LDS, variant 1 one quartet per lane in registers.  The reference
+ * tables in LDS, variant 1 one quartet per lane in registers, variant 2 the
+ * register kernel with the bra row's Coulomb and exchange sums kept in LDS
+ * (used for the whole-list calls; per-bra-pair calls fall back to 1).  The
+ * reference
  * has no counterpart (its CPU engines pick by angular momentum only,
  * integrals.rs:323-329); this replaces a fixed dispatch on the GPU.
  *
- * ig_direct_autotune times both variants of every class pair with work, on
+ * ig_direct_autotune times every variant of every class pair with work, on
  * the given device densities and threshold (min over `reps` launches each),
  * and keeps the faster in the plan.  Trial results go to plan scratch; no
  * caller buffer is written.  Synchronises `stream`. */
 typedef struct ig_schedule_entry {
   int32_t la, lb, lc, ld;  /* kernel orientation, pair class (la,lb) >= (lc,ld) */
-  int32_t variant;         /* 0 team/LDS, 1 lane/registers; -1 on write = default */
+  int32_t variant;         /* 0 team/LDS, 1 lane/registers, 2 registers + LDS row sums;
+                              -1 on write = default */
   int32_t tuned;           /* read: 1 if set by the autotuner or a write */
   double ms;               /* read: the autotuner's time for the chosen variant */
 } ig_schedule_entry;

@@ -127,7 +127,8 @@ class _ShellDesc(ctypes.Structure):
 
 class _ClassProfile(ctypes.Structure):
     _fields_ = [("la", ctypes.c_int32), ("lb", ctypes.c_int32), ("lc", ctypes.c_int32),
-                ("ld", ctypes.c_int32), ("quartets", ctypes.c_int64),
+                ("ld", ctypes.c_int32), ("fp32", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("quartets", ctypes.c_int64),
                 ("prim_quartets", ctypes.c_int64), ("ms", ctypes.c_double)]
 
 
@@ -147,6 +148,13 @@ def library():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError("integral_amd: %s is not built; run `make -C integral_amd/csrc`" % LIB_PATH)
+    # PyTorch ships its own HIP runtime (same soname libamdhip64.so.7).  Loaded
+    # after ours, it would start a second runtime that finds no device; loaded
+    # first, the library binds to it and both share one runtime and one device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     sz = ctypes.c_size_t
     i64 = ctypes.c_int64
@@ -180,6 +188,7 @@ def library():
     lib.ig_direct_set_profiling.argtypes = [_vp, ctypes.c_int]
     lib.ig_direct_profile_read.argtypes = [_vp, ctypes.POINTER(_ClassProfile), sz,
                                            ctypes.POINTER(sz)]
+    lib.ig_direct_set_mixed_precision.argtypes = [_vp, ctypes.c_double]
     lib.ig_direct_autotune.argtypes = [_vp, _vp, sz, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        _vp]
     lib.ig_direct_schedule_read.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz,
@@ -457,10 +466,10 @@ class DirectContractor:
         _raise(library().ig_direct_set_profiling(self._h, 1 if enable else 0))
 
     # J/K kernel variants of a compiled quartet class
-    TEAM, REGISTER = 0, 1
+    TEAM, REGISTER, ROW_ACCUMULATE = 0, 1, 2
 
     def autotune(self, dens, threshold=0.0, want_exchange=True, reps=2, stream=None):
-        """Time both J/K kernel variants of every class pair on the device
+        """Time every J/K kernel variant of every class pair on the device
         densities `dens` (float64 CUDA tensor, ndens x nao x nao) and keep the
         faster per class pair; returns the resulting schedule."""
         if not (dens.is_cuda and dens.element_size() == 8 and dens.is_contiguous()):
@@ -493,12 +502,17 @@ class DirectContractor:
     def profile(self):
         """Per-class records of the last call: dicts with la..ld, quartets,
         prim_quartets and ms (event time when profiling is on)."""
-        cap = 512
+        cap = 1024
         buf = (_ClassProfile * cap)()
         n = ctypes.c_size_t()
         _raise(library().ig_direct_profile_read(self._h, buf, cap, ctypes.byref(n)))
-        return [{"cls": (r.la, r.lb, r.lc, r.ld), "quartets": r.quartets,
+        return [{"cls": (r.la, r.lb, r.lc, r.ld), "fp32": bool(r.fp32), "quartets": r.quartets,
                  "prim_quartets": r.prim_quartets, "ms": r.ms} for r in buf[:min(n.value, cap)]]
+
+    def set_mixed_precision(self, fp32_below):
+        """Quartets with Q_ij*Q_kl < fp32_below run with FP32 tables and block
+        in the whole-list J/K calls (0 = FP64 only)."""
+        _raise(library().ig_direct_set_mixed_precision(self._h, float(fp32_below)))
 
 
 class GradDensityTerm:

@@ -281,8 +281,9 @@ __global__ void __launch_bounds__(kBigThreads)
       if (!((mask >> dd.eff_src[e]) & 1ull)) continue;
       const bool anti = dd.eff_anti[e];
       const double *D = dd.dc + (size_t)e * n2;
-      double *J = dd.jc + (size_t)e * n2;
-      double *Kp = dd.kc ? dd.kc + (size_t)e * n2 : nullptr;
+      const size_t xo = jk_copy_offset(dd);
+      double *J = dd.jc + xo + (size_t)e * n2;
+      double *Kp = dd.kc ? dd.kc + xo + (size_t)e * n2 : nullptr;
       for (int o = threadIdx.x; o < NO; o += blockDim.x) {
         if (anti && o < NJ) continue;  // antisymmetric parts feed K only
         double v = 0.0;
@@ -341,7 +342,7 @@ __global__ void __launch_bounds__(kBigThreads)
           v *= f;
           dst = Kp;
         }
-        if (v != 0.0) atomicAdd(dst + (size_t)row * ncrt + col, v);
+        if (v != 0.0) jk_add(dd, dst + (size_t)row * ncrt + col, v);
       }
     }
     __syncthreads();  // the slot is overwritten by the next quartet

@@ -9,6 +9,18 @@ namespace ig {
 
 constexpr double kTwoPi52 = 34.98683665524972;  // 2 pi^{5/2}
 
+// XCD executing this wavefront: hardware register XCC_ID, field [3:0]
+// (s_getreg_b32 hwreg(HW_REG_XCC_ID = 20, offset 0, size 4)).
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 15; }
+
+// Offset of this wavefront's accumulator copy (see DensDev).
+__device__ __forceinline__ size_t jk_copy_offset(const DensDev &d) {
+  return d.ncopy > 1 ? size_t(xcc_id() % d.ncopy) * d.copy_stride : 0;
+}
+
+// J'/K' accumulation (device-scope FP64 atomic add).
+__device__ __forceinline__ void jk_add(const DensDev &, double *p, double v) { atomicAdd(p, v); }
+
 // Lanes of one team always sit in one wavefront, so ordering LDS traffic
 // between phases only needs a wavefront-scope fence plus a scheduling barrier.
 __device__ __forceinline__ void team_sync() {
@@ -60,23 +72,24 @@ __device__ __forceinline__ double rys_series(const RysDev &R, int n, double T, i
 // horizontal transfers (engine/rys.rs:234-308 recurrences), written to `out`
 // as ((i*(LB+1)+j)*(LC+1)+k)*(LD+1)+l.  Rolling levels keep the register
 // footprint at two (NB+1)x(NK+1) slabs.
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void rys_axis_table(double c00, double cp0, double b10, double b01,
-                                               double b00, double ab, double cd,
-                                               double *__restrict__ out) {
+// F: arithmetic type of the tables (double, or float for the mixed-precision
+// quartets).
+template <int LA, int LB, int LC, int LD, typename F = double>
+__device__ __forceinline__ void rys_axis_table(F c00, F cp0, F b10, F b01, F b00, F ab, F cd,
+                                               F *__restrict__ out) {
   constexpr int NB = LA + LB, NK = LC + LD;
-  double g[NB + 1][NK + 1];
-  g[0][0] = 1.0;
+  F g[NB + 1][NK + 1];
+  g[0][0] = F(1);
   if constexpr (NB >= 1) g[1][0] = c00;
 #pragma unroll
-  for (int n = 1; n < NB; ++n) g[n + 1][0] = c00 * g[n][0] + double(n) * b10 * g[n - 1][0];
+  for (int n = 1; n < NB; ++n) g[n + 1][0] = c00 * g[n][0] + F(n) * b10 * g[n - 1][0];
 #pragma unroll
   for (int m = 0; m < NK; ++m) {
 #pragma unroll
     for (int n = 0; n <= NB; ++n) {
-      double t = cp0 * g[n][m];
-      if (m >= 1) t += double(m) * b01 * g[n][m - 1];
-      if (n >= 1) t += double(n) * b00 * g[n - 1][m];
+      F t = cp0 * g[n][m];
+      if (m >= 1) t += F(m) * b01 * g[n][m - 1];
+      if (n >= 1) t += F(n) * b00 * g[n - 1][m];
       g[n][m + 1] = t;
     }
   }
@@ -92,7 +105,7 @@ __device__ __forceinline__ void rys_axis_table(double c00, double cp0, double b1
     }
 #pragma unroll
     for (int i = 0; i <= LA; ++i) {
-      double t[NK + 1];
+      F t[NK + 1];
 #pragma unroll
       for (int k = 0; k <= NK; ++k) t[k] = g[i][k];
 #pragma unroll
@@ -116,6 +129,7 @@ struct RowCursor {
   int P = 0;
   long long row_end = -1;  // rowptr[P + 1]
   long long row_begin = 0;
+  int qb = 0;              // first ket of row P (qs.qbase[P], 0 without it)
   __device__ __forceinline__ void next(const QuartetSet &qs, long long t, int &Pout, int &Qout) {
     if (qs.list) {
       int2 v = qs.list[t];
@@ -135,14 +149,18 @@ struct RowCursor {
       P = lo;
       row_begin = qs.rowptr[P];
       row_end = qs.rowptr[P + 1];
+      qb = qs.qbase ? qs.qbase[P] : 0;
     }
-    while (t >= row_end) {
-      ++P;
-      row_begin = row_end;
-      row_end = qs.rowptr[P + 1];
+    if (t >= row_end) {
+      while (t >= row_end) {
+        ++P;
+        row_begin = row_end;
+        row_end = qs.rowptr[P + 1];
+      }
+      qb = qs.qbase ? qs.qbase[P] : 0;
     }
     Pout = P;
-    Qout = int(t - row_begin);
+    Qout = qb + int(t - row_begin);
   }
 };
 
@@ -172,7 +190,7 @@ __device__ __forceinline__ void decode_quartet(const QuartetSet &qs, long long t
       hi = mid - 1;
   }
   P = lo;
-  Q = int(t - qs.rowptr[lo]);
+  Q = int(t - qs.rowptr[lo]) + (qs.qbase ? qs.qbase[lo] : 0);
 }
 
 }  // namespace ig

@@ -60,6 +60,10 @@ struct BasisDev {
   const int *atom;       // nsh
   const double *c2s;     // packed scaled c2s matrices for l=0..6
   int c2s_off[kMaxL + 1];
+  // shells by angular momentum: lshells[lshell_off[l] + r] is the r-th shell
+  // of momentum l, rank[s] the position of shell s in its group
+  const int *rank, *lshells;
+  int lshell_off[kMaxL + 2];
 };
 
 // Shell pairs in kernel orientation (first shell has the larger l), grouped
@@ -83,6 +87,8 @@ struct QuartetSet {
   int y_off, y_n;
   int same;        // X == Y
   const long long *rowptr;  // x_n + 1 prefix sums (implicit mode) or null
+  const int *qbase;         // implicit mode: first ket of each row (null: 0), so a row
+                            // holds kets [qbase, qbase + count) of Y's sorted list
   const int2 *list;         // explicit (P, Q) pairs relative to x_off / y_off, or null
   long long total;
 };
@@ -91,7 +97,13 @@ struct DensDev {
   int ndens;            // original densities
   int neff;             // effective densities (symmetric + antisymmetric parts)
   const double *dc;     // neff x nao_cart^2 Cartesian densities
-  double *jc, *kc;      // neff x nao_cart^2 accumulators (kc may be null)
+  // accumulators (kc may be null): ncopy copies of neff x nao_cart^2, copy_stride
+  // doubles apart; wavefronts add into the copy of the XCD they run on (its
+  // XCC_ID), which spreads concurrent atomics to one element over ncopy
+  // addresses.  The copies are summed before the finishing transform.
+  double *jc, *kc;
+  size_t copy_stride;
+  int ncopy;
   const int *eff_src;   // neff: original density index
   const int *eff_anti;  // neff: 1 if antisymmetric part (K only)
   const double *bound;  // ndens x nsh^2, symmetrised max |D| per shell block
@@ -115,12 +127,29 @@ bool class_supported(int la, int lb, int lc, int ld);
 hipError_t launch_schwarz(int lcls1, int lcls2, const BasisDev &b, const PairsDev &p,
                           const RysDev &r, int first, int count, double *qout, hipStream_t s);
 // J/K kernel variants of a compiled class: one quartet per team of lanes
-// with LDS tables, or one quartet per lane in registers (sliced blocks)
-enum JkVariant { kJkTeam = 0, kJkReg = 1, kJkNumVariants = 2 };
+// with LDS tables, one quartet per lane in registers (sliced blocks), or the
+// register path with per-row LDS accumulation of the bra-side updates
+enum JkVariant { kJkTeam = 0, kJkReg = 1, kJkRowAcc = 2, kJkNumVariants = 3 };
+constexpr size_t kRowAccLdsMax = 64 * 1024;  // dynamic LDS of one row-accumulating workgroup
 int jk_default_variant(int la, int lb, int lc, int ld);
+// width of the row accumulators' compact column space for ket momenta lc >= ld
+int rowacc_width(const BasisDev &b, int lc, int ld);
+// can `variant` run this class on this basis (row accumulation: implicit
+// quartet sets only, and its LDS must fit)?
+bool jk_variant_fits(int variant, int la, int lb, int lc, int ld, const BasisDev &b, int neff,
+                     bool listed);
 hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                      const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
                      int variant, unsigned long long *counters, hipStream_t s);
+// the lane-per-quartet variants (kJkReg, kJkRowAcc) of launch_jk, FP64 and
+// with FP32 tables and block (mixed-precision quartets)
+hipError_t launch_jk_lane(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                          const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
+                          int variant, unsigned long long *counters, hipStream_t s);
+hipError_t launch_jk_lane_f32(int la, int lb, int lc, int ld, const BasisDev &b,
+                              const PairsDev &p, const RysDev &r, const QuartetSet &qs,
+                              const DensDev &d, double omega, int variant,
+                              unsigned long long *counters, hipStream_t s);
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                       const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s);
 hipError_t launch_grad(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
@@ -160,5 +189,9 @@ hipError_t launch_to_cart(const BasisDev &b, const double *m_func, int nmats, do
                           hipStream_t s);
 hipError_t launch_jk_finish(const BasisDev &b, const DensDev &d, double *d_J, double *d_K,
                             hipStream_t s);
+// sum the ncopy per-XCD accumulator copies of d into the first one
+hipError_t launch_fold_copies(const DensDev &d, size_t nc2, hipStream_t s);
+// histogram (16 bins) of the XCC_ID seen by `groups` workgroups
+hipError_t launch_xcc_probe(unsigned *hist, int groups, hipStream_t s);
 
 }  // namespace ig

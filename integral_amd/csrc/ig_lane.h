@@ -1,0 +1,68 @@
+// ig_lane.h -- lane-per-quartet J/K kernels (one shell quartet per lane,
+// block in registers): the plain register path (ig_reg.h) and the register
+// path with per-row LDS sums (ig_rowacc.h), with their class dispatch, for the
+// arithmetic type IG_LANE_T (double, or float for the mixed-precision
+// quartets), exported as IG_LANE_FN.  Each type is its own translation unit
+// (ig_jk_lane64.hip, ig_jk_lane32.hip) so they compile side by side.
+#include "ig_cls.h"
+
+namespace ig {
+
+#include "ig_reg.h"
+#include "ig_rowacc.h"
+
+#ifdef IG_LANE_HOST_HELPERS
+int rowacc_width(const BasisDev &b, int lc, int ld) {
+  const int nC = b.lshell_off[lc + 1] - b.lshell_off[lc];
+  const int nD = b.lshell_off[ld + 1] - b.lshell_off[ld];
+  return nC * ncart(lc) + (lc == ld ? 0 : nD * ncart(ld));
+}
+
+bool jk_variant_fits(int variant, int la, int lb, int lc, int ld, const BasisDev &b, int neff,
+                     bool listed) {
+  if (variant != kJkRowAcc) return true;
+  if (listed) return false;
+  const int nr = (la + lb + lc + ld) / 2 + 1;
+  const size_t lds = size_t(2) * rys_host_tables().nsub[nr] * nr * kRysTerms * sizeof(double) +
+                     rowacc_doubles(ncart(la), ncart(lb), neff, rowacc_width(b, lc, ld)) *
+                         sizeof(double);
+  return lds <= kRowAccLdsMax;
+}
+#endif
+
+namespace {
+
+template <int LA, int LB, int LC, int LD, typename F>
+hipError_t jk_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r, const QuartetSet &qs,
+                       const DensDev &d, double omega, int variant, unsigned long long *ctr,
+                       hipStream_t s) {
+  using C = Cls<LA, LB, LC, LD>;
+  if (qs.total <= 0) return hipSuccess;
+  const size_t lds = size_t(2) * r.nsub[C::NR] * C::NR * kRysTerms * sizeof(double);
+  if (variant == kJkRowAcc) {
+    const size_t acc = rowacc_doubles(C::na, C::nb, d.neff, rowacc_width(b, LC, LD));
+    if (qs.list || lds + acc * sizeof(double) > kRowAccLdsMax) return hipErrorInvalidValue;
+    long long blocks = (qs.total + 64 * kRowWaves - 1) / (64 * kRowWaves);
+    if (blocks > 1024) blocks = 1024;
+    return launch_jk_rowacc_slices<C, 0, F>(b, p, r, qs, d, omega, ctr, s, int(blocks),
+                                            lds + acc * sizeof(double));
+  }
+  long long blocks = (qs.total + 63) / 64;
+  if (blocks > 8192) blocks = 8192;
+  return launch_jk_reg_slices<C, 0, F>(b, p, r, qs, d, omega, ctr, s, int(blocks), lds);
+}
+
+}  // namespace
+
+hipError_t IG_LANE_FN(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                     const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
+                     int variant, unsigned long long *ctr, hipStream_t s) {
+#define IG_CASE(A, B, Cc, D)                       \
+  if (la == A && lb == B && lc == Cc && ld == D) \
+    return jk_lane_one<A, B, Cc, D, IG_LANE_T>(b, p, r, qs, d, omega, variant, ctr, s);
+  IG_FOR_CLASSES(IG_CASE)
+#undef IG_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace ig

@@ -203,6 +203,30 @@ RysDevice &rys_device() {
   return *r;
 }
 
+// Number of per-XCD J'/K' accumulator copies: one per XCC_ID that a probe
+// launch of many workgroups observes (the dispatcher spreads workgroups over
+// every XCD of the device); 1 when the probe sees a single id.
+int xcd_copies() {
+  static int copies = 0;
+  static std::mutex m;
+  std::lock_guard<std::mutex> g(m);
+  if (copies) return copies;
+  DBuf hist;
+  hist.reserve(16 * sizeof(unsigned));
+  check(hipMemset(hist.p, 0, 16 * sizeof(unsigned)), "zero probe");
+  check(launch_xcc_probe(hist.as<unsigned>(), 4096, nullptr), "xcc probe");
+  unsigned h[16];
+  check(hipMemcpy(h, hist.p, sizeof(h), hipMemcpyDeviceToHost), "copy probe");
+  int distinct = 0, top = 0;
+  for (int i = 0; i < 16; ++i)
+    if (h[i]) {
+      ++distinct;
+      top = i + 1;
+    }
+  copies = distinct >= 2 ? top : 1;
+  return copies;
+}
+
 struct ClassRange {
   int off = 0, n = 0, l1 = 0, l2 = 0;
 };
@@ -212,7 +236,7 @@ struct Plan {
   int nsh, nao, nao_cart, natom;
   std::vector<int> sh_l, sh_kind, sh_atom, sh_off_func, sh_off_cart, sh_nprim;
   // device basis
-  DBuf d_xyz, d_l, d_kind, d_off_func, d_off_cart, d_atom, d_c2s;
+  DBuf d_xyz, d_l, d_kind, d_off_func, d_off_cart, d_atom, d_c2s, d_rank, d_lshells;
   BasisDev bdev{};
   // pairs, final (class, Q-descending) order
   int npairs = 0;
@@ -225,27 +249,33 @@ struct Plan {
   PairsDev pdev{};
   std::vector<double> schwarz;  // nsh x nsh (function space)
   std::mutex mu;                // serialises calls that use the scratch below
-  DBuf s_dfunc, s_ds, s_da, s_dc, s_bound, s_jc, s_kc, s_J, s_K, s_eff, s_rowptr, s_list,
-      s_counters, s_flag;
+  DBuf s_dfunc, s_ds, s_da, s_dc, s_bound, s_jc, s_kc, s_J, s_K, s_eff, s_rowptr, s_qbase,
+      s_list, s_counters, s_flag;
   DBuf s_big;  // block slots of the runtime-l kernels (classes beyond the compiled ones)
   // J/K kernel variant per (bra class, ket class): -1 = the built-in default,
   // else set by the autotuner or a loaded schedule (with its measured time)
   std::vector<int8_t> variant = std::vector<int8_t>(kNumPairClass * kNumPairClass, -1);
   std::vector<float> variant_ms = std::vector<float>(kNumPairClass * kNumPairClass, 0.0f);
-  double cached_tau = NAN;
+  double cached_tau = NAN, cached_cut = NAN;
   std::vector<long long> cached_rowptr_host;
+  std::vector<int> cached_qbase_host;
+  // mixed precision: quartets whose Schwarz product Q_P Q_Q is below this
+  // (and at or above the threshold) run with FP32 tables; 0 = FP64 only
+  double fp32_below = 0.0;
   struct CP {
     int x, y;
     size_t rp_off;
     long long total;
+    int fp32 = 0;         // 1: the FP32 part (rows start at qbase)
+    long long qb_off = -1;  // offset of its per-row first kets in s_qbase
   };
   std::vector<CP> cached_cps;
   int eff_key = -1;
   // per-launch attribution of the last call (counters on the device, events
   // when profiling); finalised lazily on the stream the call used
-  static constexpr int kMaxLaunch = kNumPairClass * (kNumPairClass + 1) / 2;
+  static constexpr int kMaxLaunch = kNumPairClass * (kNumPairClass + 1);  // FP64 + FP32 sets
   struct Launch {
-    int la, lb, lc, ld;
+    int la, lb, lc, ld, fp32;
   };
   std::vector<Launch> launches;
   std::vector<unsigned long long> launch_ctr;  // 2 per launch, host copy
@@ -271,10 +301,11 @@ void begin_launches(Plan &P, hipStream_t s) {
   }
 }
 
-unsigned long long *next_launch(Plan &P, const ClassRange &X, const ClassRange &Y, hipStream_t s) {
+unsigned long long *next_launch(Plan &P, const ClassRange &X, const ClassRange &Y, hipStream_t s,
+                                int fp32 = 0) {
   const int k = int(P.launches.size());
   if (k >= Plan::kMaxLaunch) throw DeviceError{IG_ERR_DEVICE, "too many class launches"};
-  P.launches.push_back({X.l1, X.l2, Y.l1, Y.l2});
+  P.launches.push_back({X.l1, X.l2, Y.l1, Y.l2, fp32});
   if (P.profiling) check(hipEventRecord(P.events[2 * k], s), "event");
   return P.s_counters.as<unsigned long long>() + 2 * k;
 }
@@ -332,6 +363,22 @@ void upload_basis(Plan &P, const ig_basis &b) {
   P.d_off_cart.upload(P.sh_off_cart);
   P.d_atom.upload(P.sh_atom);
   P.d_c2s.upload(c2s);
+  // shells grouped by angular momentum: rank of each shell inside its group
+  // and the groups' shell lists (compact per-l column spaces)
+  std::vector<int> rank(P.nsh), lshells;
+  for (int l = 0; l <= kMaxL; ++l) {
+    P.bdev.lshell_off[l] = int(lshells.size());
+    for (int s = 0; s < P.nsh; ++s)
+      if (P.sh_l[s] == l) {
+        rank[s] = int(lshells.size()) - P.bdev.lshell_off[l];
+        lshells.push_back(s);
+      }
+  }
+  P.bdev.lshell_off[kMaxL + 1] = int(lshells.size());
+  P.d_rank.upload(rank);
+  P.d_lshells.upload(lshells);
+  P.bdev.rank = P.d_rank.as<int>();
+  P.bdev.lshells = P.d_lshells.as<int>();
   P.bdev.nsh = P.nsh;
   P.bdev.nao = P.nao;
   P.bdev.nao_cart = P.nao_cart;
@@ -530,38 +577,61 @@ int count_kets(const double *qy, int ny, double qb, double tau) {
 
 // implicit quartet sets for every (X >= Y) class pair at threshold tau;
 // pairs with Q == 0 contribute nothing (their integrals vanish) and are skipped.
-void build_rowptrs(Plan &P, double tau) {
-  if (P.cached_tau == tau && !P.cached_cps.empty()) return;
+// implicit quartet sets for every (X >= Y) class pair at threshold tau;
+// pairs with Q == 0 contribute nothing (their integrals vanish) and are
+// skipped.  With cut > tau each class pair has two sets: kets with
+// Q_P Q_Q >= cut (a prefix of the row) in FP64 and the rest of the row, down
+// to tau, as the FP32 set.
+void build_rowptrs(Plan &P, double tau, double cut) {
+  if (!(cut > tau)) cut = tau;
+  if (P.cached_tau == tau && P.cached_cut == cut && !P.cached_cps.empty()) return;
   P.cached_cps.clear();
   P.cached_rowptr_host.clear();
+  P.cached_qbase_host.clear();
+  std::vector<int> lo, hi;
   for (int x = 0; x < kNumPairClass; ++x) {
     const ClassRange &X = P.cls[x];
     if (!X.n) continue;
     for (int y = 0; y <= x; ++y) {
       const ClassRange &Y = P.cls[y];
       if (!Y.n) continue;
-      Plan::CP cp{x, y, P.cached_rowptr_host.size(), 0};
-      long long acc = 0;
-      P.cached_rowptr_host.push_back(0);
       const double *qy = P.p_q.data() + Y.off;
+      lo.assign(X.n, 0);
+      hi.assign(X.n, 0);
       for (int r = 0; r < X.n; ++r) {
-        double qb = P.p_q[X.off + r];
-        int cnt = 0;
-        if (qb != 0.0) {
-          cnt = count_kets(qy, Y.n, qb, tau);
-          if (x == y) cnt = std::min(cnt, r + 1);
-          while (cnt > 0 && qy[cnt - 1] == 0.0) --cnt;  // Q == 0 kets
-        }
-        acc += cnt;
-        P.cached_rowptr_host.push_back(acc);
+        const double qb = P.p_q[X.off + r];
+        if (qb == 0.0) continue;
+        int cnt = count_kets(qy, Y.n, qb, tau);
+        if (x == y) cnt = std::min(cnt, r + 1);
+        while (cnt > 0 && qy[cnt - 1] == 0.0) --cnt;  // Q == 0 kets
+        hi[r] = cnt;
+        lo[r] = cut > tau ? std::min(cnt, count_kets(qy, Y.n, qb, cut)) : cnt;
       }
-      cp.total = acc;
-      P.cached_cps.push_back(cp);
+      for (int part = 0; part < (cut > tau ? 2 : 1); ++part) {
+        Plan::CP cp{x, y, P.cached_rowptr_host.size(), 0};
+        long long acc = 0;
+        P.cached_rowptr_host.push_back(0);
+        for (int r = 0; r < X.n; ++r) {
+          acc += part == 0 ? lo[r] : hi[r] - lo[r];
+          P.cached_rowptr_host.push_back(acc);
+        }
+        cp.total = acc;
+        if (part == 1) {
+          cp.fp32 = 1;
+          cp.qb_off = (long long)P.cached_qbase_host.size();
+          P.cached_qbase_host.insert(P.cached_qbase_host.end(), lo.begin(), lo.end());
+        }
+        P.cached_cps.push_back(cp);
+      }
     }
   }
   P.s_rowptr.upload(P.cached_rowptr_host);
+  if (!P.cached_qbase_host.empty()) P.s_qbase.upload(P.cached_qbase_host);
   P.cached_tau = tau;
+  P.cached_cut = cut;
 }
+
+void build_rowptrs(Plan &P, double tau) { build_rowptrs(P, tau, tau); }
 
 QuartetSet implicit_set(Plan &P, const Plan::CP &cp) {
   QuartetSet qs{};
@@ -571,6 +641,7 @@ QuartetSet implicit_set(Plan &P, const Plan::CP &cp) {
   qs.y_n = P.cls[cp.y].n;
   qs.same = cp.x == cp.y;
   qs.rowptr = P.s_rowptr.as<long long>() + cp.rp_off;
+  qs.qbase = cp.qb_off >= 0 ? P.s_qbase.as<int>() + cp.qb_off : nullptr;
   qs.list = nullptr;
   qs.total = cp.total;
   return qs;
@@ -664,12 +735,23 @@ int jk_variant(const Plan &P, const ClassRange &X, const ClassRange &Y) {
   return v >= 0 ? v : jk_default_variant(X.l1, X.l2, Y.l1, Y.l2);
 }
 
+// fp32: the set's quartets run with FP32 tables and block (lane kernels only;
+// a class scheduled on the team kernel takes the register kernel, and the
+// runtime-l classes stay FP64).
 hipError_t jk_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const QuartetSet &qs,
-                     const DensDev &dd, unsigned long long *ctr, hipStream_t s, int variant = -1) {
+                     const DensDev &dd, unsigned long long *ctr, hipStream_t s, int variant = -1,
+                     bool fp32 = false) {
   RysDevice &R = rys_device();
-  if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
-    return launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0,
-                     variant >= 0 ? variant : jk_variant(P, X, Y), ctr, s);
+  if (class_supported(X.l1, X.l2, Y.l1, Y.l2)) {
+    int v = variant >= 0 ? variant : jk_variant(P, X, Y);
+    if (fp32 && v == kJkTeam) v = kJkReg;
+    if (!jk_variant_fits(v, X.l1, X.l2, Y.l1, Y.l2, P.bdev, dd.neff, qs.list != nullptr))
+      v = kJkReg;
+    if (fp32)
+      return launch_jk_lane_f32(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, v,
+                                ctr, s);
+    return launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, v, ctr, s);
+  }
   return launch_jk_big(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0,
                        P.s_big.as<double>(), ctr, s);
 }
@@ -693,8 +775,10 @@ DensDev prep_densities(Plan &P, const double *d_dens, int ndens, bool has_anti, 
   P.s_da.reserve(n2 * ndens * sizeof(double) * (has_anti ? 1 : 0) + 8);
   P.s_dc.reserve(nc2 * neff * sizeof(double));
   P.s_bound.reserve(size_t(P.nsh) * P.nsh * ndens * sizeof(double));
-  P.s_jc.reserve(nc2 * neff * sizeof(double));
-  if (want_k) P.s_kc.reserve(nc2 * neff * sizeof(double));
+  // per-XCD copies while each stays under 1 GiB (large bases: one copy)
+  const int ncopy = nc2 * neff * xcd_copies() * sizeof(double) <= (size_t(1) << 30) ? xcd_copies() : 1;
+  P.s_jc.reserve(nc2 * neff * ncopy * sizeof(double));
+  if (want_k) P.s_kc.reserve(nc2 * neff * ncopy * sizeof(double));
   if (P.eff_key != neff * 128 + ndens) {  // (source, antisymmetric) per effective density
     std::vector<int> eff(2 * neff);
     for (int e = 0; e < neff; ++e) {
@@ -707,9 +791,11 @@ DensDev prep_densities(Plan &P, const double *d_dens, int ndens, bool has_anti, 
   check(launch_density_prep(P.bdev, d_dens, ndens, has_anti, P.s_ds.as<double>(),
                             P.s_da.as<double>(), P.s_dc.as<double>(), P.s_bound.as<double>(), s),
         "density prep");
-  check(launch_fill(P.s_jc.as<double>(), nc2 * neff, 0.0, s), "zero J");
-  if (want_k) check(launch_fill(P.s_kc.as<double>(), nc2 * neff, 0.0, s), "zero K");
+  check(launch_fill(P.s_jc.as<double>(), nc2 * neff * ncopy, 0.0, s), "zero J");
+  if (want_k) check(launch_fill(P.s_kc.as<double>(), nc2 * neff * ncopy, 0.0, s), "zero K");
   DensDev dd{};
+  dd.copy_stride = nc2 * neff;
+  dd.ncopy = ncopy;
   dd.ndens = ndens;
   dd.neff = neff;
   dd.dc = P.s_dc.as<double>();
@@ -729,7 +815,7 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
   begin_launches(P, s);
   const DensDev dd = prep_densities(P, d_dens, ndens, has_anti, tau, want_k, s);
   if (!ex) {
-    build_rowptrs(P, tau);
+    build_rowptrs(P, tau, P.fp32_below);
     size_t big = 0;
     for (auto &cp : P.cached_cps) big = std::max(big, big_need(P.cls[cp.x], P.cls[cp.y], cp.total, 0));
     P.s_big.reserve(big * sizeof(double));
@@ -738,8 +824,8 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
       if (!it->total) continue;
       QuartetSet qs = implicit_set(P, *it);
       const ClassRange &X = P.cls[it->x], &Y = P.cls[it->y];
-      unsigned long long *ctr = next_launch(P, X, Y, s);
-      check(jk_launch(P, X, Y, qs, dd, ctr, s), "jk launch");
+      unsigned long long *ctr = next_launch(P, X, Y, s, it->fp32);
+      check(jk_launch(P, X, Y, qs, dd, ctr, s, -1, it->fp32 != 0), "jk launch");
       end_launch(P, s);
     }
   } else {
@@ -771,6 +857,7 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
       end_launch(P, s);
     }
   }
+  check(launch_fold_copies(dd, size_t(P.nao_cart) * P.nao_cart, s), "fold copies");
   check(launch_jk_finish(P.bdev, dd, d_J, d_K, s), "jk finish");
   P.stats_stream = s;
   P.stats_pending = true;
@@ -974,6 +1061,7 @@ extern "C" int ig_direct_autotune(ig_direct *d, const double *d_dens, size_t nde
         int best = -1;
         float best_ms = 0.0f;
         for (int v = 0; v < kJkNumVariants; ++v) {
+          if (!jk_variant_fits(v, X.l1, X.l2, Y.l1, Y.l2, P.bdev, dd.neff, false)) continue;
           float vmin = -1.0f;
           for (int r = 0; r < reps; ++r) {
             check(hipEventRecord(e0, s), "event");
@@ -1059,6 +1147,15 @@ extern "C" int ig_direct_schedule_write(ig_direct *d, const ig_schedule_entry *i
   });
 }
 
+extern "C" int ig_direct_set_mixed_precision(ig_direct *d, double fp32_below) {
+  if (!d) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (!(std::isfinite(fp32_below) && fp32_below >= 0.0))
+    return fail(IG_ERR_INVALID_ARGUMENT, "fp32_below must be finite and >= 0");
+  std::lock_guard<std::mutex> g(d->plan->mu);
+  d->plan->fp32_below = fp32_below;
+  return IG_OK;
+}
+
 extern "C" int ig_direct_set_profiling(ig_direct *d, int enable) {
   if (!d) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> g(d->plan->mu);
@@ -1075,6 +1172,7 @@ extern "C" int ig_direct_profile_read(const ig_direct *d, ig_class_profile *out,
     finalize_stats(P);
     *n = P.launches.size();
     for (size_t k = 0; k < P.launches.size() && k < cap; ++k) {
+      out[k].fp32 = P.launches[k].fp32;
       out[k].la = P.launches[k].la;
       out[k].lb = P.launches[k].lb;
       out[k].lc = P.launches[k].lc;

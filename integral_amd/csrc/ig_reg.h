@@ -1,6 +1,6 @@
 // ig_reg.h -- lane-per-quartet J/K path (included inside namespace ig by
-// ig_kernels.hip).  Rys roots and weights come from Chebyshev tables staged
-// in LDS once per workgroup; per-root 2D tables and the contracted block live
+// ig_lane.h).  Rys roots and weights come from the polynomial tables
+// staged in LDS once per workgroup; per-root 2D tables and the contracted block live
 // in registers with compile-time indices; J/K are digested from registers.
 //
 // Blocks larger than one lane's register budget are cut into NS element
@@ -15,7 +15,7 @@
 // reduced across the wavefront before one atomic per element.
 
 // Rys coefficient tables of NR roots copied to LDS: roots then weights,
-// nsub * NR * 17 doubles each.
+// nsub * NR * kRysTerms doubles each.
 template <int NR>
 __device__ __forceinline__ void stage_rys(const RysDev &R, double *lds) {
   const int n = R.nsub[NR] * NR * kRysTerms;
@@ -55,11 +55,13 @@ __device__ __forceinline__ void rys_all_lds(const RysDev &R, const double *lds, 
 // Elements [LO, HI) of the contracted Cartesian block of (pb | pk) into
 // acc[e - LO].  Ket primitives in the outer loop (per-lane data), bra
 // primitives inside (usually uniform across the wavefront).
-template <class C, int LO, int HI>
+// F is the arithmetic type of the tables and the block (float for the
+// mixed-precision quartets); primitive-quartet scalars and the Rys roots and
+// weights are always evaluated in double and rounded once.
+template <class C, int LO, int HI, typename F>
 __device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDev &R,
                                                   const double *rys_lds, int pb, int pk,
-                                                  const QuartetGeom &g, double omega,
-                                                  double *acc) {
+                                                  const QuartetGeom &g, double omega, F *acc) {
   constexpr int NR = C::NR, T = C::T;
   constexpr ElemTab<C> O{};
   double AB[3], CD[3];
@@ -106,16 +108,16 @@ __device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDe
       for (int r = 0; r < NR; ++r) {
         const double x = rx[r] * s_att;
         const double fq = x * qz * ipq, fp = x * pz * ipq;
-        const double b00 = 0.5 * x * ipq;
-        const double b10 = (1.0 - fq) * i2p;
-        const double b01 = (1.0 - fp) * i2q;
-        double I[3][T];
+        const F b00 = F(0.5 * x * ipq);
+        const F b10 = F((1.0 - fq) * i2p);
+        const F b01 = F((1.0 - fp) * i2q);
+        F I[3][T];
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
-          rys_axis_table<C::LA, C::LB, C::LC, C::LD>((P[ax] - g.A[ax]) - fq * PQ[ax],
-                                                     QC[ax] + fp * PQ[ax], b10, b01, b00, AB[ax],
-                                                     CD[ax], I[ax]);
-        const double pw = pref * rwt[r];
+          rys_axis_table<C::LA, C::LB, C::LC, C::LD, F>(
+              F((P[ax] - g.A[ax]) - fq * PQ[ax]), F(QC[ax] + fp * PQ[ax]), b10, b01, b00,
+              F(AB[ax]), F(CD[ax]), I[ax]);
+        const F pw = F(pref * rwt[r]);
 #pragma unroll
         for (int e = LO; e < HI; ++e)
           acc[e - LO] += pw * (I[0][O.x[e]] * I[1][O.y[e]] * I[2][O.z[e]]);
@@ -136,11 +138,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 // across the wavefront first and written by `leader`, the first lane with
 // work (its geometry names that shared bra pair; a lane without work may hold
 // another one).
-template <class C, int LO, int HI>
+template <class C, int LO, int HI, typename F>
 __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
                                            const QuartetGeom &g, bool same_pair,
                                            unsigned long long mask, bool on, bool bra_uniform,
-                                           int leader, const double *acc) {
+                                           int leader, const F *acc) {
   constexpr int nb = C::nb, nc = C::nc, nd = C::nd;
   constexpr int A0 = LO / (nb * nc * nd), A1 = (HI - 1) / (nb * nc * nd) + 1;  // a rows touched
 #define IG_E(a, bb, c, d) (((a) * nb + (bb)) * nc + (c)) * nd + (d)
@@ -150,12 +152,13 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
   const size_t a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
                d0 = b.off_cart[g.sD];
   const int lane = threadIdx.x & 63;
+  const size_t xo = jk_copy_offset(dd);
   for (int e = 0; e < dd.neff; ++e) {
     const bool use = on && ((mask >> dd.eff_src[e]) & 1ull);
     const bool anti = dd.eff_anti[e];  // uniform
     const double *D = dd.dc + (size_t)e * n2;
     if (!anti) {
-      double *J = dd.jc + (size_t)e * n2;
+      double *J = dd.jc + xo + (size_t)e * n2;
       double jcd[nc * nd];
 #pragma unroll
       for (int q = 0; q < nc * nd; ++q) jcd[q] = 0.0;
@@ -174,7 +177,7 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
               for (int d = 0; d < nd; ++d) {
                 const int ee = IG_E(a, bb, c, d);
                 if (IG_IN(ee)) {
-                  const double gv = acc[ee - LO];
+                  const double gv = double(acc[ee - LO]);
                   v += gv * D[(c0 + c) * n + d0 + d];
                   jcd[c * nd + d] += gv * dab;
                 }
@@ -182,9 +185,9 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
           }
           if (bra_uniform) {  // f depends on the lane's ket: scale before summing
             v = wave_sum(2.0 * f * v);
-            if (lane == leader && v != 0.0) atomicAdd(J + (a0 + a) * n + b0 + bb, v);
+            if (lane == leader && v != 0.0) jk_add(dd, J + (a0 + a) * n + b0 + bb, v);
           } else if (use) {
-            atomicAdd(J + (a0 + a) * n + b0 + bb, 2.0 * f * v);
+            jk_add(dd, J + (a0 + a) * n + b0 + bb, 2.0 * f * v);
           }
         }
       if (use) {
@@ -192,11 +195,11 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
         for (int c = 0; c < nc; ++c)
 #pragma unroll
           for (int d = 0; d < nd; ++d)
-            atomicAdd(J + (c0 + c) * n + d0 + d, 2.0 * f * jcd[c * nd + d]);
+            jk_add(dd, J + (c0 + c) * n + d0 + d, 2.0 * f * jcd[c * nd + d]);
       }
     }
     if (use && dd.kc && dd.want_k) {
-      double *K = dd.kc + (size_t)e * n2;
+      double *K = dd.kc + xo + (size_t)e * n2;
 #pragma unroll
       for (int a = A0; a < A1; ++a)
 #pragma unroll
@@ -208,10 +211,10 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
 #pragma unroll
             for (int d = 0; d < nd; ++d)
               if (IG_IN(IG_E(a, bb, c, d))) {
-                v += acc[IG_E(a, bb, c, d) - LO] * D[(b0 + bb) * n + d0 + d];
+                v += double(acc[IG_E(a, bb, c, d) - LO]) * D[(b0 + bb) * n + d0 + d];
                 any = true;
               }
-          if (any) atomicAdd(K + (a0 + a) * n + c0 + c, f * v);
+          if (any) jk_add(dd, K + (a0 + a) * n + c0 + c, f * v);
         }
 #pragma unroll
       for (int bb = 0; bb < nb; ++bb)
@@ -224,10 +227,10 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
 #pragma unroll
             for (int d = 0; d < nd; ++d)
               if (IG_IN(IG_E(a, bb, c, d))) {
-                v += acc[IG_E(a, bb, c, d) - LO] * D[(a0 + a) * n + d0 + d];
+                v += double(acc[IG_E(a, bb, c, d) - LO]) * D[(a0 + a) * n + d0 + d];
                 any = true;
               }
-          if (any) atomicAdd(K + (b0 + bb) * n + c0 + c, f * v);
+          if (any) jk_add(dd, K + (b0 + bb) * n + c0 + c, f * v);
         }
 #pragma unroll
       for (int a = A0; a < A1; ++a)
@@ -240,10 +243,10 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
 #pragma unroll
             for (int c = 0; c < nc; ++c)
               if (IG_IN(IG_E(a, bb, c, d))) {
-                v += acc[IG_E(a, bb, c, d) - LO] * D[(b0 + bb) * n + c0 + c];
+                v += double(acc[IG_E(a, bb, c, d) - LO]) * D[(b0 + bb) * n + c0 + c];
                 any = true;
               }
-          if (any) atomicAdd(K + (a0 + a) * n + d0 + d, f * v);
+          if (any) jk_add(dd, K + (a0 + a) * n + d0 + d, f * v);
         }
 #pragma unroll
       for (int bb = 0; bb < nb; ++bb)
@@ -256,10 +259,10 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
 #pragma unroll
             for (int c = 0; c < nc; ++c)
               if (IG_IN(IG_E(a, bb, c, d))) {
-                v += acc[IG_E(a, bb, c, d) - LO] * D[(a0 + a) * n + c0 + c];
+                v += double(acc[IG_E(a, bb, c, d) - LO]) * D[(a0 + a) * n + c0 + c];
                 any = true;
               }
-          if (any) atomicAdd(K + (b0 + bb) * n + d0 + d, f * v);
+          if (any) jk_add(dd, K + (b0 + bb) * n + d0 + d, f * v);
         }
     }
   }
@@ -271,7 +274,7 @@ __device__ __forceinline__ void digest_reg(const BasisDev &b, const DensDev &dd,
 // wavefront runs the same number of iterations (lanes past the range or with
 // an empty density mask compute nothing) so wavefront reductions see all 64
 // lanes.  Only slice 0 counts quartets.
-template <class C, int LO, int HI>
+template <class C, int LO, int HI, typename F>
 __global__ void __launch_bounds__(64) jk_reg_kernel(BasisDev b, PairsDev p, RysDev R,
                                                     QuartetSet qs, DensDev dd, double omega,
                                                     unsigned long long *counters) {
@@ -306,11 +309,11 @@ __global__ void __launch_bounds__(64) jk_reg_kernel(BasisDev b, PairsDev p, RysD
       }
       on = mask != 0;
     }
-    double acc[HI - LO];
+    F acc[HI - LO];
 #pragma unroll
-    for (int e = 0; e < HI - LO; ++e) acc[e] = 0.0;
+    for (int e = 0; e < HI - LO; ++e) acc[e] = F(0);
     if (on) {
-      quartet_block_reg<C, LO, HI>(p, R, rys_lds, pb, pk, g, omega, acc);
+      quartet_block_reg<C, LO, HI, F>(p, R, rys_lds, pb, pk, g, omega, acc);
       if (LO == 0) {
         ++my_quartets;
         my_prims += (unsigned long long)p.pp_n[pb] * p.pp_n[pk];
@@ -321,7 +324,7 @@ __global__ void __launch_bounds__(64) jk_reg_kernel(BasisDev b, PairsDev p, RysD
     const int leader = __ffsll((unsigned long long)__ballot(on)) - 1;
     const int pb_first = __shfl(pb, leader < 0 ? 0 : leader);
     const bool bra_uniform = __all(!on || pb == pb_first);
-    digest_reg<C, LO, HI>(b, dd, g, pb == pk, mask, on, bra_uniform, leader, acc);
+    digest_reg<C, LO, HI, F>(b, dd, g, pb == pk, mask, on, bra_uniform, leader, acc);
   }
   if (LO == 0) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -335,25 +338,27 @@ __global__ void __launch_bounds__(64) jk_reg_kernel(BasisDev b, PairsDev p, RysD
   }
 }
 
-// number of register slices for a class and the launch of all of them
-template <class C>
+// number of register slices for a class and the launch of all of them (a
+// float block holds twice the elements of a double one in the same VGPRs)
+template <class C, typename F>
 struct RegSlices {
-  static constexpr int NS = (C::N4 + kRegBlockMax - 1) / kRegBlockMax;
+  static constexpr int CAP = kRegBlockMax * int(sizeof(double) / sizeof(F));
+  static constexpr int NS = (C::N4 + CAP - 1) / CAP;
   static constexpr int E = (C::N4 + NS - 1) / NS;
 };
 
-template <class C, int S>
+template <class C, int S, typename F>
 hipError_t launch_jk_reg_slices(const BasisDev &b, const PairsDev &p, const RysDev &r,
                                 const QuartetSet &qs, const DensDev &d, double omega,
                                 unsigned long long *ctr, hipStream_t s, int blocks, size_t lds) {
-  constexpr int NS = RegSlices<C>::NS, E = RegSlices<C>::E;
+  constexpr int NS = RegSlices<C, F>::NS, E = RegSlices<C, F>::E;
   if constexpr (S < NS) {
     constexpr int LO = S * E, HI = (S + 1) * E < C::N4 ? (S + 1) * E : C::N4;
-    hipLaunchKernelGGL((jk_reg_kernel<C, LO, HI>), dim3(blocks), dim3(64), lds, s, b, p, r, qs,
-                       d, omega, ctr);
+    hipLaunchKernelGGL((jk_reg_kernel<C, LO, HI, F>), dim3(blocks), dim3(64), lds, s, b, p, r,
+                       qs, d, omega, ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_jk_reg_slices<C, S + 1>(b, p, r, qs, d, omega, ctr, s, blocks, lds);
+    return launch_jk_reg_slices<C, S + 1, F>(b, p, r, qs, d, omega, ctr, s, blocks, lds);
   } else {
     return hipSuccess;
   }

@@ -1,0 +1,330 @@
+// ig_rowacc.h -- lane-per-quartet J/K path with per-row LDS accumulation
+// (included inside namespace ig by ig_lane.h, after ig_reg.h).
+//
+// Device-scope FP64 atomics are the ceiling of the plain register path: every
+// quartet sends na*nc + nb*nc + na*nd + nb*nd exchange updates (and the
+// ket-side Coulomb ones) to memory, and a whole build issues them at a fixed
+// rate whatever the class.  Here a workgroup of kRowWaves wavefronts walks a
+// contiguous range of the screened quartet list, 64 quartets per wavefront per
+// step, and keeps the running sums of the bra row under accumulation (the row
+// of the first quartet of the step) in LDS:
+//   JB[e][a][b]                   bra-side Coulomb
+//   KA[e][a][col], KB[e][b][col]  exchange rows of the bra's two shells
+// `col` runs over a compact column space: the Cartesian functions of every
+// shell with the ket class's first momentum, then (when it differs) of every
+// shell with its second.  Updates go to LDS with LDS atomics; when the row
+// changes the workgroup adds the nonzero entries to global memory with one
+// atomic each and clears them.  A row's kets share few shells, so each entry
+// collects many quartets.  Quartets of another row inside a step (row
+// boundaries) and the ket-side Coulomb updates take the global path of
+// jk_reg_kernel.
+
+constexpr int kRowWaves = 4;
+
+// Compact column space of the ket class (momenta LC >= LD) of class C.
+template <class C>
+struct RowCols {
+  int W, dbase;
+  __device__ explicit RowCols(const BasisDev &b) {
+    const int nC = b.lshell_off[C::LC + 1] - b.lshell_off[C::LC];
+    const int nD = b.lshell_off[C::LD + 1] - b.lshell_off[C::LD];
+    dbase = (C::LC == C::LD) ? 0 : nC * C::nc;
+    W = nC * C::nc + (C::LC == C::LD ? 0 : nD * C::nd);
+  }
+  __device__ int col_c(const BasisDev &b, int sC, int c) const { return b.rank[sC] * C::nc + c; }
+  __device__ int col_d(const BasisDev &b, int sD, int d) const {
+    return dbase + b.rank[sD] * C::nd + d;
+  }
+  // column -> Cartesian function index
+  __device__ int cart(const BasisDev &b, int col) const {
+    if (C::LC == C::LD || col < dbase)
+      return b.off_cart[b.lshells[b.lshell_off[C::LC] + col / C::nc]] + col % C::nc;
+    col -= dbase;
+    return b.off_cart[b.lshells[b.lshell_off[C::LD] + col / C::nd]] + col % C::nd;
+  }
+};
+
+// Doubles of LDS the accumulators of class C need for `neff` densities and W
+// columns (host and device agree on this layout: JB, KA, KB).
+__host__ __device__ constexpr size_t rowacc_doubles(int na, int nb, int neff, int W) {
+  return size_t(neff) * (size_t(na) * nb + size_t(na + nb) * W);
+}
+
+// Add the accumulated row `row` to global J'/K' and clear the accumulators.
+template <class C>
+__device__ void rowacc_flush(const BasisDev &b, const PairsDev &p, const QuartetSet &qs,
+                             const DensDev &dd, const RowCols<C> &rc, int row, double *JB,
+                             size_t xo) {
+  constexpr int na = C::na, nb = C::nb;
+  const int pb = qs.x_off + row;
+  const size_t a0 = b.off_cart[p.sh1[pb]], b0 = b.off_cart[p.sh2[pb]];
+  const size_t n = b.nao_cart, n2 = n * n;
+  const int neff = dd.neff, W = rc.W;
+  const int nJ = neff * na * nb, nKA = neff * na * W;
+  const int ntot = int(rowacc_doubles(na, nb, neff, W));
+  for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
+    const double v = JB[i];
+    if (v == 0.0) continue;
+    JB[i] = 0.0;
+    double *dst;
+    if (i < nJ) {
+      const int e = i / (na * nb), r = i % (na * nb);
+      dst = dd.jc + xo + e * n2 + (a0 + r / nb) * n + b0 + r % nb;
+    } else if (i < nJ + nKA) {
+      const int k = i - nJ, e = k / (na * W), r = k % (na * W);
+      dst = dd.kc + xo + e * n2 + (a0 + r / W) * n + rc.cart(b, r % W);
+    } else {
+      const int k = i - nJ - nKA, e = k / (nb * W), r = k % (nb * W);
+      dst = dd.kc + xo + e * n2 + (b0 + r / W) * n + rc.cart(b, r % W);
+    }
+    atomicAdd(dst, v);
+  }
+  __syncthreads();
+}
+
+// Digest of the block elements [LO, HI) of lanes in the accumulated row:
+// bra-side Coulomb and exchange into LDS, ket-side Coulomb to global memory.
+// Called by every lane (`in` false for lanes that do not take part).
+template <class C, int LO, int HI, typename F>
+__device__ __forceinline__ void digest_row(const BasisDev &b, const DensDev &dd,
+                                           const QuartetGeom &g, bool same_pair,
+                                           unsigned long long mask, bool in, bool all_in,
+                                           const RowCols<C> &rc, double *JB, size_t xo,
+                                           const F *acc) {
+  constexpr int nb = C::nb, nc = C::nc, nd = C::nd;
+  constexpr int na = C::na;
+  constexpr int A0 = LO / (nb * nc * nd), A1 = (HI - 1) / (nb * nc * nd) + 1;
+#define IG_E(a, bb, c, d) (((a) * nb + (bb)) * nc + (c)) * nd + (d)
+#define IG_IN(e) ((e) >= LO && (e) < HI)
+  const double f = (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (same_pair ? 0.5 : 1.0);
+  const size_t n = (size_t)b.nao_cart, n2 = n * n;
+  const size_t a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
+               d0 = b.off_cart[g.sD];
+  const int lane = threadIdx.x & 63;
+  const int W = rc.W;
+  const int colc = in ? rc.col_c(b, g.sC, 0) : 0, cold = in ? rc.col_d(b, g.sD, 0) : 0;
+  for (int e = 0; e < dd.neff; ++e) {
+    const bool use = in && ((mask >> dd.eff_src[e]) & 1ull);
+    const bool anti = dd.eff_anti[e];  // uniform
+    const double *D = dd.dc + (size_t)e * n2;
+    if (!anti) {
+      double *J = dd.jc + xo + (size_t)e * n2;
+      double *jb = JB + e * na * nb;
+      double jcd[nc * nd];
+#pragma unroll
+      for (int q = 0; q < nc * nd; ++q) jcd[q] = 0.0;
+#pragma unroll
+      for (int a = A0; a < A1; ++a)
+#pragma unroll
+        for (int bb = 0; bb < nb; ++bb) {
+          if (IG_E(a, bb, 0, 0) >= HI || IG_E(a, bb, nc - 1, nd - 1) < LO) continue;
+          double v = 0.0;
+          if (use) {
+            const double dab = D[(a0 + a) * n + b0 + bb];
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+#pragma unroll
+              for (int d = 0; d < nd; ++d) {
+                const int ee = IG_E(a, bb, c, d);
+                if (IG_IN(ee)) {
+                  const double gv = double(acc[ee - LO]);
+                  v += gv * D[(c0 + c) * n + d0 + d];
+                  jcd[c * nd + d] += gv * dab;
+                }
+              }
+          }
+          if (all_in) {  // one row for the whole wavefront: reduce, one LDS add
+            v = wave_sum(2.0 * f * v);
+            if (lane == 0 && v != 0.0) atomicAdd(jb + a * nb + bb, v);
+          } else if (use) {
+            atomicAdd(jb + a * nb + bb, 2.0 * f * v);
+          }
+        }
+      if (use) {
+#pragma unroll
+        for (int c = 0; c < nc; ++c)
+#pragma unroll
+          for (int d = 0; d < nd; ++d)
+            atomicAdd(J + (c0 + c) * n + d0 + d, 2.0 * f * jcd[c * nd + d]);
+      }
+    }
+    if (use && dd.kc && dd.want_k) {
+      double *ka = JB + dd.neff * na * nb + (size_t)e * na * W;
+      double *kb = JB + dd.neff * (na * nb + na * W) + (size_t)e * nb * W;
+#pragma unroll
+      for (int a = A0; a < A1; ++a)
+#pragma unroll
+        for (int c = 0; c < nc; ++c) {  // Kp[a,c] += g D[b,d]
+          double v = 0.0;
+          bool any = false;
+#pragma unroll
+          for (int bb = 0; bb < nb; ++bb)
+#pragma unroll
+            for (int d = 0; d < nd; ++d)
+              if (IG_IN(IG_E(a, bb, c, d))) {
+                v += double(acc[IG_E(a, bb, c, d) - LO]) * D[(b0 + bb) * n + d0 + d];
+                any = true;
+              }
+          if (any) atomicAdd(ka + a * W + colc + c, f * v);
+        }
+#pragma unroll
+      for (int bb = 0; bb < nb; ++bb)
+#pragma unroll
+        for (int c = 0; c < nc; ++c) {  // Kp[b,c] += g D[a,d]
+          double v = 0.0;
+          bool any = false;
+#pragma unroll
+          for (int a = A0; a < A1; ++a)
+#pragma unroll
+            for (int d = 0; d < nd; ++d)
+              if (IG_IN(IG_E(a, bb, c, d))) {
+                v += double(acc[IG_E(a, bb, c, d) - LO]) * D[(a0 + a) * n + d0 + d];
+                any = true;
+              }
+          if (any) atomicAdd(kb + bb * W + colc + c, f * v);
+        }
+#pragma unroll
+      for (int a = A0; a < A1; ++a)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {  // Kp[a,d] += g D[b,c]
+          double v = 0.0;
+          bool any = false;
+#pragma unroll
+          for (int bb = 0; bb < nb; ++bb)
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+              if (IG_IN(IG_E(a, bb, c, d))) {
+                v += double(acc[IG_E(a, bb, c, d) - LO]) * D[(b0 + bb) * n + c0 + c];
+                any = true;
+              }
+          if (any) atomicAdd(ka + a * W + cold + d, f * v);
+        }
+#pragma unroll
+      for (int bb = 0; bb < nb; ++bb)
+#pragma unroll
+        for (int d = 0; d < nd; ++d) {  // Kp[b,d] += g D[a,c]
+          double v = 0.0;
+          bool any = false;
+#pragma unroll
+          for (int a = A0; a < A1; ++a)
+#pragma unroll
+            for (int c = 0; c < nc; ++c)
+              if (IG_IN(IG_E(a, bb, c, d))) {
+                v += double(acc[IG_E(a, bb, c, d) - LO]) * D[(a0 + a) * n + c0 + c];
+                any = true;
+              }
+          if (any) atomicAdd(kb + bb * W + cold + d, f * v);
+        }
+    }
+  }
+#undef IG_E
+#undef IG_IN
+}
+
+template <class C, int LO, int HI, typename F>
+__global__ void __launch_bounds__(64 * kRowWaves)
+    jk_rowacc_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs, DensDev dd, double omega,
+                     unsigned long long *counters) {
+  extern __shared__ double lds[];
+  __shared__ int s_row[2];  // by step parity: a slow wavefront still reading
+                            // step i's value cannot see step i+1's
+  stage_rys<C::NR>(R, lds);
+  const int nrys = 2 * R.nsub[C::NR] * C::NR * kRysTerms;
+  const RowCols<C> rc(b);
+  double *JB = lds + nrys;
+  const int nacc = int(rowacc_doubles(C::na, C::nb, dd.neff, rc.W));
+  for (int i = threadIdx.x; i < nacc; i += blockDim.x) JB[i] = 0.0;
+  // the barrier at the top of the first step orders these stores
+  unsigned long long my_quartets = 0, my_prims = 0;
+  const int nsh = b.nsh;
+  const int lane = threadIdx.x & 63;
+  const size_t xo = jk_copy_offset(dd);
+  const long long chunk = (qs.total + gridDim.x - 1) / gridDim.x;
+  const long long begin = (long long)blockIdx.x * chunk;
+  const long long end = begin + chunk < qs.total ? begin + chunk : qs.total;
+  RowCursor cur;
+  int racc = -1;  // accumulated row, uniform over the workgroup
+  int step = 0;
+  for (long long base = begin; base < end; base += blockDim.x, ++step) {
+    const long long t = base + threadIdx.x;
+    bool on = t < end;
+    int P = 0, Q = 0;
+    if (on) cur.next(qs, t, P, Q);
+    if (threadIdx.x == 0) s_row[step & 1] = P;  // t = base < end: thread 0 has a quartet
+    __syncthreads();
+    const int row = s_row[step & 1];
+    if (row != racc) {
+      if (racc >= 0) rowacc_flush<C>(b, p, qs, dd, rc, racc, JB, xo);
+      racc = row;
+    }
+    const int pb = qs.x_off + P, pk = qs.y_off + Q;
+    QuartetGeom g;
+    load_geom(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk], g);
+    unsigned long long mask = 0;
+    if (on) {
+      const double pair_bound = p.q[pb] * p.q[pk];
+      for (int d = 0; d < dd.ndens; ++d) {
+        const double *bd = dd.bound + (size_t)d * nsh * nsh;
+        double m = bd[g.sA * nsh + g.sB];
+        m = fmax(m, bd[g.sC * nsh + g.sD]);
+        m = fmax(m, bd[g.sA * nsh + g.sC]);
+        m = fmax(m, bd[g.sA * nsh + g.sD]);
+        m = fmax(m, bd[g.sB * nsh + g.sC]);
+        m = fmax(m, bd[g.sB * nsh + g.sD]);
+        if (pair_bound * m >= dd.tau) mask |= 1ull << d;
+      }
+      on = mask != 0;
+    }
+    F acc[HI - LO];
+#pragma unroll
+    for (int e = 0; e < HI - LO; ++e) acc[e] = F(0);
+    if (on) {
+      quartet_block_reg<C, LO, HI, F>(p, R, lds, pb, pk, g, omega, acc);
+      if (LO == 0) {
+        ++my_quartets;
+        my_prims += (unsigned long long)p.pp_n[pb] * p.pp_n[pk];
+      }
+    }
+    const bool in = on && P == racc;
+    const bool out = on && P != racc;
+    if (__any(in))
+      digest_row<C, LO, HI, F>(b, dd, g, pb == pk, mask, in, __all(!on || in), rc, JB, xo, acc);
+    if (__any(out)) {
+      // other rows: the global path, wavefront-reduced when they share a bra
+      const int leader = __ffsll((unsigned long long)__ballot(out)) - 1;
+      const int pb_first = __shfl(pb, leader);
+      const bool bra_uniform = __all(!out || pb == pb_first);
+      digest_reg<C, LO, HI, F>(b, dd, g, pb == pk, mask, out, bra_uniform, leader, acc);
+    }
+  }
+  __syncthreads();
+  if (racc >= 0) rowacc_flush<C>(b, p, qs, dd, rc, racc, JB, xo);
+  if (LO == 0) {
+    for (int off = 32; off > 0; off >>= 1) {
+      my_quartets += __shfl_down(my_quartets, off);
+      my_prims += __shfl_down(my_prims, off);
+    }
+    if (counters && lane == 0) {
+      atomicAdd(counters, my_quartets);
+      atomicAdd(counters + 1, my_prims);
+    }
+  }
+}
+
+template <class C, int S, typename F>
+hipError_t launch_jk_rowacc_slices(const BasisDev &b, const PairsDev &p, const RysDev &r,
+                                   const QuartetSet &qs, const DensDev &d, double omega,
+                                   unsigned long long *ctr, hipStream_t s, int blocks,
+                                   size_t lds) {
+  constexpr int NS = RegSlices<C, F>::NS, E = RegSlices<C, F>::E;
+  if constexpr (S < NS) {
+    constexpr int LO = S * E, HI = (S + 1) * E < C::N4 ? (S + 1) * E : C::N4;
+    hipLaunchKernelGGL((jk_rowacc_kernel<C, LO, HI, F>), dim3(blocks), dim3(64 * kRowWaves), lds,
+                       s, b, p, r, qs, d, omega, ctr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_jk_rowacc_slices<C, S + 1, F>(b, p, r, qs, d, omega, ctr, s, blocks, lds);
+  } else {
+    return hipSuccess;
+  }
+}

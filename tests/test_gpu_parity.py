@@ -164,10 +164,11 @@ def test_grad_bra_pairs_sum_in_any_order():
     assert np.max(np.abs(parts[0] + parts[1] - serial)) < 1e-12
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_direct_jk_every_kernel_variant(variant):
-    """Both J/K kernels of every compiled class (team/LDS and lane/register)
-    against the oracle, forced through a loaded schedule."""
+    """Every J/K kernel of every compiled class (team/LDS, lane/register and
+    register with per-row LDS sums) against the oracle, forced through a
+    loaded schedule."""
     shells = (B.grad_basis(True, max_l=2) + B.fd_basis()
               + [(2, (1.3, -0.8, 0.9), [0.45], [1.0], True)])
     n = B.nfunc(shells)
@@ -193,7 +194,7 @@ def test_autotune_keeps_results():
     dc = gpu_basis(shells).direct_contractor()
     D = torch.tensor(p[None], dtype=torch.float64, device="cuda:0").contiguous()
     sched = dc.autotune(D, threshold=1e-12, reps=1)
-    assert sched and all(e["tuned"] and e["variant"] in (0, 1) for e in sched)
+    assert sched and all(e["tuned"] and e["variant"] in (0, 1, 2) for e in sched)
     J, K = dc.jk([p], 1e-12)
     oJ, oK = O.Basis(shells).direct_jk([p], 1e-12)
     assert np.max(np.abs(J[0] - oJ[0])) < TOL and np.max(np.abs(K[0] - oK[0])) < TOL
@@ -218,7 +219,7 @@ def _support_density(shells, keep, seed):
     return p * np.outer(rows, rows)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_direct_jk_density_screened_lanes(variant):
     """Quartets dropped by the density mask next to kept ones in the same
     wavefront, across bra rows (regression: the wavefront-reduced bra J sum
@@ -237,16 +238,19 @@ def test_direct_jk_density_screened_lanes(variant):
     assert np.max(np.abs(K[0] - oK[0])) < TOL
 
 
-def test_direct_jk_twelve_atom_molecule():
+@pytest.mark.parametrize("variant", [None, 1, 2])
+def test_direct_jk_twelve_atom_molecule(variant):
     """The benchmark's molecule recipe at 12 atoms (spherical s,s,p,p,d,
     3/3/3/3/1 primitives, nao = 156), two densities, tau = 1e-12: every
-    kernel at a size where each wavefront walks many quartets."""
+    kernel at a size where each wavefront walks many quartets and rows."""
     shells, rng = synthetic.molecule(20260007, 12, 8.0, [(0, 3), (0, 3), (1, 3), (1, 3), (2, 1)],
                                      (0.08, 40.0), True)
     n = B.nfunc(shells)
     dens = [synthetic.density(rng, n, 6), synthetic.density(rng, n, 6)]
-    note("jk 12-atom molecule nao=%d" % n)
+    note("jk 12-atom molecule nao=%d variant %s" % (n, variant))
     dc = gpu_basis(shells).direct_contractor()
+    if variant is not None:
+        dc.set_schedule([dict(e, variant=variant) for e in dc.schedule()])
     J, K = dc.jk(dens, 1e-12)
     ob = O.Basis(shells)
     oJ, oK = ob.direct_jk(dens, 1e-12)
@@ -254,3 +258,34 @@ def test_direct_jk_twelve_atom_molecule():
     for x in range(2):
         assert np.max(np.abs(J[x] - oJ[x])) < TOL
         assert np.max(np.abs(K[x] - oK[x])) < TOL
+
+
+@pytest.mark.parametrize("variant", [None, 1, 2])
+def test_mixed_precision(variant):
+    """FP32 tables for quartets with Q_ij Q_kl below the cut.  At a 1e-8 cut
+    |dJ|, |dK| <= 1e-9 against the oracle (the mixed-precision bar); at 1e-6
+    the error grows with the FP32 share (<= 1e-7 here).  The screening counts
+    do not change and FP64 is restored exactly."""
+    shells, rng = synthetic.molecule(20260007, 12, 8.0, [(0, 3), (0, 3), (1, 3), (1, 3), (2, 1)],
+                                     (0.08, 40.0), True)
+    n = B.nfunc(shells)
+    dens = [synthetic.density(rng, n, 6), synthetic.density(rng, n, 6)]
+    note("mixed precision 12-atom variant %s" % variant)
+    dc = gpu_basis(shells).direct_contractor()
+    if variant is not None:
+        dc.set_schedule([dict(e, variant=variant) for e in dc.schedule()])
+    J64, K64 = dc.jk(dens, 1e-12)
+    q64 = dc.last_stats()[0]
+    oJ, oK = O.Basis(shells).direct_jk(dens, 1e-12)
+    for cut, bar in ((1e-8, 1e-9), (1e-6, 1e-7)):
+        dc.set_mixed_precision(cut)
+        J, K = dc.jk(dens, 1e-12)
+        assert dc.last_stats()[0] == q64
+        err = max(max(np.max(np.abs(J[x] - oJ[x])), np.max(np.abs(K[x] - oK[x]))) for x in range(2))
+        print("[case] cut %.0e max |dJ,dK| %.3e" % (cut, err), flush=True)
+        assert err <= bar, (cut, err)
+        # FP32 really ran: not bit-identical to FP64
+        assert max(np.max(np.abs(J[x] - J64[x])) for x in range(2)) > 0.0
+    dc.set_mixed_precision(0.0)
+    J2, K2 = dc.jk(dens, 1e-12)
+    assert np.max(np.abs(J2[0] - oJ[0])) < TOL and np.max(np.abs(K2[0] - oK[0])) < TOL
