@@ -43,7 +43,7 @@ def class_flops(rec, spherical=True, ndens=2):
     return per * q
 
 
-VARIANTS = ["team", "register", "register+row sums"]
+VARIANTS = ["team", "register", "register+row sums", "OS/HGP+row sums"]
 
 
 def kernel_name(cls, variant):
@@ -56,8 +56,9 @@ def kernel_name(cls, variant):
         return "jk_kernel<Cls<%s>>" % c
     slices = (n4 + 80) // 81
     name = "jk_reg_kernel" if variant == 1 else "jk_rowacc_kernel"
-    return "%s<Cls<%s>, LO, HI, double> (%d element slice%s)" % (name, c, slices,
-                                                                 "s" if slices > 1 else "")
+    eng = ", 1" if variant == 3 else ""
+    return "%s<Cls<%s>, LO, HI, double%s> (%d element slice%s)" % (name, c, eng, slices,
+                                                                   "s" if slices > 1 else "")
 
 
 def cpu_baseline(shells, dens, tau, seconds):

@@ -70,6 +70,14 @@ const char *ig_last_error_message(void);
 /* Number of visible HIP devices (0 when no GPU); never fails. */
 int ig_device_count(void);
 
+/* ---- the quadrature building blocks, evaluated on the device ------------- */
+/* boys_array (math/boys.rs:146-193): F_0..F_m_max (m_max <= 25) at each of the
+ * n points T (host arrays); out receives n*(m_max+1) doubles, point-major. */
+int ig_boys_eval(int m_max, const double *T, size_t n, double *out);
+/* rys_roots_weights (math/rys.rs:174-185): the nroots-point rule (1..13) at
+ * each T; roots (squared, ascending) and weights receive n*nroots doubles. */
+int ig_rys_eval(int nroots, const double *T, size_t n, double *roots, double *weights);
+
 /* ---- basis (shell.rs:64-118 Shell::with_kind, :205 Basis::new) ----------- */
 typedef struct ig_shell_desc {
   int32_t l;             /* angular momentum, 0..=6 (engine/os.rs:34 MAX_L) */
@@ -170,11 +178,13 @@ int ig_direct_set_profiling(ig_direct *direct, int enable);
 int ig_direct_set_mixed_precision(ig_direct *direct, double fp32_below);
 int ig_direct_profile_read(const ig_direct *direct, ig_class_profile *out, size_t cap, size_t *n);
 
-/* Class-to-kernel schedule of a plan.  Every compiled quartet class has three
+/* Class-to-kernel schedule of a plan.  Every compiled quartet class has these
  * J/K kernels: variant 0 runs one quartet per team of lanes with its 2D
  * tables in LDS, variant 1 one quartet per lane in registers, variant 2 the
  * register kernel with the bra row's Coulomb and exchange sums kept in LDS
- * (used for the whole-list calls; per-bra-pair calls fall back to 1).  The
+ * (used for the whole-list calls; per-bra-pair calls fall back to 1), and
+ * variant 3, for total angular momentum <= 2, variant 2 with an Obara-Saika /
+ * Head-Gordon-Pople block from the Boys function instead of Rys roots.  The
  * reference
  * has no counterpart (its CPU engines pick by angular momentum only,
  * integrals.rs:323-329); this replaces a fixed dispatch on the GPU.
@@ -185,8 +195,8 @@ int ig_direct_profile_read(const ig_direct *direct, ig_class_profile *out, size_
  * caller buffer is written.  Synchronises `stream`. */
 typedef struct ig_schedule_entry {
   int32_t la, lb, lc, ld;  /* kernel orientation, pair class (la,lb) >= (lc,ld) */
-  int32_t variant;         /* 0 team/LDS, 1 lane/registers, 2 registers + LDS row sums;
-                              -1 on write = default */
+  int32_t variant;         /* 0 team/LDS, 1 lane/registers, 2 registers + LDS row sums,
+                              3 as 2 with an OS/HGP block; -1 on write = default */
   int32_t tuned;           /* read: 1 if set by the autotuner or a write */
   double ms;               /* read: the autotuner's time for the chosen variant */
 } ig_schedule_entry;

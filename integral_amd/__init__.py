@@ -189,6 +189,8 @@ def library():
     lib.ig_direct_profile_read.argtypes = [_vp, ctypes.POINTER(_ClassProfile), sz,
                                            ctypes.POINTER(sz)]
     lib.ig_direct_set_mixed_precision.argtypes = [_vp, ctypes.c_double]
+    lib.ig_boys_eval.argtypes = [ctypes.c_int, _dp, sz, _dp]
+    lib.ig_rys_eval.argtypes = [ctypes.c_int, _dp, sz, _dp, _dp]
     lib.ig_direct_autotune.argtypes = [_vp, _vp, sz, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                        _vp]
     lib.ig_direct_schedule_read.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz,
@@ -243,6 +245,24 @@ def _stream_ptr(stream):
     if stream is None:
         return None
     return ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
+
+
+def boys(m_max, T):
+    """F_0..F_m_max at the points T, evaluated on the GPU (n x (m_max+1))."""
+    T = _f64(np.atleast_1d(T))
+    out = np.zeros((T.size, m_max + 1))
+    _raise(library().ig_boys_eval(int(m_max), _ptr(T), T.size, _ptr(out)))
+    return out
+
+
+def rys(nroots, T):
+    """Rys roots (squared) and weights of the nroots-point rule at the points
+    T, evaluated on the GPU (two n x nroots arrays)."""
+    T = _f64(np.atleast_1d(T))
+    x = np.zeros((T.size, nroots))
+    w = np.zeros((T.size, nroots))
+    _raise(library().ig_rys_eval(int(nroots), _ptr(T), T.size, _ptr(x), _ptr(w)))
+    return x, w
 
 
 class Shell:
@@ -466,7 +486,7 @@ class DirectContractor:
         _raise(library().ig_direct_set_profiling(self._h, 1 if enable else 0))
 
     # J/K kernel variants of a compiled quartet class
-    TEAM, REGISTER, ROW_ACCUMULATE = 0, 1, 2
+    TEAM, REGISTER, ROW_ACCUMULATE, OS_HGP = 0, 1, 2, 3
 
     def autotune(self, dens, threshold=0.0, want_exchange=True, reps=2, stream=None):
         """Time every J/K kernel variant of every class pair on the device

@@ -49,6 +49,38 @@ __device__ __forceinline__ int rys_cell(const RysDev &R, int n, double T, double
   return s;
 }
 
+// Boys function F_0..F_mmax (mmax <= kBoysMaxM - 8) at T, the reference's
+// method (math/boys.rs:146-193): below T = 93.05 the nearest grid point T0 of
+// the table, a 9-term Taylor series for the top order
+// F_m(T0 + d) = sum_k F_{m+k}(T0) (-d)^k / k!, e^-T = e^-T0 e^-d (Taylor), and
+// the downward recurrence F_{m-1} = (2T F_m + e^-T) / (2m-1); above it the
+// asymptotic F_0 = sqrt(pi/T)/2 and F_m = F_{m-1} (2m-1) / (2T) (e^-T < 1e-40
+// is dropped).
+__device__ __forceinline__ void boys_eval(const double *__restrict__ tab, int mmax, double T,
+                                          double *F) {
+  if (T < 93.05) {
+    const int i = int(T * 10.0 + 0.5);
+    const double x = 0.1 * double(i) - T;  // -d
+    const double *row = tab + i * kBoysCols;
+    constexpr double inv_fact[9] = {1.0, 1.0, 0.5, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
+                                    1.0 / 5040, 1.0 / 40320};
+    double f = row[mmax + 8] * inv_fact[8];
+#pragma unroll
+    for (int k = 7; k >= 0; --k) f = fma(f, x, row[mmax + k] * inv_fact[k]);
+    double e = 1.0 / 362880.0;  // e^x, |x| <= 0.05, 10 terms
+#pragma unroll
+    for (int k = 8; k >= 0; --k) e = fma(e, x, inv_fact[k]);
+    const double eT = row[kBoysMaxM + 1] * e;
+    F[mmax] = f;
+    for (int m = mmax; m > 0; --m) F[m - 1] = (2.0 * T * F[m] + eT) * (1.0 / double(2 * m - 1));
+  } else {
+    const double rs = rsqrt(T);
+    F[0] = 0.88622692545275801365 * rs;  // sqrt(pi)/2
+    const double h = 0.5 * rs * rs;
+    for (int m = 1; m <= mmax; ++m) F[m] = F[m - 1] * double(2 * m - 1) * h;
+  }
+}
+
 // Series `idx` of the n-point Rys rule at T: idx < n is squared root idx,
 // otherwise weight idx - n.  Finite branch: polynomial cell lookup; T at or
 // beyond 33 + 2.5(2n-1): scaled Gauss-Laguerre(-1/2) (math/rys.rs:174-185).

@@ -38,6 +38,9 @@ struct RysHostTables {
   double worst[kMaxRys + 1] = {};
 };
 const RysHostTables &rys_host_tables();
+// Boys table: kBoysRows grid points T0 = 0.1 i, columns F_0..F_kBoysMaxM, exp(-T0)
+constexpr int kBoysMaxM = 33, kBoysRows = 931, kBoysCols = kBoysMaxM + 2;
+const std::vector<double> &boys_host_table();
 void rys_host_eval(int n, double T, double *x, double *w);
 double shell_norm(double alpha, int l);
 std::vector<double> c2s_scaled(int l);  // (2l+1) x ncart(l), includes sqrt(4pi/(2l+1))
@@ -45,6 +48,7 @@ std::vector<double> c2s_scaled(int l);  // (2l+1) x ncart(l), includes sqrt(4pi/
 // ---------------------------------------------------------------------------
 // device-visible views (passed by value as kernel arguments)
 struct RysDev {
+  const double *boys;  // Boys table on the device (kBoysRows x kBoysCols)
   const double *buf;
   int nsub[kMaxRys + 1];
   int root_off[kMaxRys + 1], wt_off[kMaxRys + 1], lag_off[kMaxRys + 1];
@@ -129,7 +133,9 @@ hipError_t launch_schwarz(int lcls1, int lcls2, const BasisDev &b, const PairsDe
 // J/K kernel variants of a compiled class: one quartet per team of lanes
 // with LDS tables, one quartet per lane in registers (sliced blocks), or the
 // register path with per-row LDS accumulation of the bra-side updates
-enum JkVariant { kJkTeam = 0, kJkReg = 1, kJkRowAcc = 2, kJkNumVariants = 3 };
+// -- or, for total angular momentum <= 2, the row-sum kernel with an OS/HGP
+// block from the Boys function instead of Rys roots
+enum JkVariant { kJkTeam = 0, kJkReg = 1, kJkRowAcc = 2, kJkHgp = 3, kJkNumVariants = 4 };
 constexpr size_t kRowAccLdsMax = 64 * 1024;  // dynamic LDS of one row-accumulating workgroup
 int jk_default_variant(int la, int lb, int lc, int ld);
 // width of the row accumulators' compact column space for ket momenta lc >= ld
@@ -191,6 +197,12 @@ hipError_t launch_jk_finish(const BasisDev &b, const DensDev &d, double *d_J, do
                             hipStream_t s);
 // sum the ncopy per-XCD accumulator copies of d into the first one
 hipError_t launch_fold_copies(const DensDev &d, size_t nc2, hipStream_t s);
+// Boys values F_0..F_mmax (out: n x (mmax+1)) and n-point Rys roots/weights
+// (x, w: count x nr) at the points T, on the device
+hipError_t launch_boys_eval(const RysDev &r, int mmax, const double *T, size_t n, double *out,
+                            hipStream_t s);
+hipError_t launch_rys_eval(const RysDev &r, int nr, const double *T, size_t n, double *x,
+                           double *w, hipStream_t s);
 // histogram (16 bins) of the XCC_ID seen by `groups` workgroups
 hipError_t launch_xcc_probe(unsigned *hist, int groups, hipStream_t s);
 

@@ -875,6 +875,38 @@ hipError_t launch_fold_copies(const DensDev &d, size_t nc2, hipStream_t s) {
   return hipGetLastError();
 }
 
+__global__ void boys_eval_kernel(RysDev R, int mmax, const double *__restrict__ T, size_t n,
+                                 double *__restrict__ out) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    double F[kBoysMaxM + 1];
+    boys_eval(R.boys, mmax, T[t], F);
+    for (int m = 0; m <= mmax; ++m) out[t * (mmax + 1) + m] = F[m];
+  }
+}
+
+__global__ void rys_eval_kernel(RysDev R, int nr, const double *__restrict__ T, size_t n,
+                                double *__restrict__ x, double *__restrict__ w) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x)
+    for (int i = 0; i < nr; ++i) {
+      x[t * nr + i] = rys_series(R, nr, T[t], i);
+      w[t * nr + i] = rys_series(R, nr, T[t], nr + i);
+    }
+}
+
+hipError_t launch_boys_eval(const RysDev &r, int mmax, const double *T, size_t n, double *out,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(boys_eval_kernel, dim3(grid1d(n)), dim3(256), 0, s, r, mmax, T, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rys_eval(const RysDev &r, int nr, const double *T, size_t n, double *x,
+                           double *w, hipStream_t s) {
+  hipLaunchKernelGGL(rys_eval_kernel, dim3(grid1d(n)), dim3(256), 0, s, r, nr, T, n, x, w);
+  return hipGetLastError();
+}
+
 hipError_t launch_xcc_probe(unsigned *hist, int groups, hipStream_t s) {
   hipLaunchKernelGGL(xcc_probe_kernel, dim3(groups), dim3(64), 0, s, hist);
   return hipGetLastError();

@@ -9,6 +9,7 @@
 namespace ig {
 
 #include "ig_reg.h"
+#include "ig_hgp.h"
 #include "ig_rowacc.h"
 
 #ifdef IG_LANE_HOST_HELPERS
@@ -20,12 +21,14 @@ int rowacc_width(const BasisDev &b, int lc, int ld) {
 
 bool jk_variant_fits(int variant, int la, int lb, int lc, int ld, const BasisDev &b, int neff,
                      bool listed) {
-  if (variant != kJkRowAcc) return true;
+  if (variant != kJkRowAcc && variant != kJkHgp) return true;
   if (listed) return false;
+  if (variant == kJkHgp && la + lb + lc + ld > 2) return false;
   const int nr = (la + lb + lc + ld) / 2 + 1;
-  const size_t lds = size_t(2) * rys_host_tables().nsub[nr] * nr * kRysTerms * sizeof(double) +
-                     rowacc_doubles(ncart(la), ncart(lb), neff, rowacc_width(b, lc, ld)) *
-                         sizeof(double);
+  const size_t rys =
+      variant == kJkHgp ? 0 : size_t(2) * rys_host_tables().nsub[nr] * nr * kRysTerms * sizeof(double);
+  const size_t lds =
+      rys + rowacc_doubles(ncart(la), ncart(lb), neff, rowacc_width(b, lc, ld)) * sizeof(double);
   return lds <= kRowAccLdsMax;
 }
 #endif
@@ -39,6 +42,18 @@ hipError_t jk_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r, co
   using C = Cls<LA, LB, LC, LD>;
   if (qs.total <= 0) return hipSuccess;
   const size_t lds = size_t(2) * r.nsub[C::NR] * C::NR * kRysTerms * sizeof(double);
+  if (variant == kJkHgp) {
+    if constexpr (HgpClass<C>::ok) {
+      const size_t acc = rowacc_doubles(C::na, C::nb, d.neff, rowacc_width(b, LC, LD));
+      if (qs.list || acc * sizeof(double) > kRowAccLdsMax) return hipErrorInvalidValue;
+      long long blocks = (qs.total + 64 * kRowWaves - 1) / (64 * kRowWaves);
+      if (blocks > 1024) blocks = 1024;
+      return launch_jk_rowacc_slices<C, 0, F, 1>(b, p, r, qs, d, omega, ctr, s, int(blocks),
+                                                 acc * sizeof(double));
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   if (variant == kJkRowAcc) {
     const size_t acc = rowacc_doubles(C::na, C::nb, d.neff, rowacc_width(b, LC, LD));
     if (qs.list || lds + acc * sizeof(double) > kRowAccLdsMax) return hipErrorInvalidValue;

@@ -178,7 +178,7 @@ extern "C" size_t ig_basis_nshell(const ig_basis *b) { return b ? b->shells.size
 namespace {
 
 struct RysDevice {
-  DBuf buf;
+  DBuf buf, boys;
   RysDev view{};
 };
 
@@ -192,6 +192,8 @@ RysDevice &rys_device() {
     const RysHostTables &t = rys_host_tables();
     x->buf.upload(t.data);
     x->view.buf = x->buf.as<double>();
+    x->boys.upload(boys_host_table());
+    x->view.boys = x->boys.as<double>();
     for (int n = 0; n <= kMaxRys; ++n) {
       x->view.nsub[n] = t.nsub[n];
       x->view.root_off[n] = t.root_off[n];
@@ -201,6 +203,45 @@ RysDevice &rys_device() {
     r = x;
   }
   return *r;
+}
+
+extern "C" int ig_boys_eval(int m_max, const double *T, size_t n, double *out) {
+  if ((n && (!T || !out)) || m_max < 0 || m_max > kBoysMaxM - 8)
+    return fail(IG_ERR_INVALID_ARGUMENT, "m_max must be in 0..25 and buffers non-null");
+  return guarded([&] {
+    if (!n) return int(IG_OK);
+    DBuf dT, dout;
+    dT.reserve(n * sizeof(double));
+    dout.reserve(n * (m_max + 1) * sizeof(double));
+    check(hipMemcpy(dT.p, T, n * sizeof(double), hipMemcpyHostToDevice), "copy T");
+    check(launch_boys_eval(rys_device().view, m_max, dT.as<double>(), n, dout.as<double>(),
+                           nullptr),
+          "boys launch");
+    check(hipMemcpy(out, dout.p, n * (m_max + 1) * sizeof(double), hipMemcpyDeviceToHost),
+          "copy F");
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_rys_eval(int nroots, const double *T, size_t n, double *roots,
+                           double *weights) {
+  if ((n && (!T || !roots || !weights)) || nroots < 1 || nroots > kMaxRys)
+    return fail(IG_ERR_INVALID_ARGUMENT, "nroots must be in 1..13 and buffers non-null");
+  return guarded([&] {
+    if (!n) return int(IG_OK);
+    DBuf dT, dx, dw;
+    dT.reserve(n * sizeof(double));
+    dx.reserve(n * nroots * sizeof(double));
+    dw.reserve(n * nroots * sizeof(double));
+    check(hipMemcpy(dT.p, T, n * sizeof(double), hipMemcpyHostToDevice), "copy T");
+    check(launch_rys_eval(rys_device().view, nroots, dT.as<double>(), n, dx.as<double>(),
+                          dw.as<double>(), nullptr),
+          "rys launch");
+    check(hipMemcpy(roots, dx.p, n * nroots * sizeof(double), hipMemcpyDeviceToHost), "copy x");
+    check(hipMemcpy(weights, dw.p, n * nroots * sizeof(double), hipMemcpyDeviceToHost),
+          "copy w");
+    return int(IG_OK);
+  });
 }
 
 // Number of per-XCD J'/K' accumulator copies: one per XCC_ID that a probe

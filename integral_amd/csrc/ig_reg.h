@@ -338,11 +338,10 @@ __global__ void __launch_bounds__(64) jk_reg_kernel(BasisDev b, PairsDev p, RysD
   }
 }
 
-// number of register slices for a class and the launch of all of them (a
-// float block holds twice the elements of a double one in the same VGPRs)
+// number of register slices for a class and the launch of all of them
 template <class C, typename F>
 struct RegSlices {
-  static constexpr int CAP = kRegBlockMax * int(sizeof(double) / sizeof(F));
+  static constexpr int CAP = kRegBlockMax;  // float blocks too: larger ones spill
   static constexpr int NS = (C::N4 + CAP - 1) / CAP;
   static constexpr int E = (C::N4 + NS - 1) / NS;
 };

@@ -221,15 +221,17 @@ __device__ __forceinline__ void digest_row(const BasisDev &b, const DensDev &dd,
 #undef IG_IN
 }
 
-template <class C, int LO, int HI, typename F>
+// ENG 0: Rys quartet block (quartet_block_reg); ENG 1: the OS/HGP block of
+// ig_hgp.h (total angular momentum <= 2, Coulomb operator only).
+template <class C, int LO, int HI, typename F, int ENG = 0>
 __global__ void __launch_bounds__(64 * kRowWaves)
     jk_rowacc_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs, DensDev dd, double omega,
                      unsigned long long *counters) {
   extern __shared__ double lds[];
   __shared__ int s_row[2];  // by step parity: a slow wavefront still reading
                             // step i's value cannot see step i+1's
-  stage_rys<C::NR>(R, lds);
-  const int nrys = 2 * R.nsub[C::NR] * C::NR * kRysTerms;
+  if constexpr (ENG == 0) stage_rys<C::NR>(R, lds);
+  const int nrys = ENG == 0 ? 2 * R.nsub[C::NR] * C::NR * kRysTerms : 0;
   const RowCols<C> rc(b);
   double *JB = lds + nrys;
   const int nacc = int(rowacc_doubles(C::na, C::nb, dd.neff, rc.W));
@@ -279,7 +281,10 @@ __global__ void __launch_bounds__(64 * kRowWaves)
 #pragma unroll
     for (int e = 0; e < HI - LO; ++e) acc[e] = F(0);
     if (on) {
-      quartet_block_reg<C, LO, HI, F>(p, R, lds, pb, pk, g, omega, acc);
+      if constexpr (ENG == 0)
+        quartet_block_reg<C, LO, HI, F>(p, R, lds, pb, pk, g, omega, acc);
+      else
+        quartet_block_hgp<C, LO, HI, F>(p, R, pb, pk, g, acc);
       if (LO == 0) {
         ++my_quartets;
         my_prims += (unsigned long long)p.pp_n[pb] * p.pp_n[pk];
@@ -311,7 +316,7 @@ __global__ void __launch_bounds__(64 * kRowWaves)
   }
 }
 
-template <class C, int S, typename F>
+template <class C, int S, typename F, int ENG = 0>
 hipError_t launch_jk_rowacc_slices(const BasisDev &b, const PairsDev &p, const RysDev &r,
                                    const QuartetSet &qs, const DensDev &d, double omega,
                                    unsigned long long *ctr, hipStream_t s, int blocks,
@@ -319,11 +324,11 @@ hipError_t launch_jk_rowacc_slices(const BasisDev &b, const PairsDev &p, const R
   constexpr int NS = RegSlices<C, F>::NS, E = RegSlices<C, F>::E;
   if constexpr (S < NS) {
     constexpr int LO = S * E, HI = (S + 1) * E < C::N4 ? (S + 1) * E : C::N4;
-    hipLaunchKernelGGL((jk_rowacc_kernel<C, LO, HI, F>), dim3(blocks), dim3(64 * kRowWaves), lds,
-                       s, b, p, r, qs, d, omega, ctr);
+    hipLaunchKernelGGL((jk_rowacc_kernel<C, LO, HI, F, ENG>), dim3(blocks), dim3(64 * kRowWaves),
+                       lds, s, b, p, r, qs, d, omega, ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_jk_rowacc_slices<C, S + 1, F>(b, p, r, qs, d, omega, ctr, s, blocks, lds);
+    return launch_jk_rowacc_slices<C, S + 1, F, ENG>(b, p, r, qs, d, omega, ctr, s, blocks, lds);
   } else {
     return hipSuccess;
   }

@@ -276,6 +276,37 @@ const RysHostTables &rys_host_tables() {
   return t;
 }
 
+// Boys function table F_m(T0), T0 = 0, 0.1, ..., 93.0 (rows), m = 0..33, and
+// exp(-T0) as the last column (math/boys.rs:63-136 layout and grid).  Each row
+// comes from the series F_m(T) = e^-T sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1))
+// at m = 33 in long double and the downward recurrence
+// F_{m-1} = (2T F_m + e^-T) / (2m-1), which is stable.
+const std::vector<double> &boys_host_table() {
+  static std::vector<double> t;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    t.assign(size_t(kBoysRows) * kBoysCols, 0.0);
+    for (int i = 0; i < kBoysRows; ++i) {
+      const long double T = 0.1L * i, eT = std::exp(-T);
+      long double term = 1.0L / (2 * kBoysMaxM + 1), sum = term;
+      for (int k = 1; k < 2000; ++k) {
+        term *= 2.0L * T / (2 * kBoysMaxM + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-22L) break;
+      }
+      long double f = eT * sum;
+      double *row = t.data() + size_t(i) * kBoysCols;
+      row[kBoysMaxM] = double(f);
+      for (int m = kBoysMaxM; m > 0; --m) {
+        f = (2.0L * T * f + eT) / (2 * m - 1);
+        row[m - 1] = double(f);
+      }
+      row[kBoysMaxM + 1] = double(eT);
+    }
+  });
+  return t;
+}
+
 void rys_host_eval(int n, double T, double *x, double *w) {
   const RysHostTables &t = rys_host_tables();
   double hi = seam(n);

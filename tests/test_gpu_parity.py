@@ -164,11 +164,12 @@ def test_grad_bra_pairs_sum_in_any_order():
     assert np.max(np.abs(parts[0] + parts[1] - serial)) < 1e-12
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_direct_jk_every_kernel_variant(variant):
-    """Every J/K kernel of every compiled class (team/LDS, lane/register and
-    register with per-row LDS sums) against the oracle, forced through a
-    loaded schedule."""
+    """Every J/K kernel of every compiled class (team/LDS, lane/register,
+    register with per-row LDS sums, and the OS/HGP block for total l <= 2)
+    against the oracle, forced through a loaded schedule (classes a variant
+    cannot run take the register kernel)."""
     shells = (B.grad_basis(True, max_l=2) + B.fd_basis()
               + [(2, (1.3, -0.8, 0.9), [0.45], [1.0], True)])
     n = B.nfunc(shells)
@@ -178,6 +179,7 @@ def test_direct_jk_every_kernel_variant(variant):
     sched = dc.schedule()
     dc.set_schedule([dict(e, variant=variant) for e in sched])
     assert all(e["variant"] == variant and e["tuned"] for e in dc.schedule())
+    assert variant != 3 or any(sum(e["cls"]) <= 2 for e in sched)
     J, K = dc.jk(dens, 0.0)
     oJ, oK = O.Basis(shells).direct_jk(dens, 0.0)
     for x in range(2):
@@ -194,7 +196,7 @@ def test_autotune_keeps_results():
     dc = gpu_basis(shells).direct_contractor()
     D = torch.tensor(p[None], dtype=torch.float64, device="cuda:0").contiguous()
     sched = dc.autotune(D, threshold=1e-12, reps=1)
-    assert sched and all(e["tuned"] and e["variant"] in (0, 1, 2) for e in sched)
+    assert sched and all(e["tuned"] and e["variant"] in (0, 1, 2, 3) for e in sched)
     J, K = dc.jk([p], 1e-12)
     oJ, oK = O.Basis(shells).direct_jk([p], 1e-12)
     assert np.max(np.abs(J[0] - oJ[0])) < TOL and np.max(np.abs(K[0] - oK[0])) < TOL
@@ -219,7 +221,7 @@ def _support_density(shells, keep, seed):
     return p * np.outer(rows, rows)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_direct_jk_density_screened_lanes(variant):
     """Quartets dropped by the density mask next to kept ones in the same
     wavefront, across bra rows (regression: the wavefront-reduced bra J sum
@@ -238,7 +240,7 @@ def test_direct_jk_density_screened_lanes(variant):
     assert np.max(np.abs(K[0] - oK[0])) < TOL
 
 
-@pytest.mark.parametrize("variant", [None, 1, 2])
+@pytest.mark.parametrize("variant", [None, 1, 2, 3])
 def test_direct_jk_twelve_atom_molecule(variant):
     """The benchmark's molecule recipe at 12 atoms (spherical s,s,p,p,d,
     3/3/3/3/1 primitives, nao = 156), two densities, tau = 1e-12: every
