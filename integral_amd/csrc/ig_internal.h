@@ -158,9 +158,11 @@ hipError_t launch_jk_lane_f32(int la, int lb, int lc, int ld, const BasisDev &b,
                               unsigned long long *counters, hipStream_t s);
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                       const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s);
-hipError_t launch_grad(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
-                       const RysDev &r, const QuartetSet &qs, const GradDev &g,
-                       unsigned long long *counters, hipStream_t s);
+// lane-per-quartet adjoint gradient (ig_grad_lane.hip); natom sizes its LDS
+// copy of the gradient
+hipError_t launch_grad_lane(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                            const RysDev &r, const QuartetSet &qs, const GradDev &g, int natom,
+                            unsigned long long *counters, hipStream_t s);
 
 // runtime-l launchers (ig_big.hip) for classes class_supported() rejects;
 // `scratch` holds big_scratch_elems(...) doubles (one block slot per

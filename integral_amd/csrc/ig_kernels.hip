@@ -306,257 +306,7 @@ __global__ void __launch_bounds__(64) eri_kernel(BasisDev b, PairsDev p, RysDev 
   }
 }
 
-// ---------------------------------------------------------------------------
-// ERI-derivative contraction (adjoint form).  For each canonical quartet the
-// Cartesian weight block W (the permutation-averaged two-particle density of
-// direct_grad.rs:490-547, built from Cartesian-transformed density matrices)
-// is contracted on the fly with the centre derivatives
-//   d/dA_x (ab|cd) = 2 alpha (a+1x b|cd) - a_x (a-1x b|cd)   (B, C alike)
-// evaluated from 2D Rys tables raised by one on A, B and C; D follows from
-// translational invariance (direct_grad.rs:425-484).  Nothing of size nao^4
-// is formed and the tables are recomputed per primitive quartet.
-template <class C>
-struct GradCls {
-  using R = Cls<C::LA + 1, C::LB + 1, C::LC + 1, C::LD>;  // raised table extents
-  static constexpr int NR = (C::LA + C::LB + C::LC + C::LD + 1) / 2 + 1;
-  static constexpr int T = R::T;
-  static constexpr int SA = (C::LB + 2) * (C::LC + 2) * (C::LD + 1);
-  static constexpr int SB = (C::LC + 2) * (C::LD + 1);
-  static constexpr int SC = C::LD + 1;
-  static constexpr int WORK = 2 * NR + 3 * NR * T;
-};
 
-// element e -> packed raised-table offsets and packed powers (2 bits each:
-// ax ay az bx by bz cx cy cz)
-template <class C>
-__device__ __forceinline__ void grad_elem(int e, int &offs, int &pows) {
-  using GC = GradCls<C>;
-  int id = e % C::nd;
-  int r = e / C::nd;
-  int ic = r % C::nc;
-  r /= C::nc;
-  int ib = r % C::nb;
-  int ia = r / C::nb;
-  int p[4][3];
-  cart_powers(C::LA, ia, p[0][0], p[0][1], p[0][2]);
-  cart_powers(C::LB, ib, p[1][0], p[1][1], p[1][2]);
-  cart_powers(C::LC, ic, p[2][0], p[2][1], p[2][2]);
-  cart_powers(C::LD, id, p[3][0], p[3][1], p[3][2]);
-  offs = 0;
-  pows = 0;
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    int o = p[0][ax] * GC::SA + p[1][ax] * GC::SB + p[2][ax] * GC::SC + p[3][ax];
-    offs |= o << (10 * ax);
-    pows |= p[0][ax] << (2 * ax);
-    pows |= p[1][ax] << (6 + 2 * ax);
-    pows |= p[2][ax] << (12 + 2 * ax);
-  }
-}
-
-template <class C>
-__global__ void __launch_bounds__(64) grad_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs,
-                                                  GradDev gd, unsigned long long *counters) {
-  using GC = GradCls<C>;
-  constexpr int G = C::G, NR = GC::NR, T = GC::T, PER = cmax(GC::WORK, 9 * G);
-  __shared__ double lds[C::TEAMS * PER];
-  const int lane = threadIdx.x, team = lane / G, tl = lane % G;
-  double *work = lds + team * PER;
-  int offs[C::E], pows[C::E];
-#pragma unroll
-  for (int k = 0; k < C::E; ++k) {
-    const int e = tl + G * k;
-    offs[k] = 0;
-    pows[k] = 0;
-    if (e < C::N4) grad_elem<C>(e, offs[k], pows[k]);
-  }
-  const int nsh = b.nsh;
-  const size_t nc2 = (size_t)b.nao_cart * b.nao_cart;
-  const int ncrt = b.nao_cart;
-  const int nterm = gd.ncoul + gd.nexch;
-  unsigned long long my_q = 0, my_p = 0;
-  for (long long t = (long long)blockIdx.x * C::TEAMS + team; t < qs.total;
-       t += (long long)gridDim.x * C::TEAMS) {
-    int P, Q;
-    decode_quartet(qs, t, P, Q);
-    const int pb = qs.x_off + P, pk = qs.y_off + Q;
-    QuartetGeom g;
-    load_geom(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk], g);
-    const int atA = b.atom[g.sA], atB = b.atom[g.sB], atC = b.atom[g.sC], atD = b.atom[g.sD];
-    if (atA == atB && atB == atC && atC == atD) continue;  // direct_grad.rs:356-358
-    const double deg = (g.sA == g.sB ? 1.0 : 2.0) * (g.sC == g.sD ? 1.0 : 2.0) * (pb == pk ? 1.0 : 2.0);
-    // density block bound (direct_grad.rs:626-644), symmetrised maxima
-    double bound = 0.0;
-    {
-      const int i = g.sA, j = g.sB, k = g.sC, l = g.sD;
-      for (int tm = 0; tm < nterm; ++tm) {
-        const double *Lm = gd.bmax + (size_t)(2 * tm) * nsh * nsh;
-        const double *Rm = Lm + (size_t)nsh * nsh;
-        if (tm < gd.ncoul)
-          bound += 0.5 * fabs(gd.fac[tm]) *
-                   (Lm[i * nsh + j] * Rm[k * nsh + l] + Lm[k * nsh + l] * Rm[i * nsh + j]);
-        else
-          bound += 0.25 * fabs(gd.fac[tm]) *
-                   (Lm[i * nsh + k] * Rm[j * nsh + l] + Lm[j * nsh + k] * Rm[i * nsh + l] +
-                    Lm[i * nsh + l] * Rm[j * nsh + k] + Lm[j * nsh + l] * Rm[i * nsh + k]);
-      }
-    }
-    if (p.q[pb] * p.q[pk] * deg * bound < gd.tau) continue;
-    // weight block for this lane's elements (Cartesian)
-    const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
-              d0 = b.off_cart[g.sD];
-    double wv[C::E];
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-      const int e = tl + G * k;
-      double w = 0.0;
-      if (e < C::N4) {
-        const int id = e % C::nd, r1 = e / C::nd, ic = r1 % C::nc, r2 = r1 / C::nc,
-                  ib = r2 % C::nb, ia = r2 / C::nb;
-        const size_t A = a0 + ia, B = b0 + ib, Cc = c0 + ic, D = d0 + id;
-        for (int tm = 0; tm < nterm; ++tm) {
-          const double *Lc = gd.mats + (size_t)(2 * tm) * nc2, *Rc = Lc + nc2;
-          if (tm < gd.ncoul)
-            w += 0.5 * deg * gd.fac[tm] *
-                 (Lc[A * ncrt + B] * Rc[Cc * ncrt + D] + Rc[A * ncrt + B] * Lc[Cc * ncrt + D]);
-          else
-            w += 0.25 * deg * gd.fac[tm] *
-                 (Lc[A * ncrt + Cc] * Rc[B * ncrt + D] + Lc[B * ncrt + Cc] * Rc[A * ncrt + D] +
-                  Lc[A * ncrt + D] * Rc[B * ncrt + Cc] + Lc[B * ncrt + D] * Rc[A * ncrt + Cc]);
-        }
-      }
-      wv[k] = w;
-    }
-    // 9 accumulators: A(x,y,z), B(x,y,z), C(x,y,z)
-    double ga[9];
-#pragma unroll
-    for (int x = 0; x < 9; ++x) ga[x] = 0.0;
-    double AB[3], CD[3];
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      AB[x] = g.A[x] - g.B[x];
-      CD[x] = g.C[x] - g.D[x];
-    }
-    double *rw = work, *tab = work + 2 * NR;
-    const int ob = p.pp_off[pb], nbp = p.pp_n[pb], ok = p.pp_off[pk], nkp = p.pp_n[pk];
-    for (int ib = 0; ib < nbp; ++ib) {
-      const double pz = p.zeta[ob + ib], al = p.alpha[ob + ib], be = p.beta[ob + ib];
-      const double Pv[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
-      const double kzp = kTwoPi52 * p.kz[ob + ib], i2p = p.i2z[ob + ib];
-      for (int ik = 0; ik < nkp; ++ik) {
-        const double qz = p.zeta[ok + ik], ga_ = p.alpha[ok + ik];
-        const double Qv[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
-        const double kzq = p.kz[ok + ik], i2q = p.i2z[ok + ik];
-        const double pq = pz + qz, rs = rsqrt(pq), ipq = rs * rs, rho = pz * qz * ipq;
-        double PQ[3], r2 = 0.0;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          PQ[x] = Pv[x] - Qv[x];
-          r2 += PQ[x] * PQ[x];
-        }
-        double Tq = rho * r2;
-        double pref = kzp * kzq * rs;
-        double s_att = 1.0;
-        if (gd.omega > 0.0) {
-          s_att = gd.omega * gd.omega / (rho + gd.omega * gd.omega);
-          Tq *= s_att;
-          pref *= sqrt(s_att);
-        }
-        for (int idx = tl; idx < 2 * NR; idx += G) {
-          double v = rys_series(R, NR, Tq, idx);
-          rw[idx] = (idx < NR) ? v * s_att : v;
-        }
-        team_sync();
-        for (int task = tl; task < 3 * NR; task += G) {
-          const int r = task / 3, ax = task - 3 * r;
-          auto pick = [ax](const double *v) { return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]); };
-          const double x = rw[r];
-          const double fq = x * qz * ipq, fp = x * pz * ipq;
-          const double b00 = 0.5 * x * ipq;
-          const double b10 = (1.0 - fq) * i2p;
-          const double b01 = (1.0 - fp) * i2q;
-          const double pqa = pick(PQ);
-          const double c00 = (pick(Pv) - pick(g.A)) - fq * pqa;
-          const double cp0 = (pick(Qv) - pick(g.C)) + fp * pqa;
-          rys_axis_table<C::LA + 1, C::LB + 1, C::LC + 1, C::LD>(c00, cp0, b10, b01, b00,
-                                                                 pick(AB), pick(CD), tab + task * T);
-        }
-        team_sync();
-        const double ta = 2.0 * al, tb = 2.0 * be, tc = 2.0 * ga_;
-#pragma unroll 1
-        for (int r = 0; r < NR; ++r) {
-          const double pw = pref * rw[NR + r];
-          const double *tt[3] = {tab + (3 * r) * T, tab + (3 * r + 1) * T, tab + (3 * r + 2) * T};
-#pragma unroll
-          for (int k = 0; k < C::E; ++k) {
-            if (tl + G * k < C::N4) {
-              const int o = offs[k], pw2 = pows[k];
-              double v0[3], dA[3], dB[3], dC[3];
-#pragma unroll
-              for (int ax = 0; ax < 3; ++ax) {
-                const double *tx = tt[ax];
-                const int oo = (o >> (10 * ax)) & 1023;
-                const int pa = (pw2 >> (2 * ax)) & 3, pb_ = (pw2 >> (6 + 2 * ax)) & 3,
-                          pc = (pw2 >> (12 + 2 * ax)) & 3;
-                v0[ax] = tx[oo];
-                // lowered reads stay in bounds: the offset only steps down when
-                // the power is non-zero (otherwise the term is multiplied by 0)
-                dA[ax] = ta * tx[oo + GC::SA] - double(pa) * tx[oo - (pa ? GC::SA : 0)];
-                dB[ax] = tb * tx[oo + GC::SB] - double(pb_) * tx[oo - (pb_ ? GC::SB : 0)];
-                dC[ax] = tc * tx[oo + GC::SC] - double(pc) * tx[oo - (pc ? GC::SC : 0)];
-              }
-              const double s = pw * wv[k];
-              const double yz = v0[1] * v0[2], xz = v0[0] * v0[2], xy = v0[0] * v0[1];
-              ga[0] += s * dA[0] * yz;
-              ga[1] += s * dA[1] * xz;
-              ga[2] += s * dA[2] * xy;
-              ga[3] += s * dB[0] * yz;
-              ga[4] += s * dB[1] * xz;
-              ga[5] += s * dB[2] * xy;
-              ga[6] += s * dC[0] * yz;
-              ga[7] += s * dC[1] * xz;
-              ga[8] += s * dC[2] * xy;
-            }
-          }
-        }
-        team_sync();
-      }
-    }
-    if (tl == 0) {
-      ++my_q;
-      my_p += (unsigned long long)nbp * nkp;
-    }
-    // team reduction of the 9 sums, then per-atom accumulation
-    double *red = work;
-#pragma unroll
-    for (int x = 0; x < 9; ++x) red[x * G + tl] = ga[x];
-    team_sync();
-    for (int comp = tl; comp < 9; comp += G) {
-      double s = 0.0;
-      for (int i = 0; i < G; ++i) s += red[comp * G + i];
-      const int which = comp / 3, ax = comp % 3;
-      const int at = which == 0 ? atA : (which == 1 ? atB : atC);
-      atomicAdd(gd.out + 3 * at + ax, s);
-      atomicAdd(gd.out + 3 * atD + ax, -s);
-    }
-    team_sync();
-  }
-  if (counters && tl == 0) {
-    atomicAdd(counters, my_q);
-    atomicAdd(counters + 1, my_p);
-  }
-}
-
-template <int LA, int LB, int LC, int LD>
-hipError_t grad_one(const BasisDev &b, const PairsDev &p, const RysDev &r, const QuartetSet &qs,
-                    const GradDev &g, unsigned long long *ctr, hipStream_t s) {
-  using C = Cls<LA, LB, LC, LD>;
-  if (qs.total <= 0) return hipSuccess;
-  long long blocks = (qs.total + C::TEAMS - 1) / C::TEAMS;
-  if (blocks > 256 * 64) blocks = 256 * 64;
-  hipLaunchKernelGGL(grad_kernel<C>, dim3(int(blocks)), dim3(64), 0, s, b, p, r, qs, g, ctr);
-  return hipGetLastError();
-}
 
 // ---------------------------------------------------------------------------
 // dispatch tables over (la, lb, lc, ld) with la >= lb, lc >= ld, l <= kMaxKernelL
@@ -611,10 +361,11 @@ bool class_supported(int la, int lb, int lc, int ld) {
 }
 
 int jk_default_variant(int la, int lb, int lc, int ld) {
-  // register slices while a quartet needs at most two of them (each slice
+  // the autotuner's usual choice on s,p,d bases: register kernels with row
+  // sums while a quartet needs at most two element slices (each slice
   // rebuilds the quartet's 2D tables), team kernels for larger blocks
   const int n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-  return n4 <= 2 * kRegBlockMax ? kJkReg : kJkTeam;
+  return n4 <= 2 * kRegBlockMax ? kJkRowAcc : kJkTeam;
 }
 
 hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
@@ -634,16 +385,6 @@ hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const P
                       const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s) {
 #define IG_CASE(A, B, Cc, D) \
   if (la == A && lb == B && lc == Cc && ld == D) return eri_one<A, B, Cc, D>(b, p, r, qs, out, s);
-  IG_FOR_CLASSES(IG_CASE)
-#undef IG_CASE
-  return hipErrorInvalidValue;
-}
-
-hipError_t launch_grad(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
-                       const RysDev &r, const QuartetSet &qs, const GradDev &g,
-                       unsigned long long *ctr, hipStream_t s) {
-#define IG_CASE(A, B, Cc, D) \
-  if (la == A && lb == B && lc == Cc && ld == D) return grad_one<A, B, Cc, D>(b, p, r, qs, g, ctr, s);
   IG_FOR_CLASSES(IG_CASE)
 #undef IG_CASE
   return hipErrorInvalidValue;

@@ -801,7 +801,8 @@ hipError_t grad_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const 
                        const GradDev &gd, unsigned long long *ctr, hipStream_t s) {
   RysDevice &R = rys_device();
   if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
-    return launch_grad(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd, ctr, s);
+    return launch_grad_lane(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd, P.natom,
+                            ctr, s);
   return launch_grad_big(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, gd,
                          P.s_big.as<double>(), ctr, s);
 }
