@@ -156,6 +156,187 @@ def bench_grad(args):
     print(json.dumps(out))
 
 
+def bench_highl(args):
+    """J/K with shells up to l = 6 (20 atoms, s,p,d,f,g per atom and an i
+    shell on every fourth, one primitive each, nao 565, one density,
+    tau 1e-12): FP64 and with FP32 quartets below a Schwarz product of 1e-7."""
+    import torch
+    import integral_amd as ig
+    from integral_amd import synthetic
+    shells, D = synthetic.high_l_config()
+    tau, cut = 1e-12, 1e-7
+    dc = synthetic.to_basis(shells).direct_contractor()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    Dd = torch.tensor(D[None], dtype=torch.float64, device=dev).contiguous()
+    J, K = torch.zeros_like(Dd), torch.zeros_like(Dd)
+    t0 = time.perf_counter()
+    dc.autotune(Dd, threshold=tau, reps=1, stream=stream)
+    tune_s = time.perf_counter() - t0
+
+    def run(steps):
+        for _ in range(steps):
+            J.zero_()
+            K.zero_()
+            dc.jk_device(Dd, J, K, threshold=tau, stream=stream)
+        torch.cuda.synchronize(dev)
+
+    run(args.warmup)
+    t0 = time.perf_counter()
+    run(args.steps)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    quartets = dc.last_stats()[0]
+    J64, K64 = J.clone(), K.clone()
+    dc.set_profiling(True)
+    run(1)
+    recs = {}
+    for r in dc.profile():
+        key = "".join("spdfghi"[l] for l in r["cls"])
+        recs[key] = recs.get(key, 0.0) + r["ms"]
+    dc.set_profiling(False)
+    dc.set_mixed_precision(cut)
+    run(1)
+    t0 = time.perf_counter()
+    run(args.steps)
+    mms = (time.perf_counter() - t0) * 1e3 / args.steps
+    dJ = float((J - J64).abs().max().item())
+    dK = float((K - K64).abs().max().item())
+    dc.set_mixed_precision(0.0)
+    log("high-l: %.1f ms/build FP64 (%d quartets), %.1f ms mixed (max |dJ| %.1e, |dK| %.1e)"
+        % (ms, quartets, mms, dJ, dK))
+    top = sorted(recs.items(), key=lambda kv: -kv[1])[:12]
+    out = {"metric": METRIC, "value": quartets / (ms * 1e-3), "unit": "quartets/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded molecule and density, no checkpoint)",
+           "config": {"workload": "integral-direct J/K, 20 atoms in a 10 bohr cube, spherical "
+                                  "s,p,d,f,g + an l=6 shell on every 4th atom, 1 primitive each, "
+                                  "nao=565, 1 density, tau=1e-12", "quartets": quartets,
+                      "autotune_s": tune_s, "parallelism": "1 GPU"},
+           "mixed": {"fp32_below": cut, "ms_per_step": mms, "speedup_vs_fp64": ms / mms,
+                     "max_abs_dJ": dJ, "max_abs_dK": dK},
+           "classes_ms": dict(top)}
+    print(json.dumps(out))
+
+
+def bench_batch(args):
+    """Batched J/K: nmol molecules of 5-60 atoms (1 atom per 20 bohr^3,
+    spherical s,s,p,p,d,f per atom, mean nao ~650), one density each,
+    tau 1e-12, all molecules' J and K in one captured HIP graph (per quartet
+    class one persistent launch over a device work queue).  Times the
+    batched graph in FP64 and with FP32 quartets below --mixed-cut, and the
+    per-molecule FP64 loop over the same plans' device path."""
+    import torch
+    import integral_amd as ig
+    from integral_amd import synthetic
+    tau = 1e-12
+    mols = synthetic.batch_config(64)[:args.nmol]
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    t0 = time.perf_counter()
+    bases = [synthetic.to_basis(s) for s, _ in mols]
+    bc = ig.BatchContractor(bases)
+    plan_s = time.perf_counter() - t0
+    naos = [b.nao() for b in bases]
+    log("batch of %d molecules (nao %d..%d, mean %.0f), plans %.2f s"
+        % (len(mols), min(naos), max(naos), sum(naos) / len(naos), plan_s))
+    dens = [torch.tensor(D, dtype=torch.float64, device=dev).contiguous() for _, D in mols]
+    J = [torch.zeros_like(d) for d in dens]
+    K = [torch.zeros_like(d) for d in dens]
+
+    def run(steps):
+        for _ in range(steps):
+            for t in J + K:
+                t.zero_()
+            bc.jk_device(dens, J, K, threshold=tau, stream=stream)
+        torch.cuda.synchronize(dev)
+
+    t0 = time.perf_counter()
+    run(1)  # captures the graph
+    capture_s = time.perf_counter() - t0
+    run(max(0, args.warmup - 1))
+    t0 = time.perf_counter()
+    run(args.steps)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    quartets, prims = bc.last_stats()
+    log("batched FP64: %.1f ms/build, %d quartets (first call with capture %.2f s)"
+        % (ms, quartets, capture_s))
+    J64 = [j.clone() for j in J]
+    K64 = [k.clone() for k in K]
+    # the per-molecule device path, one contractor per molecule
+    dcs = [b.direct_contractor() for b in bases]
+    dens3 = [d.reshape(1, *d.shape) for d in dens]
+    Jm = [torch.zeros_like(d) for d in dens3]
+    Km = [torch.zeros_like(d) for d in dens3]
+
+    def loop():
+        for m, dc in enumerate(dcs):
+            Jm[m].zero_()
+            Km[m].zero_()
+            dc.jk_device(dens3[m], Jm[m], Km[m], threshold=tau, stream=stream)
+        torch.cuda.synchronize(dev)
+
+    loop()
+    t0 = time.perf_counter()
+    loop()
+    loop_ms = (time.perf_counter() - t0) * 1e3
+    dloop = max(float((Jm[m][0] - J64[m]).abs().max().item()) for m in range(len(mols)))
+    log("per-molecule FP64 loop: %.1f ms/build (max |dJ| vs batched %.1e)" % (loop_ms, dloop))
+    mixed = None
+    if args.mixed_cut > 0:
+        bc.set_mixed_precision(args.mixed_cut)
+        run(1)
+        t0 = time.perf_counter()
+        run(args.steps)
+        mms = (time.perf_counter() - t0) * 1e3 / args.steps
+        dJ = max(float((J[m] - J64[m]).abs().max().item()) for m in range(len(mols)))
+        dK = max(float((K[m] - K64[m]).abs().max().item()) for m in range(len(mols)))
+        bc.set_mixed_precision(0.0)
+        mixed = {"fp32_below": args.mixed_cut, "ms_per_step": mms, "speedup_vs_fp64": ms / mms,
+                 "max_abs_dJ": dJ, "max_abs_dK": dK}
+        log("batched mixed (FP32 below %.0e): %.1f ms/build, max |dJ| %.1e, |dK| %.1e"
+            % (args.mixed_cut, mms, dJ, dK))
+    out = {"metric": METRIC, "value": quartets / (ms * 1e-3), "unit": "quartets/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded molecules and densities, no checkpoint)",
+           "config": {"workload": "batched J/K, %d molecules of 5-60 atoms (1 atom per 20 "
+                                  "bohr^3), spherical s,s,p,p,d,f, nao %d..%d, 1 density each, "
+                                  "tau=1e-12, one HIP graph" % (len(mols), min(naos), max(naos)),
+                      "nmol": len(mols), "nao_total": sum(naos), "quartets": quartets,
+                      "prim_quartets": prims, "plan_build_s": plan_s,
+                      "first_call_with_capture_s": capture_s,
+                      "per_molecule_loop_ms": loop_ms, "parallelism": "1 GPU"},
+           "mixed": mixed}
+    print(json.dumps(out))
+
+
+def bench_eri(args):
+    """The dense ERI tensor of the 10-atom Cartesian s,s,p molecule (nao 50,
+    6.25e6 doubles to host memory), FP64."""
+    from integral_amd import synthetic
+    shells, _ = synthetic.dense_eri_config()
+    basis = synthetic.to_basis(shells)
+    for _ in range(args.warmup):
+        basis.eri()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        basis.eri()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    nsh = len(shells)
+    npair = nsh * (nsh + 1) // 2
+    quartets = npair * (npair + 1) // 2
+    log("dense eri: %.2f ms per tensor (plan, kernels, 50 MB to host)" % ms)
+    print(json.dumps({"metric": METRIC, "value": quartets / (ms * 1e-3), "unit": "quartets/s",
+                      "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f64",
+                      "data": "synthetic (seeded molecule, no checkpoint)",
+                      "config": {"workload": "dense ERI tensor, 10 atoms, Cartesian s,s,p "
+                                             "(3 primitives), nao=50, host output",
+                                 "quartets": quartets, "parallelism": "1 GPU"}}))
+
+
 def cpu_baseline_grad(shells, D, cj, ck, tau, seconds):
     """Oracle adjoint gradient on the host cores over a contiguous bra-pair
     sample, reported as gradient quartets per second."""
@@ -190,9 +371,13 @@ def main():
     ap.add_argument("--mixed-cut", type=float, default=1e-6,
                     help="also time the build with FP32 quartets below this Schwarz product "
                          "(0: skip)")
-    ap.add_argument("--workload", choices=["jk", "grad"], default="jk",
+    ap.add_argument("--nmol", type=int, default=64,
+                    help="batch workload: the first NMOL of the 64 seeded molecules")
+    ap.add_argument("--workload", choices=["jk", "grad", "highl", "eri", "batch"], default="jk",
                     help="jk: the direct J/K build (default); grad: seconds per geometry of "
-                         "the J/K energy and its adjoint forces")
+                         "the J/K energy and its adjoint forces; highl: J/K with shells up to "
+                         "l = 6; eri: the dense ERI tensor; batch: many molecules' J/K in "
+                         "one HIP graph")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once and exit: the reference "
                          "kernel of known traffic that scales the profiler's FETCH_SIZE and "
@@ -216,6 +401,12 @@ def main():
         return
     if args.workload == "grad":
         return bench_grad(args)
+    if args.workload == "highl":
+        return bench_highl(args)
+    if args.workload == "eri":
+        return bench_eri(args)
+    if args.workload == "batch":
+        return bench_batch(args)
     import integral_amd as ig
     from integral_amd import synthetic
 

@@ -209,6 +209,37 @@ int ig_direct_schedule_read(const ig_direct *direct, ig_schedule_entry *out, siz
 /* Load a schedule (e.g. saved from an earlier autotune of the same basis). */
 int ig_direct_schedule_write(ig_direct *direct, const ig_schedule_entry *in, size_t n);
 
+/* ---- Batched J/K over many molecules -------------------------------------
+ * No counterpart in the reference, whose DirectJkContractor serves one basis
+ * per object (direct.rs:75-131); a caller with many small molecules (a
+ * screening or dataset workload) would loop over contractors.  Here one
+ * object holds a plan per basis and runs every molecule's J/K in one HIP
+ * graph: per quartet class one persistent launch whose workgroups take
+ * chunks of the concatenated quartet lists of all molecules from a device
+ * work queue.  The graph is captured on the first call and replayed while
+ * threshold, exchange flag, precision policy and the device pointers stay the
+ * same; any change re-captures it. */
+typedef struct ig_batch ig_batch;
+/* One plan per basis (as ig_direct_create with IG_ENGINE_AUTO). */
+int ig_batch_create(const ig_basis *const *bases, size_t n, ig_batch **out);
+void ig_batch_destroy(ig_batch *batch);
+size_t ig_batch_size(const ig_batch *batch);
+/* As ig_direct_set_mixed_precision, for every molecule of the batch. */
+int ig_batch_set_mixed_precision(ig_batch *batch, double fp32_below);
+/* J (and K when want_exchange) of one density per molecule: d_dens[m],
+ * d_J[m], d_K[m] are DEVICE pointers to nao_m*nao_m matrices; J and K are
+ * accumulated into them.  Densities are taken as symmetric: the result is
+ * that of (D + D^T)/2.  Asynchronous on
+ * `stream` (the graph runs on the batch's own stream, ordered after the work
+ * queued on `stream` before the call and before the work queued after it).
+ * Calls on one batch are serialised; a batch is not shared between threads
+ * that need them to overlap. */
+int ig_batch_jk_device(ig_batch *batch, const double *const *d_dens, double threshold,
+                       int want_exchange, double *const *d_J, double *const *d_K, void *stream);
+/* Quartets accepted ([0]) and primitive quartets ([1]) of the last call,
+ * summed over the molecules.  Synchronises the batch's stream. */
+int ig_batch_last_stats(ig_batch *batch, int64_t *stats2);
+
 /* ---- ERI-derivative contraction: DirectGradContractor (direct_grad.rs) --- */
 typedef struct ig_grad ig_grad;
 typedef struct ig_grad_density ig_grad_density;

@@ -196,6 +196,15 @@ def library():
     lib.ig_direct_schedule_read.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz,
                                             ctypes.POINTER(sz)]
     lib.ig_direct_schedule_write.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz]
+    lib.ig_batch_create.argtypes = [ctypes.POINTER(_vp), sz, ctypes.POINTER(_vp)]
+    lib.ig_batch_destroy.argtypes = [_vp]
+    lib.ig_batch_destroy.restype = None
+    lib.ig_batch_size.argtypes = [_vp]
+    lib.ig_batch_size.restype = sz
+    lib.ig_batch_set_mixed_precision.argtypes = [_vp, ctypes.c_double]
+    lib.ig_batch_jk_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_int,
+                                       ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]
+    lib.ig_batch_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
     lib.ig_grad_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
     lib.ig_grad_destroy.argtypes = [_vp]
     lib.ig_grad_natom.argtypes = [_vp]
@@ -533,6 +542,68 @@ class DirectContractor:
         """Quartets with Q_ij*Q_kl < fp32_below run with FP32 tables and block
         in the whole-list J/K calls (0 = FP64 only)."""
         _raise(library().ig_direct_set_mixed_precision(self._h, float(fp32_below)))
+
+
+class BatchContractor:
+    """J/K of many molecules in one HIP graph (no counterpart in the
+    reference, whose DirectContractor serves one basis, direct.rs:39-131):
+    one plan per basis, one persistent launch per quartet class over the
+    concatenated quartet lists of every molecule, the sequence captured once
+    and replayed while threshold, exchange flag and buffers stay the same."""
+
+    def __init__(self, bases):
+        bases = list(bases)
+        if not bases:
+            raise UsageError("a batch needs at least one basis")
+        hs = (_vp * len(bases))(*[b._h for b in bases])
+        h = _vp()
+        _raise(library().ig_batch_create(hs, len(bases), ctypes.byref(h)))
+        self._h = h
+        self._bases = bases  # the plans copy them; kept for nao()
+        self._nao = [b.nao() for b in bases]
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.ig_batch_destroy(self._h)
+            self._h = None
+
+    def __len__(self):
+        return int(library().ig_batch_size(self._h))
+
+    def nao(self, m):
+        return self._nao[m]
+
+    def set_mixed_precision(self, fp32_below):
+        """As DirectContractor.set_mixed_precision, for every molecule."""
+        _raise(library().ig_batch_set_mixed_precision(self._h, float(fp32_below)))
+
+    def jk_device(self, dens, J, K=None, threshold=0.0, stream=None):
+        """dens, J, K: lists (one per molecule) of contiguous float64 CUDA
+        tensors of nao_m x nao_m; the J and K of the symmetrised densities
+        are added to J/K.  Asynchronous on `stream`."""
+        n = len(self._nao)
+        lists = [dens, J] + ([K] if K is not None else [])
+        for lst in lists:
+            if len(lst) != n:
+                raise UsageError("one density and one output per molecule")
+            for m, t in enumerate(lst):
+                if not (t.is_cuda and t.element_size() == 8 and t.dtype.is_floating_point
+                        and t.is_contiguous() and t.numel() == self._nao[m] ** 2):
+                    raise UsageError("device buffers must be contiguous float64 CUDA tensors "
+                                     "of nao*nao for their molecule")
+
+        def arr(lst):
+            return (_vp * n)(*[t.data_ptr() for t in lst])
+
+        _raise(library().ig_batch_jk_device(self._h, arr(dens), float(threshold),
+                                            1 if K is not None else 0, arr(J),
+                                            arr(K) if K is not None else None,
+                                            _stream_ptr(stream)))
+
+    def last_stats(self):
+        st = (ctypes.c_int64 * 2)()
+        _raise(library().ig_batch_last_stats(self._h, st))
+        return int(st[0]), int(st[1])
 
 
 class GradDensityTerm:

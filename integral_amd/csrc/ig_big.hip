@@ -190,15 +190,31 @@ __device__ __forceinline__ void tables_rt(int la, int lb, int lc, int ld, int r0
 }
 
 // Contracted Cartesian block of quartet (pb | pk) into blk (N4 doubles).
+// Table offsets of every block element of class (la..ld), packed 21 bits
+// per axis (x | y << 21 | z << 42): one table for the whole launch, read with
+// coalesced loads instead of decoding the element's Cartesian powers.
+__global__ void big_offsets_kernel(int la, int lb, int lc, int ld,
+                                   unsigned long long *__restrict__ out) {
+  const BigClass K(la, lb, lc, ld, 0);
+  const int SA = (lb + 1) * (lc + 1) * (ld + 1), SB = (lc + 1) * (ld + 1), SC = ld + 1;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < K.N4; e += gridDim.x * blockDim.x) {
+    int pa[3], pb[3], pc[3], pd[3];
+    K.powers(e, pa, pb, pc, pd);
+    unsigned long long v = 0;
+    for (int ax = 0; ax < 3; ++ax)
+      v |= (unsigned long long)(pa[ax] * SA + pb[ax] * SB + pc[ax] * SC + pd[ax]) << (21 * ax);
+    out[e] = v;
+  }
+}
+
 __device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const RysDev &R, int pb,
                           int pk, const BigGeom &g, double omega, BigStatic &S, double *dyn,
-                          double *__restrict__ blk) {
+                          const unsigned long long *__restrict__ eoff, double *__restrict__ blk) {
   const int T = table_size(K.la, K.lb, K.lc, K.ld);
   double *tab = dyn, *work = dyn + 3 * rb * T;
   for (int e = threadIdx.x; e < K.N4; e += blockDim.x) blk[e] = 0.0;
   const int ob = p.pp_off[pb], nbp = p.pp_n[pb], ok = p.pp_off[pk], nkp = p.pp_n[pk];
-  const int SA = (K.lb + 1) * (K.lc + 1) * (K.ld + 1), SB = (K.lc + 1) * (K.ld + 1),
-            SC = K.ld + 1;
+  constexpr unsigned long long m21 = (1ull << 21) - 1;
   for (int ib = 0; ib < nbp; ++ib)
     for (int ik = 0; ik < nkp; ++ik) {
       PrimQ q;
@@ -208,11 +224,8 @@ __device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const Ry
         const int nr = K.NR - r0 < rb ? K.NR - r0 : rb;
         tables_rt(K.la, K.lb, K.lc, K.ld, r0, nr, q, g, S, tab, work);
         for (int e = threadIdx.x; e < K.N4; e += blockDim.x) {
-          int pa[3], pbp[3], pc[3], pd[3];
-          K.powers(e, pa, pbp, pc, pd);
-          const int ox = pa[0] * SA + pbp[0] * SB + pc[0] * SC + pd[0];
-          const int oy = pa[1] * SA + pbp[1] * SB + pc[1] * SC + pd[1];
-          const int oz = pa[2] * SA + pbp[2] * SB + pc[2] * SC + pd[2];
+          const unsigned long long o = eoff[e];
+          const int ox = int(o & m21), oy = int((o >> 21) & m21), oz = int(o >> 42);
           double v = 0.0;
           for (int k = 0; k < nr; ++k) {
             const double *tx = tab + 3 * k * T;
@@ -237,8 +250,8 @@ __device__ __forceinline__ void next_quartet(const QuartetSet &qs, long long t, 
 // contract, Cartesian digestion of the canonical quartet with weight f).
 __global__ void __launch_bounds__(kBigThreads)
     jk_big_kernel(int la, int lb, int lc, int ld, int rb, BasisDev b, PairsDev p, RysDev R,
-                  QuartetSet qs, DensDev dd, double omega, double *scratch, size_t slot,
-                  unsigned long long *counters) {
+                  QuartetSet qs, DensDev dd, double omega, const unsigned long long *eoff,
+                  double *scratch, size_t slot, unsigned long long *counters) {
   __shared__ BigStatic S;
   extern __shared__ double dyn[];
   const BigClass K(la, lb, lc, ld, 0);
@@ -264,7 +277,7 @@ __global__ void __launch_bounds__(kBigThreads)
       if (pair_bound * m >= dd.tau) mask |= 1ull << d;
     }
     if (!mask) continue;  // uniform over the workgroup
-    big_block(K, rb, p, R, pb, pk, g, omega, S, dyn, blk);
+    big_block(K, rb, p, R, pb, pk, g, omega, S, dyn, eoff, blk);
     __syncthreads();  // the epilogue reads elements written by other lanes
     if (threadIdx.x == 0) {
       ++my_q;
@@ -362,7 +375,8 @@ __device__ __constant__ int kBigPerms8[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 
 
 __global__ void __launch_bounds__(kBigThreads)
     eri_big_kernel(int la, int lb, int lc, int ld, int rb, BasisDev b, PairsDev p, RysDev R,
-                   QuartetSet qs, double *out, double *scratch, size_t slot) {
+                   QuartetSet qs, double *out, const unsigned long long *eoff, double *scratch,
+                   size_t slot) {
   __shared__ BigStatic S;
   extern __shared__ double dyn[];
   const BigClass K(la, lb, lc, ld, 0);
@@ -373,7 +387,7 @@ __global__ void __launch_bounds__(kBigThreads)
     next_quartet(qs, t, pb, pk);
     BigGeom g;
     g.load(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk]);
-    big_block(K, rb, p, R, pb, pk, g, 0.0, S, dyn, half0);
+    big_block(K, rb, p, R, pb, pk, g, 0.0, S, dyn, eoff, half0);
     __syncthreads();
     double *c0 = half0, *c1 = half1;
     const int sh[4] = {g.sA, g.sB, g.sC, g.sD};
@@ -435,7 +449,8 @@ __global__ void __launch_bounds__(kBigThreads)
 // (screening.rs:17-52), spherical shells transformed on the fly.
 __global__ void __launch_bounds__(kBigThreads)
     schwarz_big_kernel(int la, int lb, int rb, BasisDev b, PairsDev p, RysDev R, int first,
-                       int count, double *qout, double *scratch, size_t slot) {
+                       int count, double *qout, const unsigned long long *eoff, double *scratch,
+                       size_t slot) {
   __shared__ BigStatic S;
   extern __shared__ double dyn[];
   const BigClass K(la, lb, la, lb, 0);
@@ -445,7 +460,7 @@ __global__ void __launch_bounds__(kBigThreads)
     const int pr = first + t;
     BigGeom g;
     g.load(b, p.sh1[pr], p.sh2[pr], p.sh1[pr], p.sh2[pr]);
-    big_block(K, rb, p, R, pr, pr, g, 0.0, S, dyn, blk);
+    big_block(K, rb, p, R, pr, pr, g, 0.0, S, dyn, eoff, blk);
     __syncthreads();
     const bool sa = b.kind[g.sA], sb = b.kind[g.sB];
     const int na = K.na, nb = K.nb;
@@ -641,10 +656,23 @@ void root_batch(int la, int lb, int lc, int ld, int nr, int &rb, size_t &bytes) 
 
 }  // namespace
 
+// Scratch layout of one launch: the element offset table (N4 64-bit words,
+// rounded up to 32 for alignment), then one block slot per workgroup.
 size_t big_scratch_elems(int la, int lb, int lc, int ld, long long total, int two_halves) {
-  const size_t slot = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld) * (two_halves ? 2 : 1);
-  return size_t(big_grid(total, slot)) * slot;
+  const size_t n4 = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
+  const size_t slot = n4 * (two_halves ? 2 : 1);
+  return (n4 + 31) / 32 * 32 + size_t(big_grid(total, slot)) * slot;
 }
+
+namespace {
+// offset table at the front of scratch; returns the first slot
+double *big_prepare(int la, int lb, int lc, int ld, double *scratch, hipStream_t s) {
+  const int n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+  hipLaunchKernelGGL(big_offsets_kernel, dim3((n4 + 255) / 256), dim3(256), 0, s, la, lb, lc, ld,
+                     reinterpret_cast<unsigned long long *>(scratch));
+  return scratch + (n4 + 31) / 32 * 32;
+}
+}  // namespace
 
 hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                          const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
@@ -654,8 +682,10 @@ hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, cons
   int rb;
   size_t lds;
   root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
-  hipLaunchKernelGGL(jk_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la, lb,
-                     lc, ld, rb, b, p, r, qs, d, omega, scratch, slot, ctr);
+  double *slots = big_prepare(la, lb, lc, ld, scratch, s);
+  hipLaunchKernelGGL(jk_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la,
+                     lb, lc, ld, rb, b, p, r, qs, d, omega,
+                     reinterpret_cast<const unsigned long long *>(scratch), slots, slot, ctr);
   return hipGetLastError();
 }
 
@@ -667,8 +697,10 @@ hipError_t launch_eri_big(int la, int lb, int lc, int ld, const BasisDev &b, con
   int rb;
   size_t lds;
   root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
-  hipLaunchKernelGGL(eri_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la, lb,
-                     lc, ld, rb, b, p, r, qs, out, scratch, slot);
+  double *slots = big_prepare(la, lb, lc, ld, scratch, s);
+  hipLaunchKernelGGL(eri_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la,
+                     lb, lc, ld, rb, b, p, r, qs, out,
+                     reinterpret_cast<const unsigned long long *>(scratch), slots, slot);
   return hipGetLastError();
 }
 
@@ -680,8 +712,10 @@ hipError_t launch_schwarz_big(int la, int lb, const BasisDev &b, const PairsDev 
   int rb;
   size_t lds;
   root_batch(la, lb, la, lb, la + lb + 1, rb, lds);
-  hipLaunchKernelGGL(schwarz_big_kernel, dim3(big_grid(count, slot)), dim3(kBigThreads), lds, s, la, lb,
-                     rb, b, p, r, first, count, q, scratch, slot);
+  double *slots = big_prepare(la, lb, la, lb, scratch, s);
+  hipLaunchKernelGGL(schwarz_big_kernel, dim3(big_grid(count, slot)), dim3(kBigThreads), lds, s,
+                     la, lb, rb, b, p, r, first, count, q,
+                     reinterpret_cast<const unsigned long long *>(scratch), slots, slot);
   return hipGetLastError();
 }
 

@@ -204,4 +204,25 @@ __device__ __forceinline__ void init_offsets(int tl, int *offs) {
   X(2, 1, 0, 0) X(2, 1, 1, 0) X(2, 1, 1, 1) X(2, 1, 2, 0) X(2, 1, 2, 1)                      \
   X(2, 2, 0, 0) X(2, 2, 1, 0) X(2, 2, 1, 1) X(2, 2, 2, 0) X(2, 2, 2, 1) X(2, 2, 2, 2)
 
+// Class pairs with an f pair on the bra and blocks of at most 3600 Cartesian
+// elements: compiled J/K team kernels ((ff|fd) and (ff|ff) stay on the
+// runtime-l kernels, as do every other f kernel: ERI, Schwarz, gradient).
+#define IG_FOR_F_TEAM_CLASSES(X)                                                             \
+  X(3, 0, 0, 0) X(3, 0, 1, 0) X(3, 0, 1, 1) X(3, 0, 2, 0) X(3, 0, 2, 1) X(3, 0, 2, 2)        \
+  X(3, 0, 3, 0)                                                                              \
+  X(3, 1, 0, 0) X(3, 1, 1, 0) X(3, 1, 1, 1) X(3, 1, 2, 0) X(3, 1, 2, 1) X(3, 1, 2, 2)        \
+  X(3, 1, 3, 0) X(3, 1, 3, 1)                                                                \
+  X(3, 2, 0, 0) X(3, 2, 1, 0) X(3, 2, 1, 1) X(3, 2, 2, 0) X(3, 2, 2, 1) X(3, 2, 2, 2)        \
+  X(3, 2, 3, 0) X(3, 2, 3, 1) X(3, 2, 3, 2)                                                  \
+  X(3, 3, 0, 0) X(3, 3, 1, 0) X(3, 3, 1, 1) X(3, 3, 2, 0) X(3, 3, 2, 1) X(3, 3, 2, 2)        \
+  X(3, 3, 3, 0) X(3, 3, 3, 1)
+
+// The f class pairs of at most 270 elements (four register slices) also have
+// the lane-per-quartet kernels (register, row sums; FP64 and FP32; batched).
+#define IG_FOR_F_LANE_CLASSES(X)                                                             \
+  X(3, 0, 0, 0) X(3, 0, 1, 0) X(3, 0, 1, 1) X(3, 0, 2, 0) X(3, 0, 2, 1) X(3, 0, 3, 0)        \
+  X(3, 1, 0, 0) X(3, 1, 1, 0) X(3, 1, 1, 1) X(3, 1, 2, 0)                                    \
+  X(3, 2, 0, 0) X(3, 2, 1, 0)                                                                \
+  X(3, 3, 0, 0)
+
 }  // namespace ig

@@ -127,7 +127,12 @@ struct GradDev {
 
 // ---------------------------------------------------------------------------
 // kernel launchers (ig_kernels.hip)
+// every compiled kernel (J/K, ERI, Schwarz, gradient): shells up to d
 bool class_supported(int la, int lb, int lc, int ld);
+// compiled J/K kernels: the team kernel (class_supported plus the f classes
+// of IG_FOR_F_TEAM_CLASSES) and the lane kernels (plus IG_FOR_F_LANE_CLASSES)
+bool jk_team_compiled(int la, int lb, int lc, int ld);
+bool jk_lane_compiled(int la, int lb, int lc, int ld);
 hipError_t launch_schwarz(int lcls1, int lcls2, const BasisDev &b, const PairsDev &p,
                           const RysDev &r, int first, int count, double *qout, hipStream_t s);
 // J/K kernel variants of a compiled class: one quartet per team of lanes
@@ -140,6 +145,8 @@ constexpr size_t kRowAccLdsMax = 64 * 1024;  // dynamic LDS of one row-accumulat
 int jk_default_variant(int la, int lb, int lc, int ld);
 // width of the row accumulators' compact column space for ket momenta lc >= ld
 int rowacc_width(const BasisDev &b, int lc, int ld);
+// LDS accumulator doubles of the row-sum kernels for class (la lb|lc ld)
+size_t rowacc_acc_doubles(const BasisDev &b, int la, int lb, int lc, int ld, int neff);
 // can `variant` run this class on this basis (row accumulation: implicit
 // quartet sets only, and its LDS must fit)?
 bool jk_variant_fits(int variant, int la, int lb, int lc, int ld, const BasisDev &b, int neff,
@@ -156,6 +163,42 @@ hipError_t launch_jk_lane_f32(int la, int lb, int lc, int ld, const BasisDev &b,
                               const PairsDev &p, const RysDev &r, const QuartetSet &qs,
                               const DensDev &d, double omega, int variant,
                               unsigned long long *counters, hipStream_t s);
+// the same for the f classes (separate translation units; the launchers
+// above hand classes they do not hold to these)
+hipError_t launch_jk_lane_fcls(int la, int lb, int lc, int ld, const BasisDev &b,
+                               const PairsDev &p, const RysDev &r, const QuartetSet &qs,
+                               const DensDev &d, double omega, int variant,
+                               unsigned long long *counters, hipStream_t s);
+hipError_t launch_jk_lane_f32_fcls(int la, int lb, int lc, int ld, const BasisDev &b,
+                                   const PairsDev &p, const RysDev &r, const QuartetSet &qs,
+                                   const DensDev &d, double omega, int variant,
+                                   unsigned long long *counters, hipStream_t s);
+// the row-sum kernel (variant kJkRowAcc or kJkHgp) over the sets of nb bases
+// in one launch per element slice: device arrays bs, ps, sets, dds (nb each),
+// seg (nb + 1 prefix sums of the set totals), queues (one zeroed counter per
+// slice); acc_doubles = LDS accumulator doubles of the widest basis
+constexpr int kMaxSlices = 16;
+hipError_t launch_jk_batch(int la, int lb, int lc, int ld, const BasisDev *bs, const PairsDev *ps,
+                           const RysDev &r, const QuartetSet *sets, const DensDev *dds,
+                           const long long *seg, int nb, long long total, unsigned *queues,
+                           int variant, size_t acc_doubles, unsigned long long *counters,
+                           hipStream_t s);
+hipError_t launch_jk_batch_f32(int la, int lb, int lc, int ld, const BasisDev *bs,
+                               const PairsDev *ps, const RysDev &r, const QuartetSet *sets,
+                               const DensDev *dds, const long long *seg, int nb, long long total,
+                               unsigned *queues, int variant, size_t acc_doubles,
+                               unsigned long long *counters, hipStream_t s);
+hipError_t launch_jk_batch_fcls(int la, int lb, int lc, int ld, const BasisDev *bs,
+                                const PairsDev *ps, const RysDev &r, const QuartetSet *sets,
+                                const DensDev *dds, const long long *seg, int nb, long long total,
+                                unsigned *queues, int variant, size_t acc_doubles,
+                                unsigned long long *counters, hipStream_t s);
+hipError_t launch_jk_batch_f32_fcls(int la, int lb, int lc, int ld, const BasisDev *bs,
+                                    const PairsDev *ps, const RysDev &r, const QuartetSet *sets,
+                                    const DensDev *dds, const long long *seg, int nb,
+                                    long long total, unsigned *queues, int variant,
+                                    size_t acc_doubles, unsigned long long *counters,
+                                    hipStream_t s);
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                       const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s);
 // lane-per-quartet adjoint gradient (ig_grad_lane.hip); natom sizes its LDS

@@ -360,12 +360,30 @@ bool class_supported(int la, int lb, int lc, int ld) {
          pair_class(la, lb) >= pair_class(lc, ld);
 }
 
+bool jk_team_compiled(int la, int lb, int lc, int ld) {
+  if (class_supported(la, lb, lc, ld)) return true;
+#define IG_CASE(A, B, Cc, D) \
+  if (la == A && lb == B && lc == Cc && ld == D) return true;
+  IG_FOR_F_TEAM_CLASSES(IG_CASE)
+#undef IG_CASE
+  return false;
+}
+
+bool jk_lane_compiled(int la, int lb, int lc, int ld) {
+  if (class_supported(la, lb, lc, ld)) return true;
+#define IG_CASE(A, B, Cc, D) \
+  if (la == A && lb == B && lc == Cc && ld == D) return true;
+  IG_FOR_F_LANE_CLASSES(IG_CASE)
+#undef IG_CASE
+  return false;
+}
+
 int jk_default_variant(int la, int lb, int lc, int ld) {
   // the autotuner's usual choice on s,p,d bases: register kernels with row
   // sums while a quartet needs at most two element slices (each slice
   // rebuilds the quartet's 2D tables), team kernels for larger blocks
   const int n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-  return n4 <= 2 * kRegBlockMax ? kJkRowAcc : kJkTeam;
+  return n4 <= 2 * kRegBlockMax && jk_lane_compiled(la, lb, lc, ld) ? kJkRowAcc : kJkTeam;
 }
 
 hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
@@ -377,6 +395,7 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const Pa
   if (la == A && lb == B && lc == Cc && ld == D) \
     return jk_team_one<A, B, Cc, D>(b, p, r, qs, d, omega, ctr, s);
   IG_FOR_CLASSES(IG_CASE)
+  IG_FOR_F_TEAM_CLASSES(IG_CASE)
 #undef IG_CASE
   return hipErrorInvalidValue;
 }

@@ -2,8 +2,10 @@
 // block in registers): the plain register path (ig_reg.h) and the register
 // path with per-row LDS sums (ig_rowacc.h), with their class dispatch, for the
 // arithmetic type IG_LANE_T (double, or float for the mixed-precision
-// quartets), exported as IG_LANE_FN.  Each type is its own translation unit
-// (ig_jk_lane64.hip, ig_jk_lane32.hip) so they compile side by side.
+// quartets), exported as IG_LANE_FN, over the class list IG_LANE_CLASSES
+// (classes it does not hold go on to IG_LANE_NEXT).  Each type and list is
+// its own translation unit (ig_jk_lane{64,32}{,f}.hip) so they compile side
+// by side.
 #include "ig_cls.h"
 
 namespace ig {
@@ -19,9 +21,15 @@ int rowacc_width(const BasisDev &b, int lc, int ld) {
   return nC * ncart(lc) + (lc == ld ? 0 : nD * ncart(ld));
 }
 
+size_t rowacc_acc_doubles(const BasisDev &b, int la, int lb, int lc, int ld, int neff) {
+  return rowacc_doubles(ncart(la), ncart(lb), neff, rowacc_width(b, lc, ld));
+}
+
 bool jk_variant_fits(int variant, int la, int lb, int lc, int ld, const BasisDev &b, int neff,
                      bool listed) {
-  if (variant != kJkRowAcc && variant != kJkHgp) return true;
+  if (variant == kJkTeam) return jk_team_compiled(la, lb, lc, ld);
+  if (!jk_lane_compiled(la, lb, lc, ld)) return false;
+  if (variant == kJkReg) return true;
   if (listed) return false;
   if (variant == kJkHgp && la + lb + lc + ld > 2) return false;
   const int nr = (la + lb + lc + ld) / 2 + 1;
@@ -69,15 +77,62 @@ hipError_t jk_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r, co
 
 }  // namespace
 
+template <int LA, int LB, int LC, int LD, typename F>
+hipError_t jk_lane_batch_one(const BasisDev *bs, const PairsDev *ps, const RysDev &r,
+                             const QuartetSet *sets, const DensDev *dds, const long long *seg,
+                             int nb, long long total, unsigned *queues, int variant,
+                             size_t acc_doubles, unsigned long long *ctr, hipStream_t s) {
+  using C = Cls<LA, LB, LC, LD>;
+  if (total <= 0) return hipSuccess;
+  const long long chunk = 4 * 64 * kRowWaves;  // quartets per work item
+  long long blocks = (total + chunk - 1) / chunk;
+  if (blocks > 1024) blocks = 1024;
+  const size_t rys = size_t(2) * r.nsub[C::NR] * C::NR * kRysTerms * sizeof(double);
+  if (variant == kJkHgp) {
+    if constexpr (HgpClass<C>::ok)
+      return launch_jk_rowacc_batch_slices<C, 0, F, 1>(bs, ps, r, sets, dds, seg, nb, queues,
+                                                       chunk, ctr, s, int(blocks),
+                                                       acc_doubles * sizeof(double));
+    else
+      return hipErrorInvalidValue;
+  }
+  return launch_jk_rowacc_batch_slices<C, 0, F, 0>(bs, ps, r, sets, dds, seg, nb, queues, chunk,
+                                                   ctr, s, int(blocks),
+                                                   rys + acc_doubles * sizeof(double));
+}
+
+hipError_t IG_LANE_BATCH_FN(int la, int lb, int lc, int ld, const BasisDev *bs, const PairsDev *ps,
+                           const RysDev &r, const QuartetSet *sets, const DensDev *dds,
+                           const long long *seg, int nb, long long total, unsigned *queues,
+                           int variant, size_t acc_doubles, unsigned long long *ctr,
+                           hipStream_t s) {
+#define IG_CASE(A, B, Cc, D)                       \
+  if (la == A && lb == B && lc == Cc && ld == D) \
+    return jk_lane_batch_one<A, B, Cc, D, IG_LANE_T>(bs, ps, r, sets, dds, seg, nb, total, \
+                                                     queues, variant, acc_doubles, ctr, s);
+  IG_LANE_CLASSES(IG_CASE)
+#undef IG_CASE
+#ifdef IG_LANE_BATCH_NEXT
+  return IG_LANE_BATCH_NEXT(la, lb, lc, ld, bs, ps, r, sets, dds, seg, nb, total, queues, variant,
+                            acc_doubles, ctr, s);
+#else
+  return hipErrorInvalidValue;
+#endif
+}
+
 hipError_t IG_LANE_FN(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                      const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
                      int variant, unsigned long long *ctr, hipStream_t s) {
 #define IG_CASE(A, B, Cc, D)                       \
   if (la == A && lb == B && lc == Cc && ld == D) \
     return jk_lane_one<A, B, Cc, D, IG_LANE_T>(b, p, r, qs, d, omega, variant, ctr, s);
-  IG_FOR_CLASSES(IG_CASE)
+  IG_LANE_CLASSES(IG_CASE)
 #undef IG_CASE
+#ifdef IG_LANE_NEXT
+  return IG_LANE_NEXT(la, lb, lc, ld, b, p, r, qs, d, omega, variant, ctr, s);
+#else
   return hipErrorInvalidValue;
+#endif
 }
 
 }  // namespace ig

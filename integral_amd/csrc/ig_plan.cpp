@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -771,6 +773,12 @@ size_t big_need(const ClassRange &X, const ClassRange &Y, long long total, int t
   return big_scratch_elems(X.l1, X.l2, Y.l1, Y.l2, total, two_halves);
 }
 
+// the same for J/K, whose compiled kernels also cover most f classes
+size_t big_need_jk(const ClassRange &X, const ClassRange &Y, long long total) {
+  if (total <= 0 || jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) return 0;
+  return big_scratch_elems(X.l1, X.l2, Y.l1, Y.l2, total, 0);
+}
+
 int jk_variant(const Plan &P, const ClassRange &X, const ClassRange &Y) {
   const int v = P.variant[pair_class(X.l1, X.l2) * kNumPairClass + pair_class(Y.l1, Y.l2)];
   return v >= 0 ? v : jk_default_variant(X.l1, X.l2, Y.l1, Y.l2);
@@ -778,17 +786,18 @@ int jk_variant(const Plan &P, const ClassRange &X, const ClassRange &Y) {
 
 // fp32: the set's quartets run with FP32 tables and block (lane kernels only;
 // a class scheduled on the team kernel takes the register kernel, and the
-// runtime-l classes stay FP64).
+// classes without lane kernels stay FP64).
 hipError_t jk_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const QuartetSet &qs,
                      const DensDev &dd, unsigned long long *ctr, hipStream_t s, int variant = -1,
                      bool fp32 = false) {
   RysDevice &R = rys_device();
-  if (class_supported(X.l1, X.l2, Y.l1, Y.l2)) {
+  if (jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) {
+    const bool lane = jk_lane_compiled(X.l1, X.l2, Y.l1, Y.l2);
     int v = variant >= 0 ? variant : jk_variant(P, X, Y);
-    if (fp32 && v == kJkTeam) v = kJkReg;
+    if (fp32 && v == kJkTeam && lane) v = kJkReg;
     if (!jk_variant_fits(v, X.l1, X.l2, Y.l1, Y.l2, P.bdev, dd.neff, qs.list != nullptr))
-      v = kJkReg;
-    if (fp32)
+      v = lane ? kJkReg : kJkTeam;
+    if (fp32 && v != kJkTeam)
       return launch_jk_lane_f32(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, v,
                                 ctr, s);
     return launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, v, ctr, s);
@@ -809,8 +818,9 @@ hipError_t grad_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const 
 
 // Density preparation of a J/K call: symmetric/antisymmetric split, Cartesian
 // densities, shell-block bounds and zeroed Cartesian J'/K' accumulators.
-DensDev prep_densities(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
-                       bool want_k, hipStream_t s) {
+// Buffers of a J/K call (host-side work only: allocations and the effective
+// density table); the returned view is what the kernels use.
+DensDev reserve_densities(Plan &P, int ndens, bool has_anti, double tau, bool want_k) {
   const size_t n2 = size_t(P.nao) * P.nao, nc2 = size_t(P.nao_cart) * P.nao_cart;
   const int neff = ndens * (has_anti ? 2 : 1);
   P.s_ds.reserve(n2 * ndens * sizeof(double));
@@ -830,11 +840,6 @@ DensDev prep_densities(Plan &P, const double *d_dens, int ndens, bool has_anti, 
     P.s_eff.upload(eff);
     P.eff_key = neff * 128 + ndens;
   }
-  check(launch_density_prep(P.bdev, d_dens, ndens, has_anti, P.s_ds.as<double>(),
-                            P.s_da.as<double>(), P.s_dc.as<double>(), P.s_bound.as<double>(), s),
-        "density prep");
-  check(launch_fill(P.s_jc.as<double>(), nc2 * neff * ncopy, 0.0, s), "zero J");
-  if (want_k) check(launch_fill(P.s_kc.as<double>(), nc2 * neff * ncopy, 0.0, s), "zero K");
   DensDev dd{};
   dd.copy_stride = nc2 * neff;
   dd.ncopy = ncopy;
@@ -851,6 +856,25 @@ DensDev prep_densities(Plan &P, const double *d_dens, int ndens, bool has_anti, 
   return dd;
 }
 
+// Device work of the density preparation (kernels only, so it can be
+// captured in a graph): split, Cartesian densities, block bounds, zeroed J'/K'.
+void launch_densities(Plan &P, const DensDev &dd, const double *d_dens, bool has_anti,
+                      hipStream_t s) {
+  const size_t nc2 = size_t(P.nao_cart) * P.nao_cart;
+  check(launch_density_prep(P.bdev, d_dens, dd.ndens, has_anti, P.s_ds.as<double>(),
+                            P.s_da.as<double>(), P.s_dc.as<double>(), P.s_bound.as<double>(), s),
+        "density prep");
+  check(launch_fill(dd.jc, nc2 * dd.neff * dd.ncopy, 0.0, s), "zero J");
+  if (dd.kc) check(launch_fill(dd.kc, nc2 * dd.neff * dd.ncopy, 0.0, s), "zero K");
+}
+
+DensDev prep_densities(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
+                       bool want_k, hipStream_t s) {
+  const DensDev dd = reserve_densities(P, ndens, has_anti, tau, want_k);
+  launch_densities(P, dd, d_dens, has_anti, s);
+  return dd;
+}
+
 // Run J/K for ndens densities already on the device (d_dens, function space).
 void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau, bool want_k,
             double *d_J, double *d_K, const ExplicitSets *ex, hipStream_t s) {
@@ -859,7 +883,7 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
   if (!ex) {
     build_rowptrs(P, tau, P.fp32_below);
     size_t big = 0;
-    for (auto &cp : P.cached_cps) big = std::max(big, big_need(P.cls[cp.x], P.cls[cp.y], cp.total, 0));
+    for (auto &cp : P.cached_cps) big = std::max(big, big_need_jk(P.cls[cp.x], P.cls[cp.y], cp.total));
     P.s_big.reserve(big * sizeof(double));
     // heavy classes first so the long launches start early
     for (auto it = P.cached_cps.rbegin(); it != P.cached_cps.rend(); ++it) {
@@ -879,8 +903,8 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
     P.s_list.upload(flat);
     size_t big = 0;
     for (size_t c = 0; c < ex->cls.size(); ++c)
-      big = std::max(big, big_need(P.cls[ex->cls[c].first], P.cls[ex->cls[c].second],
-                                   (long long)ex->lists[c].size(), 0));
+      big = std::max(big, big_need_jk(P.cls[ex->cls[c].first], P.cls[ex->cls[c].second],
+                                      (long long)ex->lists[c].size()));
     P.s_big.reserve(big * sizeof(double));
     size_t off = 0;
     for (size_t c = 0; c < ex->cls.size(); ++c) {
@@ -1098,7 +1122,7 @@ extern "C" int ig_direct_autotune(ig_direct *d, const double *d_dens, size_t nde
     try {
       for (auto &cp : P.cached_cps) {
         const ClassRange &X = P.cls[cp.x], &Y = P.cls[cp.y];
-        if (!cp.total || !class_supported(X.l1, X.l2, Y.l1, Y.l2)) continue;
+        if (!cp.total || !jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) continue;
         const QuartetSet qs = implicit_set(P, cp);
         int best = -1;
         float best_ms = 0.0f;
@@ -1146,7 +1170,7 @@ extern "C" int ig_direct_schedule_read(const ig_direct *d, ig_schedule_entry *ou
     for (int x = 0; x < kNumPairClass; ++x)
       for (int y = 0; y <= x; ++y) {
         const ClassRange &X = P.cls[x], &Y = P.cls[y];
-        if (!X.n || !Y.n || !class_supported(X.l1, X.l2, Y.l1, Y.l2)) continue;
+        if (!X.n || !Y.n || !jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) continue;
         if (k < cap) {
           const int key = x * kNumPairClass + y;
           out[k].la = X.l1;
@@ -1170,7 +1194,7 @@ extern "C" int ig_direct_schedule_write(ig_direct *d, const ig_schedule_entry *i
     const ig_schedule_entry &e = in[k];
     const bool ok_l = e.la >= 0 && e.lb >= 0 && e.lc >= 0 && e.ld >= 0 && e.la <= kMaxL &&
                       e.lc <= kMaxL && e.lb <= e.la && e.ld <= e.lc;
-    if (!ok_l || !class_supported(e.la, e.lb, e.lc, e.ld))
+    if (!ok_l || !jk_team_compiled(e.la, e.lb, e.lc, e.ld))
       return fail(IG_ERR_INVALID_ARGUMENT,
                   "schedule entry " + std::to_string(k) + " names a class without compiled kernels");
     if (e.variant < -1 || e.variant >= kJkNumVariants)
@@ -1493,4 +1517,273 @@ extern "C" int ig_grad_last_stats(const ig_grad *g, int64_t *st) {
   st[0] = g->plan->last_stats[0];
   st[1] = g->plan->last_stats[1];
   return IG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched J/K over many bases (one symmetric density per basis)
+
+struct ig_batch {
+  std::vector<std::unique_ptr<Plan>> plans;
+  std::mutex mu;
+  hipStream_t stream = nullptr;  // the graph's own stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  // the configuration the graph was captured for
+  double tau = NAN, cut = NAN;
+  int want_k = -1;
+  std::vector<const double *> dens;
+  std::vector<double *> J, K;
+  // device views and work lists of the captured graph
+  DBuf d_bs, d_ps, d_dds, d_sets, d_seg, d_queue, d_ctr;
+  size_t nlaunch = 0;
+  double fp32_below = 0.0;
+  int64_t last_stats[2] = {0, 0};
+  bool stats_pending = false;
+  ~ig_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Capture the whole batched J/K into b.graph: per basis the density kernels,
+// then per (class pair, precision) one launch of the row-sum kernel over every
+// basis (work queue) -- or per-basis launches for classes that kernel cannot
+// run -- then per basis the fold and finishing transform.
+void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
+  const int nb = int(B.plans.size());
+  RysDevice &R = rys_device();
+  std::vector<BasisDev> bs(nb);
+  std::vector<PairsDev> ps(nb);
+  std::vector<DensDev> dds(nb);
+  for (int m = 0; m < nb; ++m) {
+    Plan &P = *B.plans[m];
+    build_rowptrs(P, tau, cut);
+    dds[m] = reserve_densities(P, 1, false, tau, want_k);
+    bs[m] = P.bdev;
+    ps[m] = P.pdev;
+  }
+  // work lists: every (x, y, fp32) key present in some basis
+  struct Key {
+    int x, y, fp32;
+    bool operator<(const Key &o) const {
+      return std::tie(x, y, fp32) < std::tie(o.x, o.y, o.fp32);
+    }
+  };
+  std::map<Key, std::vector<std::pair<int, size_t>>> work;  // key -> (basis, cp index)
+  for (int m = 0; m < nb; ++m) {
+    Plan &P = *B.plans[m];
+    for (size_t c = 0; c < P.cached_cps.size(); ++c) {
+      const Plan::CP &cp = P.cached_cps[c];
+      if (cp.total) work[{cp.x, cp.y, cp.fp32}].push_back({m, c});
+    }
+  }
+  struct Launch {
+    Key k;
+    bool batched;
+    int variant;
+    size_t acc;
+    long long total;
+    size_t set_off, seg_off;
+  };
+  std::vector<Launch> launches;
+  std::vector<QuartetSet> sets;
+  std::vector<long long> seg;
+  for (auto &kv : work) {
+    const Key &k = kv.first;
+    const Plan &P0 = *B.plans[kv.second[0].first];
+    const ClassRange &X0 = P0.cls[k.x], &Y0 = P0.cls[k.y];
+    Launch L{k, false, -1, 0, 0, sets.size(), seg.size()};
+    if (jk_lane_compiled(X0.l1, X0.l2, Y0.l1, Y0.l2)) {
+      int v = jk_default_variant(X0.l1, X0.l2, Y0.l1, Y0.l2);
+      if (v == kJkTeam && k.fp32) v = kJkRowAcc;
+      if (v == kJkRowAcc || v == kJkHgp) {
+        bool fits = true;
+        for (auto &mc : kv.second) {
+          const Plan &P = *B.plans[mc.first];
+          fits = fits && jk_variant_fits(v, X0.l1, X0.l2, Y0.l1, Y0.l2, P.bdev, 1, false);
+          L.acc = std::max(L.acc, rowacc_acc_doubles(P.bdev, X0.l1, X0.l2, Y0.l1, Y0.l2, 1));
+        }
+        if (fits) {
+          L.batched = true;
+          L.variant = v;
+        }
+      }
+    }
+    if (L.batched) {  // every basis has a slot; those without the key hold an empty set
+      std::vector<QuartetSet> row(nb, QuartetSet{});
+      std::vector<long long> pre(nb + 1, 0);
+      for (auto &mc : kv.second) row[mc.first] = implicit_set(*B.plans[mc.first],
+                                                              B.plans[mc.first]->cached_cps[mc.second]);
+      for (int m = 0; m < nb; ++m) pre[m + 1] = pre[m] + row[m].total;
+      L.total = pre[nb];
+      sets.insert(sets.end(), row.begin(), row.end());
+      seg.insert(seg.end(), pre.begin(), pre.end());
+    }
+    launches.push_back(L);
+  }
+  B.d_bs.upload(bs);
+  B.d_ps.upload(ps);
+  B.d_dds.upload(dds);
+  B.d_sets.upload(sets);
+  B.d_seg.upload(seg);
+  B.nlaunch = launches.size();
+  B.d_queue.reserve(std::max<size_t>(1, launches.size()) * kMaxSlices * sizeof(unsigned));
+  B.d_ctr.reserve(std::max<size_t>(1, launches.size()) * 2 * sizeof(unsigned long long));
+  // runtime-l scratch of the per-basis launches, sized before capture
+  for (auto &L : launches) {
+    if (L.batched) continue;
+    for (auto &mc : work[L.k]) {
+      Plan &P = *B.plans[mc.first];
+      const Plan::CP &cp = P.cached_cps[mc.second];
+      const size_t need = big_need_jk(P.cls[cp.x], P.cls[cp.y], cp.total);
+      if (need > P.s_big.bytes / sizeof(double)) P.s_big.reserve(need * sizeof(double));
+    }
+  }
+  hipStream_t s = B.stream;
+  check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    for (int m = 0; m < nb; ++m) launch_densities(*B.plans[m], dds[m], B.dens[m], false, s);
+    // counters and queues cleared by kernels (no memset nodes in the graph)
+    check(launch_fill(B.d_queue.as<double>(), B.d_queue.bytes / sizeof(double), 0.0, s), "zero queues");
+    check(launch_fill(B.d_ctr.as<double>(), B.d_ctr.bytes / sizeof(double), 0.0, s), "zero counters");
+    for (size_t li = 0; li < launches.size(); ++li) {
+      const Launch &L = launches[li];
+      unsigned long long *ctr = B.d_ctr.as<unsigned long long>() + 2 * li;
+      const Plan &P0 = *B.plans[work[L.k][0].first];
+      const ClassRange &X0 = P0.cls[L.k.x], &Y0 = P0.cls[L.k.y];
+      if (L.batched) {
+        auto fn = L.k.fp32 ? launch_jk_batch_f32 : launch_jk_batch;
+        check(fn(X0.l1, X0.l2, Y0.l1, Y0.l2, B.d_bs.as<BasisDev>(), B.d_ps.as<PairsDev>(), R.view,
+                 B.d_sets.as<QuartetSet>() + L.set_off, B.d_dds.as<DensDev>(),
+                 B.d_seg.as<long long>() + L.seg_off, nb, L.total,
+                 B.d_queue.as<unsigned>() + li * kMaxSlices, L.variant, L.acc, ctr, s),
+              "batched jk launch");
+      } else {
+        for (auto &mc : work[L.k]) {
+          Plan &P = *B.plans[mc.first];
+          const Plan::CP &cp = P.cached_cps[mc.second];
+          check(jk_launch(P, P.cls[cp.x], P.cls[cp.y], implicit_set(P, cp), dds[mc.first], ctr, s,
+                          -1, cp.fp32 != 0),
+                "jk launch");
+        }
+      }
+    }
+    for (int m = 0; m < nb; ++m) {
+      Plan &P = *B.plans[m];
+      check(launch_fold_copies(dds[m], size_t(P.nao_cart) * P.nao_cart, s), "fold copies");
+      check(launch_jk_finish(P.bdev, dds[m], B.J[m], want_k ? B.K[m] : nullptr, s), "jk finish");
+    }
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  if (B.exec) (void)hipGraphExecDestroy(B.exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  B.exec = nullptr;
+  B.graph = nullptr;
+  check(hipStreamEndCapture(s, &B.graph), "end capture");
+  check(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "instantiate graph");
+}
+
+}  // namespace
+
+extern "C" int ig_batch_create(const ig_basis *const *bases, size_t n, ig_batch **out) {
+  if (!out || (n && !bases)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (n == 0) return fail(IG_ERR_INVALID_ARGUMENT, "a batch needs at least one basis");
+  for (size_t m = 0; m < n; ++m)
+    if (!bases[m]) return fail(IG_ERR_INVALID_ARGUMENT, "null basis in the batch");
+  return guarded([&] {
+    auto B = std::make_unique<ig_batch>();
+    for (size_t m = 0; m < n; ++m) B->plans.push_back(build_plan(*bases[m], nullptr));
+    check(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking), "stream");
+    check(hipEventCreateWithFlags(&B->ev_in, hipEventDisableTiming), "event");
+    check(hipEventCreateWithFlags(&B->ev_out, hipEventDisableTiming), "event");
+    *out = B.release();
+    return int(IG_OK);
+  });
+}
+
+extern "C" void ig_batch_destroy(ig_batch *b) {
+  if (b) (void)hipStreamSynchronize(b->stream);
+  delete b;
+}
+
+extern "C" size_t ig_batch_size(const ig_batch *b) { return b ? b->plans.size() : 0; }
+
+extern "C" int ig_batch_set_mixed_precision(ig_batch *b, double fp32_below) {
+  if (!b) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (!(std::isfinite(fp32_below) && fp32_below >= 0.0))
+    return fail(IG_ERR_INVALID_ARGUMENT, "fp32_below must be finite and >= 0");
+  std::lock_guard<std::mutex> g(b->mu);
+  b->fp32_below = fp32_below;
+  return IG_OK;
+}
+
+extern "C" int ig_batch_jk_device(ig_batch *b, const double *const *d_dens, double threshold,
+                                  int want_k, double *const *d_J, double *const *d_K,
+                                  void *stream) {
+  if (!b || !d_dens || !d_J || (want_k && !d_K)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  const size_t nb = b->plans.size();
+  for (size_t m = 0; m < nb; ++m)
+    if (!d_dens[m] || !d_J[m] || (want_k && !d_K[m]))
+      return fail(IG_ERR_INVALID_ARGUMENT, "null device pointer for basis " + std::to_string(m));
+  return guarded([&] {
+    std::lock_guard<std::mutex> g(b->mu);
+    const double cut = b->fp32_below > threshold ? b->fp32_below : threshold;
+    std::vector<const double *> dens(d_dens, d_dens + nb);
+    std::vector<double *> J(d_J, d_J + nb), K(nb, nullptr);
+    if (want_k) K.assign(d_K, d_K + nb);
+    if (!b->exec || b->tau != threshold || b->cut != cut || b->want_k != (want_k != 0) ||
+        b->dens != dens || b->J != J || b->K != K) {
+      check(hipStreamSynchronize(b->stream), "sync");
+      b->dens = dens;
+      b->J = J;
+      b->K = K;
+      b->tau = threshold;
+      b->cut = cut;
+      b->want_k = want_k != 0;
+      batch_capture(*b, threshold, cut, want_k != 0);
+    }
+    hipStream_t s = as_stream(stream);
+    check(hipEventRecord(b->ev_in, s), "event");
+    check(hipStreamWaitEvent(b->stream, b->ev_in, 0), "wait");
+    check(hipGraphLaunch(b->exec, b->stream), "graph launch");
+    check(hipEventRecord(b->ev_out, b->stream), "event");
+    check(hipStreamWaitEvent(s, b->ev_out, 0), "wait");
+    b->stats_pending = true;
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_batch_last_stats(ig_batch *b, int64_t *st) {
+  if (!b || !st) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  return guarded([&] {
+    std::lock_guard<std::mutex> g(b->mu);
+    if (b->stats_pending) {
+      std::vector<unsigned long long> c(2 * std::max<size_t>(b->nlaunch, 1), 0);
+      check(hipStreamSynchronize(b->stream), "sync");
+      if (b->nlaunch)
+        check(hipMemcpy(c.data(), b->d_ctr.p, 2 * b->nlaunch * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost),
+              "copy counters");
+      b->last_stats[0] = b->last_stats[1] = 0;
+      for (size_t k = 0; k < b->nlaunch; ++k) {
+        b->last_stats[0] += int64_t(c[2 * k]);
+        b->last_stats[1] += int64_t(c[2 * k + 1]);
+      }
+      b->stats_pending = false;
+    }
+    st[0] = b->last_stats[0];
+    st[1] = b->last_stats[1];
+    return int(IG_OK);
+  });
 }

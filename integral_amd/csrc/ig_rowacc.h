@@ -221,29 +221,20 @@ __device__ __forceinline__ void digest_row(const BasisDev &b, const DensDev &dd,
 #undef IG_IN
 }
 
-// ENG 0: Rys quartet block (quartet_block_reg); ENG 1: the OS/HGP block of
-// ig_hgp.h (total angular momentum <= 2, Coulomb operator only).
-template <class C, int LO, int HI, typename F, int ENG = 0>
-__global__ void __launch_bounds__(64 * kRowWaves)
-    jk_rowacc_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs, DensDev dd, double omega,
-                     unsigned long long *counters) {
-  extern __shared__ double lds[];
-  __shared__ int s_row[2];  // by step parity: a slow wavefront still reading
-                            // step i's value cannot see step i+1's
-  if constexpr (ENG == 0) stage_rys<C::NR>(R, lds);
-  const int nrys = ENG == 0 ? 2 * R.nsub[C::NR] * C::NR * kRysTerms : 0;
+// Quartets [begin, end) of one basis's set: the steps, the row sums and the
+// final flush.  Called uniformly by the whole workgroup; JB holds zeros on
+// entry and on return.  ENG 0: Rys quartet block (quartet_block_reg); ENG 1:
+// the OS/HGP block of ig_hgp.h (total angular momentum <= 2, Coulomb only).
+template <class C, int LO, int HI, typename F, int ENG>
+__device__ __forceinline__ void rowacc_range(const BasisDev &b, const PairsDev &p,
+                                             const RysDev &R, const QuartetSet &qs,
+                                             const DensDev &dd, double omega, long long begin,
+                                             long long end, const double *rys_lds, double *JB,
+                                             int *s_row, unsigned long long &my_quartets,
+                                             unsigned long long &my_prims) {
   const RowCols<C> rc(b);
-  double *JB = lds + nrys;
-  const int nacc = int(rowacc_doubles(C::na, C::nb, dd.neff, rc.W));
-  for (int i = threadIdx.x; i < nacc; i += blockDim.x) JB[i] = 0.0;
-  // the barrier at the top of the first step orders these stores
-  unsigned long long my_quartets = 0, my_prims = 0;
   const int nsh = b.nsh;
-  const int lane = threadIdx.x & 63;
   const size_t xo = jk_copy_offset(dd);
-  const long long chunk = (qs.total + gridDim.x - 1) / gridDim.x;
-  const long long begin = (long long)blockIdx.x * chunk;
-  const long long end = begin + chunk < qs.total ? begin + chunk : qs.total;
   RowCursor cur;
   int racc = -1;  // accumulated row, uniform over the workgroup
   int step = 0;
@@ -252,6 +243,8 @@ __global__ void __launch_bounds__(64 * kRowWaves)
     bool on = t < end;
     int P = 0, Q = 0;
     if (on) cur.next(qs, t, P, Q);
+    // s_row by step parity: a slow wavefront still reading step i's value
+    // cannot see step i+1's
     if (threadIdx.x == 0) s_row[step & 1] = P;  // t = base < end: thread 0 has a quartet
     __syncthreads();
     const int row = s_row[step & 1];
@@ -282,7 +275,7 @@ __global__ void __launch_bounds__(64 * kRowWaves)
     for (int e = 0; e < HI - LO; ++e) acc[e] = F(0);
     if (on) {
       if constexpr (ENG == 0)
-        quartet_block_reg<C, LO, HI, F>(p, R, lds, pb, pk, g, omega, acc);
+        quartet_block_reg<C, LO, HI, F>(p, R, rys_lds, pb, pk, g, omega, acc);
       else
         quartet_block_hgp<C, LO, HI, F>(p, R, pb, pk, g, acc);
       if (LO == 0) {
@@ -304,16 +297,96 @@ __global__ void __launch_bounds__(64 * kRowWaves)
   }
   __syncthreads();
   if (racc >= 0) rowacc_flush<C>(b, p, qs, dd, rc, racc, JB, xo);
-  if (LO == 0) {
-    for (int off = 32; off > 0; off >>= 1) {
-      my_quartets += __shfl_down(my_quartets, off);
-      my_prims += __shfl_down(my_prims, off);
-    }
-    if (counters && lane == 0) {
-      atomicAdd(counters, my_quartets);
-      atomicAdd(counters + 1, my_prims);
+}
+
+template <int LO>
+__device__ __forceinline__ void rowacc_count(unsigned long long q, unsigned long long pr,
+                                             unsigned long long *counters) {
+  if (LO != 0) return;
+  for (int off = 32; off > 0; off >>= 1) {
+    q += __shfl_down(q, off);
+    pr += __shfl_down(pr, off);
+  }
+  if (counters && (threadIdx.x & 63) == 0) {
+    atomicAdd(counters, q);
+    atomicAdd(counters + 1, pr);
+  }
+}
+
+template <class C, int LO, int HI, typename F, int ENG = 0>
+__global__ void __launch_bounds__(64 * kRowWaves)
+    jk_rowacc_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs, DensDev dd, double omega,
+                     unsigned long long *counters) {
+  extern __shared__ double lds[];
+  __shared__ int s_row[2];
+  if constexpr (ENG == 0) stage_rys<C::NR>(R, lds);
+  const int nrys = ENG == 0 ? 2 * R.nsub[C::NR] * C::NR * kRysTerms : 0;
+  double *JB = lds + nrys;
+  const int nacc = int(rowacc_doubles(C::na, C::nb, dd.neff, RowCols<C>(b).W));
+  for (int i = threadIdx.x; i < nacc; i += blockDim.x) JB[i] = 0.0;
+  // the barrier at the top of the first step orders these stores
+  unsigned long long my_quartets = 0, my_prims = 0;
+  const long long chunk = (qs.total + gridDim.x - 1) / gridDim.x;
+  const long long begin = (long long)blockIdx.x * chunk;
+  const long long end = begin + chunk < qs.total ? begin + chunk : qs.total;
+  rowacc_range<C, LO, HI, F, ENG>(b, p, R, qs, dd, omega, begin, end, lds, JB, s_row,
+                                  my_quartets, my_prims);
+  rowacc_count<LO>(my_quartets, my_prims, counters);
+}
+
+// Many bases in one launch: the quartet sets of class C of every basis are
+// one concatenated list (seg: nb + 1 prefix sums of their totals).
+// Workgroups take chunks of it from a device work queue (one atomic counter,
+// zeroed before the launch), so molecules of uneven size balance.  A chunk
+// may span bases; each part runs with that basis's views (read from the
+// device arrays bs, ps, sets, dds).  The LDS accumulators are sized for the
+// widest basis.
+template <class C, int LO, int HI, typename F, int ENG = 0>
+__global__ void __launch_bounds__(64 * kRowWaves)
+    jk_rowacc_batch_kernel(const BasisDev *bs, const PairsDev *ps, RysDev R,
+                           const QuartetSet *sets, const DensDev *dds, const long long *seg,
+                           int nb, unsigned *queue, long long chunk,
+                           unsigned long long *counters) {
+  extern __shared__ double lds[];
+  __shared__ int s_row[2];
+  __shared__ long long s_item[2];
+  if constexpr (ENG == 0) stage_rys<C::NR>(R, lds);
+  const int nrys = ENG == 0 ? 2 * R.nsub[C::NR] * C::NR * kRysTerms : 0;
+  double *JB = lds + nrys;
+  int nacc = 0;
+  for (int m = 0; m < nb; ++m) {
+    const int w = int(rowacc_doubles(C::na, C::nb, dds[m].neff, RowCols<C>(bs[m]).W));
+    nacc = w > nacc ? w : nacc;
+  }
+  for (int i = threadIdx.x; i < nacc; i += blockDim.x) JB[i] = 0.0;
+  unsigned long long my_quartets = 0, my_prims = 0;
+  const long long total = seg[nb];
+  for (int it = 0;; ++it) {
+    if (threadIdx.x == 0) s_item[it & 1] = (long long)atomicAdd(queue, 1u);
+    __syncthreads();  // also orders the accumulator zeroing before first use
+    long long g0 = s_item[it & 1] * chunk;
+    if (g0 >= total) break;  // uniform: every workgroup leaves once the list is done
+    const long long g1 = g0 + chunk < total ? g0 + chunk : total;
+    while (g0 < g1) {
+      int lo = 0, hi = nb - 1;  // basis holding g0: seg[m] <= g0 < seg[m + 1]
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg[mid] <= g0)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const long long e = g1 < seg[lo + 1] ? g1 : seg[lo + 1];
+      const BasisDev b = bs[lo];
+      const PairsDev p = ps[lo];
+      const QuartetSet qs = sets[lo];
+      const DensDev dd = dds[lo];
+      rowacc_range<C, LO, HI, F, ENG>(b, p, R, qs, dd, 0.0, g0 - seg[lo], e - seg[lo], lds, JB,
+                                      s_row, my_quartets, my_prims);
+      g0 = e;
     }
   }
+  rowacc_count<LO>(my_quartets, my_prims, counters);
 }
 
 template <class C, int S, typename F, int ENG = 0>
@@ -329,6 +402,27 @@ hipError_t launch_jk_rowacc_slices(const BasisDev &b, const PairsDev &p, const R
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_jk_rowacc_slices<C, S + 1, F, ENG>(b, p, r, qs, d, omega, ctr, s, blocks, lds);
+  } else {
+    return hipSuccess;
+  }
+}
+
+template <class C, int S, typename F, int ENG = 0>
+hipError_t launch_jk_rowacc_batch_slices(const BasisDev *bs, const PairsDev *ps, const RysDev &r,
+                                         const QuartetSet *sets, const DensDev *dds,
+                                         const long long *seg, int nb, unsigned *queues,
+                                         long long chunk, unsigned long long *ctr, hipStream_t s,
+                                         int blocks, size_t lds) {
+  constexpr int NS = RegSlices<C, F>::NS, E = RegSlices<C, F>::E;
+  if constexpr (S < NS) {
+    constexpr int LO = S * E, HI = (S + 1) * E < C::N4 ? (S + 1) * E : C::N4;
+    hipLaunchKernelGGL((jk_rowacc_batch_kernel<C, LO, HI, F, ENG>), dim3(blocks),
+                       dim3(64 * kRowWaves), lds, s, bs, ps, r, sets, dds, seg, nb, queues + S,
+                       chunk, ctr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_jk_rowacc_batch_slices<C, S + 1, F, ENG>(bs, ps, r, sets, dds, seg, nb, queues,
+                                                          chunk, ctr, s, blocks, lds);
   } else {
     return hipSuccess;
   }

@@ -108,6 +108,48 @@ def gradient_config():
     return shells, D
 
 
+def high_l_config():
+    """20 atoms in a 10 bohr cube, spherical s,p,d,f,g per atom plus an l = 6
+    shell on every fourth atom, one primitive each, exponents log-uniform in
+    [0.2, 10]: nao = 565; one density D = C C^T with C nao x 20 ~ N(0, 0.1).
+    Rys rules up to 13 roots."""
+    rng = SplitMix64(20260005)
+    pts = positions(rng, 20, 10.0)
+    shells = []
+    for k, p in enumerate(pts):
+        ls = [0, 1, 2, 3, 4] + ([6] if k % 4 == 0 else [])
+        for l in ls:
+            e = _exps(rng, 1, 0.2, 10.0)
+            c = [0.1 + 0.9 * rng.uniform()]
+            shells.append((l, tuple(float(x) for x in p), e, c, True))
+    nao = sum(2 * s[0] + 1 for s in shells)
+    return shells, density(rng, nao, 20)
+
+
+def batch_config(nmol=64):
+    """nmol molecules with uniform atom counts in [5, 60] at 1 atom per
+    20 bohr^3 (cube side (20 N)^(1/3), min separation 1.5 bohr), spherical
+    s,s,p,p,d,f per atom (f one primitive, the others 1-3), exponents
+    log-uniform in [0.05, 60]; one density per molecule D = C C^T with
+    C nao x 20 ~ N(0, 0.1)."""
+    rng = SplitMix64(20260003)
+    mols = []
+    for _ in range(nmol):
+        natom = 5 + int(rng.uniform() * 56)
+        side = (20.0 * natom) ** (1.0 / 3.0)
+        pts = positions(rng, natom, side)
+        shells = []
+        for p in pts:
+            for l in (0, 0, 1, 1, 2, 3):
+                npr = 1 if l == 3 else 1 + int(rng.uniform() * 3)
+                e = _exps(rng, npr, 0.05, 60.0)
+                c = [0.1 + 0.9 * rng.uniform() for _ in range(npr)]
+                shells.append((l, tuple(float(x) for x in p), e, c, True))
+        nao = sum(2 * s[0] + 1 for s in shells)
+        mols.append((shells, density(rng, nao, 20)))
+    return mols
+
+
 def to_basis(shells):
     return Basis([Shell(l, c, e, k, ShellKind.Spherical if sph else ShellKind.Cartesian)
                   for (l, c, e, k, sph) in shells])

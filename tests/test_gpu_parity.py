@@ -187,6 +187,31 @@ def test_direct_jk_every_kernel_variant(variant):
         assert np.max(np.abs(K[x] - oK[x])) < TOL
 
 
+@pytest.mark.parametrize("variant", [None, 0, 1, 2])
+def test_direct_jk_f_classes(variant):
+    """The compiled f-shell J/K kernels (team kernels for f class pairs up to
+    3600 elements, lane kernels up to 270) against the oracle at tau = 0 on
+    non-symmetric densities, spherical and Cartesian f shells; (ff|fd) and
+    (ff|ff) run on the runtime-l kernels."""
+    shells, _ = synthetic.molecule(20260111, 3, 4.0, [(0, 2), (1, 2), (2, 1), (3, 1)],
+                                   (0.1, 30.0), True)
+    shells += [(3, (0.4, 1.1, -0.7), [0.8, 0.3], [0.6, 0.5], False),
+               (1, (0.4, 1.1, -0.7), [0.5], [1.0], True)]
+    n = B.nfunc(shells)
+    dens = B.direct_contractor_densities(n)
+    note("jk f classes variant %s nsh=%d nao=%d" % (variant, len(shells), n))
+    dc = gpu_basis(shells).direct_contractor()
+    sched = dc.schedule()
+    assert any(max(e["cls"]) == 3 for e in sched)
+    if variant is not None:
+        dc.set_schedule([dict(e, variant=variant) for e in sched])
+    J, K = dc.jk(dens, 0.0)
+    oJ, oK = O.Basis(shells).direct_jk(dens, 0.0)
+    for x in range(2):
+        assert np.max(np.abs(J[x] - oJ[x])) < TOL
+        assert np.max(np.abs(K[x] - oK[x])) < TOL
+
+
 def test_autotune_keeps_results():
     import torch
     shells = B.grad_basis(True, max_l=2) + B.shared_centers_basis()
@@ -201,7 +226,9 @@ def test_autotune_keeps_results():
     oJ, oK = O.Basis(shells).direct_jk([p], 1e-12)
     assert np.max(np.abs(J[0] - oJ[0])) < TOL and np.max(np.abs(K[0] - oK[0])) < TOL
     with pytest.raises(ig.IntegralError):
-        dc.set_schedule([{"cls": (3, 0, 0, 0), "variant": 0}])
+        dc.set_schedule([{"cls": (3, 3, 3, 3), "variant": 0}])
+    with pytest.raises(ig.IntegralError):
+        dc.set_schedule([{"cls": (4, 0, 0, 0), "variant": 0}])
 
 
 def _support_density(shells, keep, seed):
