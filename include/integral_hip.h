@@ -182,10 +182,14 @@ int ig_direct_profile_read(const ig_direct *direct, ig_class_profile *out, size_
  * J/K kernels: variant 0 runs one quartet per team of lanes with its 2D
  * tables in LDS, variant 1 one quartet per lane in registers, variant 2 the
  * register kernel with the bra row's Coulomb and exchange sums kept in LDS
- * (used for the whole-list calls; per-bra-pair calls fall back to 1), and
+ * (used for the whole-list calls; per-bra-pair calls fall back to 1),
  * variant 3, for total angular momentum <= 2, variant 2 with an Obara-Saika /
- * Head-Gordon-Pople block from the Boys function instead of Rys roots.  The
- * reference
+ * Head-Gordon-Pople block from the Boys function instead of Rys roots, and
+ * variant 4 the team kernel with only the vertical recurrence per primitive
+ * and the horizontal transfers once per contracted quartet (classes with a
+ * p or higher shell in position b or d).  Classes with f shells have the
+ * team kernels (up to 3600 block elements) and, up to 270, the lane ones.
+ * The reference
  * has no counterpart (its CPU engines pick by angular momentum only,
  * integrals.rs:323-329); this replaces a fixed dispatch on the GPU.
  *
@@ -196,7 +200,8 @@ int ig_direct_profile_read(const ig_direct *direct, ig_class_profile *out, size_
 typedef struct ig_schedule_entry {
   int32_t la, lb, lc, ld;  /* kernel orientation, pair class (la,lb) >= (lc,ld) */
   int32_t variant;         /* 0 team/LDS, 1 lane/registers, 2 registers + LDS row sums,
-                              3 as 2 with an OS/HGP block; -1 on write = default */
+                              3 as 2 with an OS/HGP block, 4 team with contracted
+                              transfers; -1 on write = default */
   int32_t tuned;           /* read: 1 if set by the autotuner or a write */
   double ms;               /* read: the autotuner's time for the chosen variant */
 } ig_schedule_entry;
@@ -236,6 +241,12 @@ int ig_batch_set_mixed_precision(ig_batch *batch, double fp32_below);
  * that need them to overlap. */
 int ig_batch_jk_device(ig_batch *batch, const double *const *d_dens, double threshold,
                        int want_exchange, double *const *d_J, double *const *d_K, void *stream);
+/* Class-to-kernel schedule for the whole batch: ig_direct_autotune on
+ * molecule `molecule` (d_dens: its nao*nao device density), then the chosen
+ * variants copied to every molecule's plan.  The next ig_batch_jk_device
+ * captures the graph again.  Synchronises `stream`. */
+int ig_batch_autotune(ig_batch *batch, size_t molecule, const double *d_dens, double threshold,
+                      int want_exchange, int reps, void *stream);
 /* Quartets accepted ([0]) and primitive quartets ([1]) of the last call,
  * summed over the molecules.  Synchronises the batch's stream. */
 int ig_batch_last_stats(ig_batch *batch, int64_t *stats2);

@@ -205,6 +205,8 @@ def library():
     lib.ig_batch_jk_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]
     lib.ig_batch_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
+    lib.ig_batch_autotune.argtypes = [_vp, sz, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                      _vp]
     lib.ig_grad_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
     lib.ig_grad_destroy.argtypes = [_vp]
     lib.ig_grad_natom.argtypes = [_vp]
@@ -495,7 +497,7 @@ class DirectContractor:
         _raise(library().ig_direct_set_profiling(self._h, 1 if enable else 0))
 
     # J/K kernel variants of a compiled quartet class
-    TEAM, REGISTER, ROW_ACCUMULATE, OS_HGP = 0, 1, 2, 3
+    TEAM, REGISTER, ROW_ACCUMULATE, OS_HGP, TEAM_HRR = 0, 1, 2, 3, 4
 
     def autotune(self, dens, threshold=0.0, want_exchange=True, reps=2, stream=None):
         """Time every J/K kernel variant of every class pair on the device
@@ -604,6 +606,23 @@ class BatchContractor:
         st = (ctypes.c_int64 * 2)()
         _raise(library().ig_batch_last_stats(self._h, st))
         return int(st[0]), int(st[1])
+
+    def autotune(self, dens, threshold=0.0, want_exchange=True, reps=1, molecule=None,
+                 stream=None):
+        """Tune the class-to-kernel schedule on one molecule (default: the
+        largest) with its density `dens[molecule]` (a list of CUDA tensors as
+        for jk_device) and use it for the whole batch."""
+        m = int(np.argmax(self._nao)) if molecule is None else int(molecule)
+        if not 0 <= m < len(self._nao):
+            raise UsageError("molecule out of range")
+        t = dens[m]
+        if not (t.is_cuda and t.element_size() == 8 and t.is_contiguous()
+                and t.numel() == self._nao[m] ** 2):
+            raise UsageError("density must be a contiguous float64 CUDA tensor of nao*nao")
+        _raise(library().ig_batch_autotune(self._h, m, _vp(t.data_ptr()), float(threshold),
+                                           1 if want_exchange else 0, int(reps),
+                                           _stream_ptr(stream)))
+        return m
 
 
 class GradDensityTerm:

@@ -156,6 +156,95 @@ __device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R
   }
 }
 
+// J/K digestion of one quartet's Cartesian block (LDS, `blk`, element
+// ((a*nb+b)*nc+c)*nd+d) by the G lanes of a team:
+// Jp[ab] += 2f g D[cd], Jp[cd] += 2f g D[ab], Kp[ac] += f g D[bd],
+// Kp[bc] += f g D[ad], Kp[ad] += f g D[bc], Kp[bd] += f g D[ac], with
+// f = 1/|stabiliser| of the shell quartet, for each effective density whose
+// screening bit is set in `mask`.
+template <class C, int G>
+__device__ __forceinline__ void team_digest(const BasisDev &b, const DensDev &dd,
+                                            const QuartetGeom &g, bool same_pair,
+                                            unsigned long long mask, const double *blk, int tl) {
+  const int ncart2 = b.nao_cart * b.nao_cart;
+  const double f = (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (same_pair ? 0.5 : 1.0);
+  const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
+            d0 = b.off_cart[g.sD];
+  const int ncrt = b.nao_cart;
+  constexpr int na = C::na, nb = C::nb, nc = C::nc, nd = C::nd;
+  constexpr int NJ = na * nb + nc * nd;
+  const int NO = NJ + (dd.want_k ? na * nc + nb * nc + na * nd + nb * nd : 0);
+  for (int e = 0; e < dd.neff; ++e) {
+    if (!((mask >> dd.eff_src[e]) & 1ull)) continue;
+    const bool anti = dd.eff_anti[e];
+    const double *D = dd.dc + (size_t)e * ncart2;
+    const size_t xo = jk_copy_offset(dd);
+    double *J = dd.jc + xo + (size_t)e * ncart2;
+    double *K = dd.kc ? dd.kc + xo + (size_t)e * ncart2 : nullptr;
+    for (int o = tl; o < NO; o += G) {
+      double v = 0.0;
+      int row, col;
+      double *dst;
+      if (o < na * nb) {  // Jp[a,b]
+        if (anti) continue;
+        const int a = o / nb, bb = o % nb;
+        for (int c = 0; c < nc; ++c)
+          for (int d = 0; d < nd; ++d)
+            v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(c0 + c) * ncrt + d0 + d];
+        v *= 2.0 * f;
+        row = a0 + a;
+        col = b0 + bb;
+        dst = J;
+      } else if (o < NJ) {  // Jp[c,d]
+        if (anti) continue;
+        const int q = o - na * nb, c = q / nd, d = q % nd;
+        for (int a = 0; a < na; ++a)
+          for (int bb = 0; bb < nb; ++bb)
+            v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + b0 + bb];
+        v *= 2.0 * f;
+        row = c0 + c;
+        col = d0 + d;
+        dst = J;
+      } else {
+        int q = o - NJ;
+        if (q < na * nc) {  // Kp[a,c] += g D[b,d]
+          const int a = q / nc, c = q % nc;
+          for (int bb = 0; bb < nb; ++bb)
+            for (int d = 0; d < nd; ++d)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * ncrt + d0 + d];
+          row = a0 + a;
+          col = c0 + c;
+        } else if ((q -= na * nc) < nb * nc) {  // Kp[b,c] += g D[a,d]
+          const int bb = q / nc, c = q % nc;
+          for (int a = 0; a < na; ++a)
+            for (int d = 0; d < nd; ++d)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + d0 + d];
+          row = b0 + bb;
+          col = c0 + c;
+        } else if ((q -= nb * nc) < na * nd) {  // Kp[a,d] += g D[b,c]
+          const int a = q / nd, d = q % nd;
+          for (int bb = 0; bb < nb; ++bb)
+            for (int c = 0; c < nc; ++c)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * ncrt + c0 + c];
+          row = a0 + a;
+          col = d0 + d;
+        } else {  // Kp[b,d] += g D[a,c]
+          q -= na * nd;
+          const int bb = q / nd, d = q % nd;
+          for (int a = 0; a < na; ++a)
+            for (int c = 0; c < nc; ++c)
+              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + c0 + c];
+          row = b0 + bb;
+          col = d0 + d;
+        }
+        v *= f;
+        dst = K;
+      }
+      jk_add(dd, dst + (size_t)row * ncrt + col, v);
+    }
+  }
+}
+
 // Compile-time element -> per-axis 2D-table offsets for the register path.
 template <class C>
 struct ElemTab {

@@ -133,6 +133,12 @@ bool class_supported(int la, int lb, int lc, int ld);
 // of IG_FOR_F_TEAM_CLASSES) and the lane kernels (plus IG_FOR_F_LANE_CLASSES)
 bool jk_team_compiled(int la, int lb, int lc, int ld);
 bool jk_lane_compiled(int la, int lb, int lc, int ld);
+// the team kernel with contracted-space horizontal transfers (ig_jk_hrr.hip):
+// team classes with lb > 0 or ld > 0
+bool jk_hrr_compiled(int la, int lb, int lc, int ld);
+hipError_t launch_jk_hrr(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
+                         const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
+                         unsigned long long *counters, hipStream_t s);
 hipError_t launch_schwarz(int lcls1, int lcls2, const BasisDev &b, const PairsDev &p,
                           const RysDev &r, int first, int count, double *qout, hipStream_t s);
 // J/K kernel variants of a compiled class: one quartet per team of lanes
@@ -140,8 +146,11 @@ hipError_t launch_schwarz(int lcls1, int lcls2, const BasisDev &b, const PairsDe
 // register path with per-row LDS accumulation of the bra-side updates
 // -- or, for total angular momentum <= 2, the row-sum kernel with an OS/HGP
 // block from the Boys function instead of Rys roots
-enum JkVariant { kJkTeam = 0, kJkReg = 1, kJkRowAcc = 2, kJkHgp = 3, kJkNumVariants = 4 };
-constexpr size_t kRowAccLdsMax = 64 * 1024;  // dynamic LDS of one row-accumulating workgroup
+enum JkVariant { kJkTeam = 0, kJkReg = 1, kJkRowAcc = 2, kJkHgp = 3, kJkTeamHrr = 4, kJkNumVariants = 5 };
+// dynamic LDS of one row-accumulating workgroup: the device's per-workgroup
+// limit (160 KB on gfx950), read once; these kernels hold one workgroup per
+// CU at most anyway (over 256 VGPRs per lane)
+size_t rowacc_lds_max();
 int jk_default_variant(int la, int lb, int lc, int ld);
 // width of the row accumulators' compact column space for ket momenta lc >= ld
 int rowacc_width(const BasisDev &b, int lc, int ld);

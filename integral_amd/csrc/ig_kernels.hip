@@ -30,7 +30,6 @@ __global__ void __launch_bounds__(64) jk_kernel(BasisDev b, PairsDev p, RysDev R
   double *work = lds + team * PER;
   int offs[C::E];
   init_offsets<C>(tl, offs);
-  const int ncart2 = b.nao_cart * b.nao_cart;
   unsigned long long my_quartets = 0, my_prims = 0;
   for (long long t = (long long)blockIdx.x * C::TEAMS + team; t < qs.total;
        t += (long long)gridDim.x * C::TEAMS) {
@@ -68,82 +67,7 @@ __global__ void __launch_bounds__(64) jk_kernel(BasisDev b, PairsDev p, RysDev R
     for (int k = 0; k < C::E; ++k)
       if (tl + G * k < C::N4) blk[tl + G * k] = acc[k];
     team_sync();
-    const double f = (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (pb == pk ? 0.5 : 1.0);
-    const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
-              d0 = b.off_cart[g.sD];
-    const int ncrt = b.nao_cart;
-    constexpr int na = C::na, nb = C::nb, nc = C::nc, nd = C::nd;
-    constexpr int NJ = na * nb + nc * nd;
-    const int NO = NJ + (dd.want_k ? na * nc + nb * nc + na * nd + nb * nd : 0);
-    for (int e = 0; e < dd.neff; ++e) {
-      if (!((mask >> dd.eff_src[e]) & 1ull)) continue;
-      const bool anti = dd.eff_anti[e];
-      const double *D = dd.dc + (size_t)e * ncart2;
-      const size_t xo = jk_copy_offset(dd);
-      double *J = dd.jc + xo + (size_t)e * ncart2;
-      double *K = dd.kc ? dd.kc + xo + (size_t)e * ncart2 : nullptr;
-      for (int o = tl; o < NO; o += G) {
-        double v = 0.0;
-        int row, col;
-        double *dst;
-        if (o < na * nb) {  // Jp[a,b]
-          if (anti) continue;
-          const int a = o / nb, bb = o % nb;
-          for (int c = 0; c < nc; ++c)
-            for (int d = 0; d < nd; ++d)
-              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(c0 + c) * ncrt + d0 + d];
-          v *= 2.0 * f;
-          row = a0 + a;
-          col = b0 + bb;
-          dst = J;
-        } else if (o < NJ) {  // Jp[c,d]
-          if (anti) continue;
-          const int q = o - na * nb, c = q / nd, d = q % nd;
-          for (int a = 0; a < na; ++a)
-            for (int bb = 0; bb < nb; ++bb)
-              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + b0 + bb];
-          v *= 2.0 * f;
-          row = c0 + c;
-          col = d0 + d;
-          dst = J;
-        } else {
-          int q = o - NJ;
-          if (q < na * nc) {  // Kp[a,c] += g D[b,d]
-            const int a = q / nc, c = q % nc;
-            for (int bb = 0; bb < nb; ++bb)
-              for (int d = 0; d < nd; ++d)
-                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * ncrt + d0 + d];
-            row = a0 + a;
-            col = c0 + c;
-          } else if ((q -= na * nc) < nb * nc) {  // Kp[b,c] += g D[a,d]
-            const int bb = q / nc, c = q % nc;
-            for (int a = 0; a < na; ++a)
-              for (int d = 0; d < nd; ++d)
-                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + d0 + d];
-            row = b0 + bb;
-            col = c0 + c;
-          } else if ((q -= nb * nc) < na * nd) {  // Kp[a,d] += g D[b,c]
-            const int a = q / nd, d = q % nd;
-            for (int bb = 0; bb < nb; ++bb)
-              for (int c = 0; c < nc; ++c)
-                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(b0 + bb) * ncrt + c0 + c];
-            row = a0 + a;
-            col = d0 + d;
-          } else {  // Kp[b,d] += g D[a,c]
-            q -= na * nd;
-            const int bb = q / nd, d = q % nd;
-            for (int a = 0; a < na; ++a)
-              for (int c = 0; c < nc; ++c)
-                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(a0 + a) * ncrt + c0 + c];
-            row = b0 + bb;
-            col = d0 + d;
-          }
-          v *= f;
-          dst = K;
-        }
-        jk_add(dd, dst + (size_t)row * ncrt + col, v);
-      }
-    }
+    team_digest<C, G>(b, dd, g, pb == pk, mask, blk, tl);
     team_sync();
   }
   if (counters && tl == 0) {
@@ -382,14 +306,19 @@ int jk_default_variant(int la, int lb, int lc, int ld) {
   // the autotuner's usual choice on s,p,d bases: register kernels with row
   // sums while a quartet needs at most two element slices (each slice
   // rebuilds the quartet's 2D tables), team kernels for larger blocks
+  // (measured on s,p,d,f molecules: the contracted-space transfers win on
+  // the largest blocks, (fd|fd), (ff|dd), (fd|dd))
   const int n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-  return n4 <= 2 * kRegBlockMax && jk_lane_compiled(la, lb, lc, ld) ? kJkRowAcc : kJkTeam;
+  if (n4 <= 2 * kRegBlockMax && jk_lane_compiled(la, lb, lc, ld)) return kJkRowAcc;
+  return n4 >= 2000 && jk_hrr_compiled(la, lb, lc, ld) ? kJkTeamHrr : kJkTeam;
 }
 
 hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                      const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
                      int variant, unsigned long long *ctr, hipStream_t s) {
-  if (variant != kJkTeam)  // lane-per-quartet kernels (ig_jk_lane.hip)
+  if (variant == kJkTeamHrr)  // contracted-space transfers (ig_jk_hrr.hip)
+    return launch_jk_hrr(la, lb, lc, ld, b, p, r, qs, d, omega, ctr, s);
+  if (variant != kJkTeam)  // lane-per-quartet kernels (ig_jk_lane*.hip)
     return launch_jk_lane(la, lb, lc, ld, b, p, r, qs, d, omega, variant, ctr, s);
 #define IG_CASE(A, B, Cc, D)                       \
   if (la == A && lb == B && lc == Cc && ld == D) \

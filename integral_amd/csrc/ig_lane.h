@@ -21,6 +21,19 @@ int rowacc_width(const BasisDev &b, int lc, int ld) {
   return nC * ncart(lc) + (lc == ld ? 0 : nD * ncart(ld));
 }
 
+size_t rowacc_lds_max() {
+  static const size_t v = [] {
+    int dev = 0, bytes = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess ||
+        bytes < 64 * 1024)
+      return size_t(64 * 1024);
+    return size_t(bytes);
+  }();
+  return v;
+}
+
 size_t rowacc_acc_doubles(const BasisDev &b, int la, int lb, int lc, int ld, int neff) {
   return rowacc_doubles(ncart(la), ncart(lb), neff, rowacc_width(b, lc, ld));
 }
@@ -28,6 +41,7 @@ size_t rowacc_acc_doubles(const BasisDev &b, int la, int lb, int lc, int ld, int
 bool jk_variant_fits(int variant, int la, int lb, int lc, int ld, const BasisDev &b, int neff,
                      bool listed) {
   if (variant == kJkTeam) return jk_team_compiled(la, lb, lc, ld);
+  if (variant == kJkTeamHrr) return jk_hrr_compiled(la, lb, lc, ld);
   if (!jk_lane_compiled(la, lb, lc, ld)) return false;
   if (variant == kJkReg) return true;
   if (listed) return false;
@@ -37,7 +51,7 @@ bool jk_variant_fits(int variant, int la, int lb, int lc, int ld, const BasisDev
       variant == kJkHgp ? 0 : size_t(2) * rys_host_tables().nsub[nr] * nr * kRysTerms * sizeof(double);
   const size_t lds =
       rys + rowacc_doubles(ncart(la), ncart(lb), neff, rowacc_width(b, lc, ld)) * sizeof(double);
-  return lds <= kRowAccLdsMax;
+  return lds <= rowacc_lds_max();
 }
 #endif
 
@@ -53,7 +67,7 @@ hipError_t jk_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r, co
   if (variant == kJkHgp) {
     if constexpr (HgpClass<C>::ok) {
       const size_t acc = rowacc_doubles(C::na, C::nb, d.neff, rowacc_width(b, LC, LD));
-      if (qs.list || acc * sizeof(double) > kRowAccLdsMax) return hipErrorInvalidValue;
+      if (qs.list || acc * sizeof(double) > rowacc_lds_max()) return hipErrorInvalidValue;
       long long blocks = (qs.total + 64 * kRowWaves - 1) / (64 * kRowWaves);
       if (blocks > 1024) blocks = 1024;
       return launch_jk_rowacc_slices<C, 0, F, 1>(b, p, r, qs, d, omega, ctr, s, int(blocks),
@@ -64,7 +78,7 @@ hipError_t jk_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r, co
   }
   if (variant == kJkRowAcc) {
     const size_t acc = rowacc_doubles(C::na, C::nb, d.neff, rowacc_width(b, LC, LD));
-    if (qs.list || lds + acc * sizeof(double) > kRowAccLdsMax) return hipErrorInvalidValue;
+    if (qs.list || lds + acc * sizeof(double) > rowacc_lds_max()) return hipErrorInvalidValue;
     long long blocks = (qs.total + 64 * kRowWaves - 1) / (64 * kRowWaves);
     if (blocks > 1024) blocks = 1024;
     return launch_jk_rowacc_slices<C, 0, F>(b, p, r, qs, d, omega, ctr, s, int(blocks),

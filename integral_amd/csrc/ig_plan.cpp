@@ -794,10 +794,11 @@ hipError_t jk_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const Qu
   if (jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) {
     const bool lane = jk_lane_compiled(X.l1, X.l2, Y.l1, Y.l2);
     int v = variant >= 0 ? variant : jk_variant(P, X, Y);
-    if (fp32 && v == kJkTeam && lane) v = kJkReg;
+    const bool team = v == kJkTeam || v == kJkTeamHrr;
+    if (fp32 && team && lane) v = kJkReg;
     if (!jk_variant_fits(v, X.l1, X.l2, Y.l1, Y.l2, P.bdev, dd.neff, qs.list != nullptr))
       v = lane ? kJkReg : kJkTeam;
-    if (fp32 && v != kJkTeam)
+    if (fp32 && v != kJkTeam && v != kJkTeamHrr)
       return launch_jk_lane_f32(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, v,
                                 ctr, s);
     return launch_jk(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, v, ctr, s);
@@ -1093,6 +1094,68 @@ extern "C" int ig_direct_jk_device(const ig_direct *d, const double *d_dens, siz
   });
 }
 
+namespace {
+
+// Time every J/K kernel variant of every compiled class pair of plan P on
+// the device densities and keep the fastest (caller holds P.mu).
+void autotune_plan(Plan &P, const double *d_dens, size_t ndens, double threshold, int want_k,
+                   int reps, hipStream_t s) {
+  finalize_stats(P);
+  P.s_flag.reserve(sizeof(int));
+  check(launch_asym_flag(d_dens, P.nao, int(ndens), P.s_flag.as<int>(), s), "symmetry check");
+  int asym = 0;
+  check(hipMemcpyAsync(&asym, P.s_flag.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flag");
+  check(hipStreamSynchronize(s), "sync");
+  // the accumulators of the trial launches are scratch: the caller's J/K
+  // are not touched
+  const DensDev dd = prep_densities(P, d_dens, int(ndens), asym != 0, threshold, want_k != 0, s);
+  build_rowptrs(P, threshold);
+  P.s_counters.reserve(2 * sizeof(unsigned long long));
+  auto *ctr = P.s_counters.as<unsigned long long>();
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "hipEventCreate");
+  check(hipEventCreate(&e1), "hipEventCreate");
+  try {
+    for (auto &cp : P.cached_cps) {
+      const ClassRange &X = P.cls[cp.x], &Y = P.cls[cp.y];
+      if (!cp.total || !jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) continue;
+      const QuartetSet qs = implicit_set(P, cp);
+      int best = -1;
+      float best_ms = 0.0f;
+      for (int v = 0; v < kJkNumVariants; ++v) {
+        if (!jk_variant_fits(v, X.l1, X.l2, Y.l1, Y.l2, P.bdev, dd.neff, false)) continue;
+        float vmin = -1.0f;
+        for (int r = 0; r < reps; ++r) {
+          check(hipEventRecord(e0, s), "event");
+          check(jk_launch(P, X, Y, qs, dd, ctr, s, v), "jk launch");
+          check(hipEventRecord(e1, s), "event");
+          check(hipEventSynchronize(e1), "event sync");
+          float ms = 0.0f;
+          check(hipEventElapsedTime(&ms, e0, e1), "event time");
+          if (vmin < 0.0f || ms < vmin) vmin = ms;
+        }
+        if (best < 0 || vmin < best_ms) {
+          best = v;
+          best_ms = vmin;
+        }
+      }
+      const int key = cp.x * kNumPairClass + cp.y;
+      P.variant[key] = int8_t(best);
+      P.variant_ms[key] = best_ms;
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  P.launches.clear();
+  P.stats_pending = false;
+}
+
+}  // namespace
+
 extern "C" int ig_direct_autotune(ig_direct *d, const double *d_dens, size_t ndens,
                                   double threshold, int want_k, int reps, void *stream) {
   if (!d || !d_dens) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
@@ -1104,58 +1167,7 @@ extern "C" int ig_direct_autotune(ig_direct *d, const double *d_dens, size_t nde
     Plan &P = *d->plan;
     std::lock_guard<std::mutex> g(P.mu);
     hipStream_t s = as_stream(stream);
-    finalize_stats(P);
-    P.s_flag.reserve(sizeof(int));
-    check(launch_asym_flag(d_dens, P.nao, int(ndens), P.s_flag.as<int>(), s), "symmetry check");
-    int asym = 0;
-    check(hipMemcpyAsync(&asym, P.s_flag.p, sizeof(int), hipMemcpyDeviceToHost, s), "copy flag");
-    check(hipStreamSynchronize(s), "sync");
-    // the accumulators of the trial launches are scratch: the caller's J/K
-    // are not touched
-    const DensDev dd = prep_densities(P, d_dens, int(ndens), asym != 0, threshold, want_k != 0, s);
-    build_rowptrs(P, threshold);
-    P.s_counters.reserve(2 * sizeof(unsigned long long));
-    auto *ctr = P.s_counters.as<unsigned long long>();
-    hipEvent_t e0, e1;
-    check(hipEventCreate(&e0), "hipEventCreate");
-    check(hipEventCreate(&e1), "hipEventCreate");
-    try {
-      for (auto &cp : P.cached_cps) {
-        const ClassRange &X = P.cls[cp.x], &Y = P.cls[cp.y];
-        if (!cp.total || !jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) continue;
-        const QuartetSet qs = implicit_set(P, cp);
-        int best = -1;
-        float best_ms = 0.0f;
-        for (int v = 0; v < kJkNumVariants; ++v) {
-          if (!jk_variant_fits(v, X.l1, X.l2, Y.l1, Y.l2, P.bdev, dd.neff, false)) continue;
-          float vmin = -1.0f;
-          for (int r = 0; r < reps; ++r) {
-            check(hipEventRecord(e0, s), "event");
-            check(jk_launch(P, X, Y, qs, dd, ctr, s, v), "jk launch");
-            check(hipEventRecord(e1, s), "event");
-            check(hipEventSynchronize(e1), "event sync");
-            float ms = 0.0f;
-            check(hipEventElapsedTime(&ms, e0, e1), "event time");
-            if (vmin < 0.0f || ms < vmin) vmin = ms;
-          }
-          if (best < 0 || vmin < best_ms) {
-            best = v;
-            best_ms = vmin;
-          }
-        }
-        const int key = cp.x * kNumPairClass + cp.y;
-        P.variant[key] = int8_t(best);
-        P.variant_ms[key] = best_ms;
-      }
-    } catch (...) {
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      throw;
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    P.launches.clear();
-    P.stats_pending = false;
+    autotune_plan(P, d_dens, ndens, threshold, want_k, reps, s);
     return int(IG_OK);
   });
 }
@@ -1600,8 +1612,8 @@ void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
     const ClassRange &X0 = P0.cls[k.x], &Y0 = P0.cls[k.y];
     Launch L{k, false, -1, 0, 0, sets.size(), seg.size()};
     if (jk_lane_compiled(X0.l1, X0.l2, Y0.l1, Y0.l2)) {
-      int v = jk_default_variant(X0.l1, X0.l2, Y0.l1, Y0.l2);
-      if (v == kJkTeam && k.fp32) v = kJkRowAcc;
+      int v = jk_variant(P0, X0, Y0);
+      if ((v == kJkTeam || v == kJkTeamHrr) && k.fp32) v = kJkRowAcc;
       if (v == kJkRowAcc || v == kJkHgp) {
         bool fits = true;
         for (auto &mc : kv.second) {
@@ -1760,6 +1772,27 @@ extern "C" int ig_batch_jk_device(ig_batch *b, const double *const *d_dens, doub
     check(hipEventRecord(b->ev_out, b->stream), "event");
     check(hipStreamWaitEvent(s, b->ev_out, 0), "wait");
     b->stats_pending = true;
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_batch_autotune(ig_batch *b, size_t molecule, const double *d_dens,
+                                 double threshold, int want_k, int reps, void *stream) {
+  if (!b || !d_dens) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (molecule >= b->plans.size()) return fail(IG_ERR_INVALID_ARGUMENT, "molecule out of range");
+  if (reps < 1) return fail(IG_ERR_INVALID_ARGUMENT, "reps must be at least 1");
+  return guarded([&] {
+    std::lock_guard<std::mutex> g(b->mu);
+    check(hipStreamSynchronize(b->stream), "sync");
+    Plan &P = *b->plans[molecule];
+    autotune_plan(P, d_dens, 1, threshold, want_k, reps, as_stream(stream));
+    for (auto &Q : b->plans) {
+      if (Q.get() == &P) continue;
+      Q->variant = P.variant;
+      Q->variant_ms = P.variant_ms;
+    }
+    // the trial launches moved plan buffers: capture again on the next call
+    b->tau = NAN;
     return int(IG_OK);
   });
 }

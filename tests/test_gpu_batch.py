@@ -160,3 +160,26 @@ def test_batch_on_a_side_stream_and_bad_arguments():
         ig.BatchContractor([])
     with pytest.raises(ig.IntegralError):
         bc.set_mixed_precision(-1.0)
+
+
+def test_batch_autotune_keeps_results():
+    mols = batch_molecules()
+    bases = [synthetic.to_basis(s) for s, _ in mols]
+    bc = ig.BatchContractor(bases)
+    dens = [dev(D) for _, D in mols]
+    note("batch autotune")
+    m = bc.autotune(dens, threshold=1e-12)
+    assert m == int(np.argmax([D.shape[0] for _, D in mols]))
+    ref = per_molecule(mols, 1e-12)
+    J = [torch.zeros_like(d) for d in dens]
+    K = [torch.zeros_like(d) for d in dens]
+    for rep in range(2):
+        for t in J + K:
+            t.zero_()
+        bc.jk_device(dens, J, K, threshold=1e-12)
+        torch.cuda.synchronize()
+        for i, (rj, rk, _) in enumerate(ref):
+            assert np.max(np.abs(J[i].cpu().numpy() - rj)) < TOL
+            assert np.max(np.abs(K[i].cpu().numpy() - rk)) < TOL
+    with pytest.raises(ig.UsageError):
+        bc.autotune(dens, molecule=len(mols))

@@ -164,12 +164,13 @@ def test_grad_bra_pairs_sum_in_any_order():
     assert np.max(np.abs(parts[0] + parts[1] - serial)) < 1e-12
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_direct_jk_every_kernel_variant(variant):
     """Every J/K kernel of every compiled class (team/LDS, lane/register,
-    register with per-row LDS sums, and the OS/HGP block for total l <= 2)
-    against the oracle, forced through a loaded schedule (classes a variant
-    cannot run take the register kernel)."""
+    register with per-row LDS sums, the OS/HGP block for total l <= 2, and
+    the team kernel with contracted-space transfers) against the oracle,
+    forced through a loaded schedule (classes a variant cannot run take the
+    register kernel)."""
     shells = (B.grad_basis(True, max_l=2) + B.fd_basis()
               + [(2, (1.3, -0.8, 0.9), [0.45], [1.0], True)])
     n = B.nfunc(shells)
@@ -187,7 +188,7 @@ def test_direct_jk_every_kernel_variant(variant):
         assert np.max(np.abs(K[x] - oK[x])) < TOL
 
 
-@pytest.mark.parametrize("variant", [None, 0, 1, 2])
+@pytest.mark.parametrize("variant", [None, 0, 1, 2, 4])
 def test_direct_jk_f_classes(variant):
     """The compiled f-shell J/K kernels (team kernels for f class pairs up to
     3600 elements, lane kernels up to 270) against the oracle at tau = 0 on
@@ -221,7 +222,7 @@ def test_autotune_keeps_results():
     dc = gpu_basis(shells).direct_contractor()
     D = torch.tensor(p[None], dtype=torch.float64, device="cuda:0").contiguous()
     sched = dc.autotune(D, threshold=1e-12, reps=1)
-    assert sched and all(e["tuned"] and e["variant"] in (0, 1, 2, 3) for e in sched)
+    assert sched and all(e["tuned"] and e["variant"] in (0, 1, 2, 3, 4) for e in sched)
     J, K = dc.jk([p], 1e-12)
     oJ, oK = O.Basis(shells).direct_jk([p], 1e-12)
     assert np.max(np.abs(J[0] - oJ[0])) < TOL and np.max(np.abs(K[0] - oK[0])) < TOL
@@ -248,7 +249,7 @@ def _support_density(shells, keep, seed):
     return p * np.outer(rows, rows)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_direct_jk_density_screened_lanes(variant):
     """Quartets dropped by the density mask next to kept ones in the same
     wavefront, across bra rows (regression: the wavefront-reduced bra J sum
@@ -267,7 +268,7 @@ def test_direct_jk_density_screened_lanes(variant):
     assert np.max(np.abs(K[0] - oK[0])) < TOL
 
 
-@pytest.mark.parametrize("variant", [None, 1, 2, 3])
+@pytest.mark.parametrize("variant", [None, 0, 1, 2, 3, 4])
 def test_direct_jk_twelve_atom_molecule(variant):
     """The benchmark's molecule recipe at 12 atoms (spherical s,s,p,p,d,
     3/3/3/3/1 primitives, nao = 156), two densities, tau = 1e-12: every
@@ -318,3 +319,23 @@ def test_mixed_precision(variant):
     dc.set_mixed_precision(0.0)
     J2, K2 = dc.jk(dens, 1e-12)
     assert np.max(np.abs(J2[0] - oJ[0])) < TOL and np.max(np.abs(K2[0] - oK[0])) < TOL
+
+
+def test_row_sums_beyond_64k_lds():
+    """The row-sum kernel with LDS accumulators over 64 KB (the 40-atom,
+    two-density J/K benchmark molecule: (dp|dp) rows need about 69 KB) against
+    the team kernel, which the oracle pins on the smaller molecules above."""
+    import torch
+    shells, dens, tau = synthetic.direct_jk_config()
+    note("row sums with > 64 KB LDS, nao 520")
+    dc = gpu_basis(shells).direct_contractor()
+    D = torch.tensor(np.stack(dens), dtype=torch.float64, device="cuda:0").contiguous()
+    out = {}
+    for v in (0, 2):
+        dc.set_schedule([dict(e, variant=v) for e in dc.schedule()])
+        J, K = torch.zeros_like(D), torch.zeros_like(D)
+        dc.jk_device(D, J, K, threshold=tau)
+        torch.cuda.synchronize()
+        out[v] = (J.cpu().numpy(), K.cpu().numpy())
+    assert np.max(np.abs(out[0][0] - out[2][0])) < TOL
+    assert np.max(np.abs(out[0][1] - out[2][1])) < TOL
