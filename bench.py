@@ -515,6 +515,18 @@ def main():
         "all_classes_achieved": round(total_flops / (kern_ms * 1e-3) / 1e12, 4),
         "all_classes_frac": round(total_flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5),
     }
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # FETCH_SIZE / WRITE_SIZE readings of the same kernel (profiles/)
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            tr = json.load(f)
+        rec = tr["kernels"].get(roofline["kernel"])
+        if rec:
+            roofline["traffic"] = 1024.0 * (tr["fetch_scale"] * sum(rec["fetch_kib"])
+                                            + tr["write_scale"] * sum(rec["write_kib"]))
+            roofline["traffic_unit"] = "bytes per class launch (measured: profiles/hbm_traffic.json)"
+    except (OSError, ValueError, KeyError):
+        pass
     cpu = None
     log("profiled classes; dominant %s %.3f ms" % (dom["cls"], dom["ms"]))
     if not args.no_cpu_baseline:

@@ -186,8 +186,8 @@ int ig_direct_profile_read(const ig_direct *direct, ig_class_profile *out, size_
  * variant 3, for total angular momentum <= 2, variant 2 with an Obara-Saika /
  * Head-Gordon-Pople block from the Boys function instead of Rys roots, and
  * variant 4 the team kernel with only the vertical recurrence per primitive
- * and the horizontal transfers once per contracted quartet (classes with a
- * p or higher shell in position b or d).  Classes with f shells have the
+ * and the horizontal transfers once per contracted quartet (classes of more
+ * than 162 block elements with a p or higher shell in position b or d).  Classes with f shells have the
  * team kernels (up to 3600 block elements) and, up to 270, the lane ones.
  * The reference
  * has no counterpart (its CPU engines pick by angular momentum only,

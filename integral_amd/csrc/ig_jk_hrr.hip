@@ -351,8 +351,11 @@ hipError_t jk_hrr_one(const BasisDev &b, const PairsDev &p, const RysDev &r, con
 
 }  // namespace
 
+// offered for the classes the team kernel is meant for (blocks over two
+// register slices); on smaller ones one lane per quartet is faster
 bool jk_hrr_compiled(int la, int lb, int lc, int ld) {
-  return jk_team_compiled(la, lb, lc, ld) && (lb > 0 || ld > 0);
+  return jk_team_compiled(la, lb, lc, ld) && (lb > 0 || ld > 0) &&
+         ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > 2 * kRegBlockMax;
 }
 
 hipError_t launch_jk_hrr(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
