@@ -225,10 +225,12 @@ def bench_highl(args):
 def bench_batch(args):
     """Batched J/K: nmol molecules of 5-60 atoms (1 atom per 20 bohr^3,
     spherical s,s,p,p,d,f per atom, mean nao ~650), one density each,
-    tau 1e-12, all molecules' J and K in one captured HIP graph (per quartet
-    class one persistent launch over a device work queue).  Times the
-    batched graph in FP64 and with FP32 quartets below --mixed-cut, and the
-    per-molecule FP64 loop over the same plans' device path."""
+    tau 1e-12.  The molecules go in groups of --group, each group's J and K
+    one captured HIP graph (per quartet class one persistent launch over a
+    device work queue); a step runs every group's graph, one after another,
+    and the line after each group is the run's progress.  Times the batched
+    graphs in FP64 and with FP32 quartets below --mixed-cut, and the
+    per-molecule FP64 loop over the same device path."""
     import torch
     import integral_amd as ig
     from integral_amd import synthetic
@@ -236,52 +238,62 @@ def bench_batch(args):
     mols = synthetic.batch_config(64)[:args.nmol]
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
+    gsz = max(1, args.group)
+    groups = [list(range(i, min(i + gsz, len(mols)))) for i in range(0, len(mols), gsz)]
     t0 = time.perf_counter()
     bases = [synthetic.to_basis(s) for s, _ in mols]
-    bc = ig.BatchContractor(bases)
+    bcs = [ig.BatchContractor([bases[m] for m in g]) for g in groups]
     plan_s = time.perf_counter() - t0
     naos = [b.nao() for b in bases]
-    log("batch of %d molecules (nao %d..%d, mean %.0f), plans %.2f s"
-        % (len(mols), min(naos), max(naos), sum(naos) / len(naos), plan_s))
+    log("batch of %d molecules in %d graph(s) (nao %d..%d, mean %.0f), plans %.2f s"
+        % (len(mols), len(groups), min(naos), max(naos), sum(naos) / len(naos), plan_s))
     dens = [torch.tensor(D, dtype=torch.float64, device=dev).contiguous() for _, D in mols]
     J = [torch.zeros_like(d) for d in dens]
     K = [torch.zeros_like(d) for d in dens]
-
-    def run(steps):
-        for _ in range(steps):
-            for t in J + K:
-                t.zero_()
-            bc.jk_device(dens, J, K, threshold=tau, stream=stream)
-        torch.cuda.synchronize(dev)
-
+    dcs = [b.direct_contractor() for b in bases]  # the per-molecule path
+    dens3 = [d.reshape(1, *d.shape) for d in dens]
     tune_s = 0.0
     if not args.no_autotune:
+        # one schedule for every molecule and graph, tuned on the largest of
+        # the first 8 molecules (a 960-function one), so the tuning time
+        # does not grow with the batch
         t0 = time.perf_counter()
-        m = bc.autotune(dens, threshold=tau, reps=1, stream=stream)
-        tune_s = time.perf_counter() - t0
-        log("schedule tuned on molecule %d (nao %d) in %.2f s" % (m, naos[m], tune_s))
-    t0 = time.perf_counter()
-    run(1)  # captures the graph
-    capture_s = time.perf_counter() - t0
-    run(max(0, args.warmup - 1))
-    t0 = time.perf_counter()
-    run(args.steps)
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    quartets, prims = bc.last_stats()
-    log("batched FP64: %.1f ms/build, %d quartets (first call with capture %.2f s)"
-        % (ms, quartets, capture_s))
-    J64 = [j.clone() for j in J]
-    K64 = [k.clone() for k in K]
-    # the per-molecule device path, one contractor per molecule, with the
-    # schedule tuned the same way (on the largest molecule)
-    dcs = [b.direct_contractor() for b in bases]
-    dens3 = [d.reshape(1, *d.shape) for d in dens]
-    if not args.no_autotune:
-        big = int(np.argmax(naos))
+        big = int(np.argmax(naos[:8]))
         sched = dcs[big].autotune(dens3[big], threshold=tau, reps=1, stream=stream)
         for m, dc in enumerate(dcs):
             if m != big:
                 dc.set_schedule(sched)
+        for bc in bcs:
+            bc.set_schedule(sched)
+        tune_s = time.perf_counter() - t0
+        log("schedule tuned on molecule %d (nao %d) in %.2f s" % (big, naos[big], tune_s))
+
+    def run(tag):
+        for gi, (g, bc) in enumerate(zip(groups, bcs)):
+            for m in g:
+                J[m].zero_()
+                K[m].zero_()
+            bc.jk_device([dens[m] for m in g], [J[m] for m in g], [K[m] for m in g],
+                         threshold=tau, stream=stream)
+            torch.cuda.synchronize(dev)
+            if len(groups) > 1:
+                log("%s: graph %d of %d done" % (tag, gi + 1, len(groups)))
+
+    t0 = time.perf_counter()
+    run("first run (capture)")
+    capture_s = time.perf_counter() - t0
+    for _ in range(max(0, args.warmup - 1)):
+        run("warm-up")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        run("timed step %d" % (k + 1))
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    quartets = sum(bc.last_stats()[0] for bc in bcs)
+    prims = sum(bc.last_stats()[1] for bc in bcs)
+    log("batched FP64: %.1f ms/build, %d quartets (first run with capture %.2f s)"
+        % (ms, quartets, capture_s))
+    J64 = [j.clone() for j in J]
+    K64 = [k.clone() for k in K]
     Jm = [torch.zeros_like(d) for d in dens3]
     Km = [torch.zeros_like(d) for d in dens3]
 
@@ -290,6 +302,9 @@ def bench_batch(args):
             Jm[m].zero_()
             Km[m].zero_()
             dc.jk_device(dens3[m], Jm[m], Km[m], threshold=tau, stream=stream)
+            if (m + 1) % gsz == 0 and m + 1 < len(dcs):
+                torch.cuda.synchronize(dev)
+                log("per-molecule loop: %d of %d done" % (m + 1, len(dcs)))
         torch.cuda.synchronize(dev)
 
     if len(mols) <= 16:  # plans' first-call host work out of the timing
@@ -301,14 +316,17 @@ def bench_batch(args):
     log("per-molecule FP64 loop: %.1f ms/build (max |dJ| vs batched %.1e)" % (loop_ms, dloop))
     mixed = None
     if args.mixed_cut > 0:
-        bc.set_mixed_precision(args.mixed_cut)
-        run(1)
+        for bc in bcs:
+            bc.set_mixed_precision(args.mixed_cut)
+        run("mixed first run")
         t0 = time.perf_counter()
-        run(args.steps)
+        for k in range(args.steps):
+            run("mixed timed step %d" % (k + 1))
         mms = (time.perf_counter() - t0) * 1e3 / args.steps
         dJ = max(float((J[m] - J64[m]).abs().max().item()) for m in range(len(mols)))
         dK = max(float((K[m] - K64[m]).abs().max().item()) for m in range(len(mols)))
-        bc.set_mixed_precision(0.0)
+        for bc in bcs:
+            bc.set_mixed_precision(0.0)
         mixed = {"fp32_below": args.mixed_cut, "ms_per_step": mms, "speedup_vs_fp64": ms / mms,
                  "max_abs_dJ": dJ, "max_abs_dK": dK}
         log("batched mixed (FP32 below %.0e): %.1f ms/build, max |dJ| %.1e, |dK| %.1e"
@@ -319,13 +337,15 @@ def bench_batch(args):
            "data": "synthetic (seeded molecules and densities, no checkpoint)",
            "config": {"workload": "batched J/K, %d molecules of 5-60 atoms (1 atom per 20 "
                                   "bohr^3), spherical s,s,p,p,d,f, nao %d..%d, 1 density each, "
-                                  "tau=1e-12, one HIP graph" % (len(mols), min(naos), max(naos)),
-                      "nmol": len(mols), "nao_total": sum(naos), "quartets": quartets,
-                      "prim_quartets": prims, "plan_build_s": plan_s,
-                      "first_call_with_capture_s": capture_s,
-                      "schedule": "autotuned on the largest molecule" if not args.no_autotune
-                                  else "built-in", "autotune_s": tune_s,
-                      "per_molecule_loop_ms": loop_ms, "parallelism": "1 GPU"},
+                                  "tau=1e-12, %d HIP graph(s) of up to %d molecules"
+                                  % (len(mols), min(naos), max(naos), len(groups), gsz),
+                      "nmol": len(mols), "graphs": len(groups), "nao_total": sum(naos),
+                      "quartets": quartets, "prim_quartets": prims, "plan_build_s": plan_s,
+                      "first_run_with_capture_s": capture_s,
+                      "schedule": "autotuned on the largest of the first 8 molecules"
+                                  if not args.no_autotune else "built-in",
+                      "autotune_s": tune_s, "per_molecule_loop_ms": loop_ms,
+                      "max_abs_dJ_loop_vs_batched": dloop, "parallelism": "1 GPU"},
            "mixed": mixed}
     print(json.dumps(out))
 
@@ -392,6 +412,8 @@ def main():
                          "(0: skip)")
     ap.add_argument("--nmol", type=int, default=64,
                     help="batch workload: the first NMOL of the 64 seeded molecules")
+    ap.add_argument("--group", type=int, default=16,
+                    help="batch workload: molecules per captured graph")
     ap.add_argument("--workload", choices=["jk", "grad", "highl", "eri", "batch"], default="jk",
                     help="jk: the direct J/K build (default); grad: seconds per geometry of "
                          "the J/K energy and its adjoint forces; highl: J/K with shells up to "

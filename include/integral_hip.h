@@ -247,6 +247,10 @@ int ig_batch_jk_device(ig_batch *batch, const double *const *d_dens, double thre
  * captures the graph again.  Synchronises `stream`. */
 int ig_batch_autotune(ig_batch *batch, size_t molecule, const double *d_dens, double threshold,
                       int want_exchange, int reps, void *stream);
+/* Load a schedule (as read from an ig_direct plan, e.g. one autotuned on a
+ * molecule like the batch's) into every molecule's plan; the next call
+ * captures the graph again. */
+int ig_batch_schedule_write(ig_batch *batch, const ig_schedule_entry *in, size_t n);
 /* Quartets accepted ([0]) and primitive quartets ([1]) of the last call,
  * summed over the molecules.  Synchronises the batch's stream. */
 int ig_batch_last_stats(ig_batch *batch, int64_t *stats2);

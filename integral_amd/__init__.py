@@ -207,6 +207,7 @@ def library():
     lib.ig_batch_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
     lib.ig_batch_autotune.argtypes = [_vp, sz, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       _vp]
+    lib.ig_batch_schedule_write.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz]
     lib.ig_grad_create.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp)]
     lib.ig_grad_destroy.argtypes = [_vp]
     lib.ig_grad_natom.argtypes = [_vp]
@@ -606,6 +607,16 @@ class BatchContractor:
         st = (ctypes.c_int64 * 2)()
         _raise(library().ig_batch_last_stats(self._h, st))
         return int(st[0]), int(st[1])
+
+    def set_schedule(self, entries):
+        """Load entries as returned by DirectContractor.schedule() into every
+        molecule's plan."""
+        buf = (_ScheduleEntry * max(len(entries), 1))()
+        for b, e in zip(buf, entries):
+            b.la, b.lb, b.lc, b.ld = e["cls"]
+            b.variant = int(e["variant"])
+            b.ms = float(e.get("ms", 0.0))
+        _raise(library().ig_batch_schedule_write(self._h, buf, len(entries)))
 
     def autotune(self, dens, threshold=0.0, want_exchange=True, reps=1, molecule=None,
                  stream=None):

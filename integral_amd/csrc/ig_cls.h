@@ -134,20 +134,21 @@ __device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R
         const double pqa = pick(PQ);
         const double c00 = (pick(P) - pick(g.A)) - fq * pqa;
         const double cp0 = (pick(Q) - pick(g.C)) + fp * pqa;
+        // the z table carries the weight and prefactor
         rys_axis_table<C::LA, C::LB, C::LC, C::LD>(c00, cp0, b10, b01, b00, pick(AB), pick(CD),
-                                                   tab + task * T);
+                                                   tab + task * T,
+                                                   ax == 2 ? pref * rw[NR + r] : 1.0);
       }
       team_sync();
       // phase 3: assembly, one root at a time so only 3E table reads are live
 #pragma unroll 1
       for (int r = 0; r < NR; ++r) {
-        const double pw = pref * rw[NR + r];
         const double *tx = tab + (3 * r) * T, *ty = tx + T, *tz = ty + T;
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
           if (tl + G * k < C::N4) {
             const int o = offs[k];
-            acc[k] += pw * (tx[o & 1023] * ty[(o >> 10) & 1023] * tz[o >> 20]);
+            acc[k] += tx[o & 1023] * ty[(o >> 10) & 1023] * tz[o >> 20];
           }
         }
       }

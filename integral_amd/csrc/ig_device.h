@@ -105,14 +105,16 @@ __device__ __forceinline__ double rys_series(const RysDev &R, int n, double T, i
 // as ((i*(LB+1)+j)*(LC+1)+k)*(LD+1)+l.  Rolling levels keep the register
 // footprint at two (NB+1)x(NK+1) slabs.
 // F: arithmetic type of the tables (double, or float for the mixed-precision
-// quartets).
+// quartets).  g00 is I(0,0|0,0): 1, or a factor every entry should carry
+// (the recurrences are linear and homogeneous in it), e.g. the quadrature
+// weight and prefactor folded into one axis at no cost.
 template <int LA, int LB, int LC, int LD, typename F = double>
 __device__ __forceinline__ void rys_axis_table(F c00, F cp0, F b10, F b01, F b00, F ab, F cd,
-                                               F *__restrict__ out) {
+                                               F *__restrict__ out, F g00 = F(1)) {
   constexpr int NB = LA + LB, NK = LC + LD;
   F g[NB + 1][NK + 1];
-  g[0][0] = F(1);
-  if constexpr (NB >= 1) g[1][0] = c00;
+  g[0][0] = g00;
+  if constexpr (NB >= 1) g[1][0] = c00 * g00;
 #pragma unroll
   for (int n = 1; n < NB; ++n) g[n + 1][0] = c00 * g[n][0] + F(n) * b10 * g[n - 1][0];
 #pragma unroll

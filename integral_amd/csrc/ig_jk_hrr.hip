@@ -143,10 +143,10 @@ __constant__ const HrrTab<C> kHrrTab = HrrTab<C>();
 // written to out[n*(NK+1)+m] (the first half of rys_axis_table).
 template <int NB, int NK>
 __device__ __forceinline__ void rys_axis_vrr(double c00, double cp0, double b10, double b01,
-                                             double b00, double *__restrict__ out) {
+                                             double b00, double *__restrict__ out, double g00) {
   double g[NB + 1][NK + 1];
-  g[0][0] = 1.0;
-  if constexpr (NB >= 1) g[1][0] = c00;
+  g[0][0] = g00;
+  if constexpr (NB >= 1) g[1][0] = c00 * g00;
 #pragma unroll
   for (int n = 1; n < NB; ++n) g[n + 1][0] = c00 * g[n][0] + double(n) * b10 * g[n - 1][0];
 #pragma unroll
@@ -256,18 +256,18 @@ __global__ void __launch_bounds__(64) jk_hrr_kernel(BasisDev b, PairsDev p, RysD
           const double pqa = pick(PQ);
           rys_axis_vrr<H::NB, H::NK>((pick(Pc) - pick(g.A)) - fq * pqa,
                                      (pick(Qc) - pick(g.C)) + fp * pqa, (1.0 - fq) * i2p,
-                                     (1.0 - fp) * i2q, 0.5 * x * ipq, tab + task * TV);
+                                     (1.0 - fp) * i2q, 0.5 * x * ipq, tab + task * TV,
+                                     ax == 2 ? pref * rw[NR + r] : 1.0);
         }
         team_sync();
 #pragma unroll 1
-        for (int r = 0; r < NR; ++r) {
-          const double pw = pref * rw[NR + r];
+        for (int r = 0; r < NR; ++r) {  // z tables carry the weight and prefactor
           const double *tx = tab + (3 * r) * TV, *ty = tx + TV, *tz = ty + TV;
 #pragma unroll
           for (int k = 0; k < H::E; ++k) {
             if (tl + G * k < H::NEF) {
               const int o = offs[k];
-              acc[k] += pw * (tx[o & 255] * ty[(o >> 8) & 255] * tz[o >> 16]);
+              acc[k] += tx[o & 255] * ty[(o >> 8) & 255] * tz[o >> 16];
             }
           }
         }

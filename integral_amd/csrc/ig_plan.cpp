@@ -1200,8 +1200,10 @@ extern "C" int ig_direct_schedule_read(const ig_direct *d, ig_schedule_entry *ou
   });
 }
 
-extern "C" int ig_direct_schedule_write(ig_direct *d, const ig_schedule_entry *in, size_t n) {
-  if (!d || (n && !in)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+namespace {
+
+// validation of a schedule to load (IG_OK or the failure status)
+int check_schedule(const ig_schedule_entry *in, size_t n) {
   for (size_t k = 0; k < n; ++k) {
     const ig_schedule_entry &e = in[k];
     const bool ok_l = e.la >= 0 && e.lb >= 0 && e.lc >= 0 && e.ld >= 0 && e.la <= kMaxL &&
@@ -1213,14 +1215,26 @@ extern "C" int ig_direct_schedule_write(ig_direct *d, const ig_schedule_entry *i
       return fail(IG_ERR_INVALID_ARGUMENT,
                   "schedule entry " + std::to_string(k) + " has an unknown variant");
   }
+  return IG_OK;
+}
+
+void apply_schedule(Plan &P, const ig_schedule_entry *in, size_t n) {
+  for (size_t k = 0; k < n; ++k) {
+    const int key = pair_class(in[k].la, in[k].lb) * kNumPairClass + pair_class(in[k].lc, in[k].ld);
+    P.variant[key] = int8_t(in[k].variant);
+    P.variant_ms[key] = float(in[k].variant >= 0 ? in[k].ms : 0.0);
+  }
+}
+
+}  // namespace
+
+extern "C" int ig_direct_schedule_write(ig_direct *d, const ig_schedule_entry *in, size_t n) {
+  if (!d || (n && !in)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_schedule(in, n)) return rc;
   return guarded([&] {
     Plan &P = *d->plan;
     std::lock_guard<std::mutex> g(P.mu);
-    for (size_t k = 0; k < n; ++k) {
-      const int key = pair_class(in[k].la, in[k].lb) * kNumPairClass + pair_class(in[k].lc, in[k].ld);
-      P.variant[key] = int8_t(in[k].variant);
-      P.variant_ms[key] = float(in[k].variant >= 0 ? in[k].ms : 0.0);
-    }
+    apply_schedule(P, in, n);
     return int(IG_OK);
   });
 }
@@ -1793,6 +1807,18 @@ extern "C" int ig_batch_autotune(ig_batch *b, size_t molecule, const double *d_d
     }
     // the trial launches moved plan buffers: capture again on the next call
     b->tau = NAN;
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_batch_schedule_write(ig_batch *b, const ig_schedule_entry *in, size_t n) {
+  if (!b || (n && !in)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_schedule(in, n)) return rc;
+  return guarded([&] {
+    std::lock_guard<std::mutex> g(b->mu);
+    check(hipStreamSynchronize(b->stream), "sync");
+    for (auto &P : b->plans) apply_schedule(*P, in, n);
+    b->tau = NAN;  // capture again with the new kernels
     return int(IG_OK);
   });
 }

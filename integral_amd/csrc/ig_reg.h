@@ -111,16 +111,16 @@ __device__ __forceinline__ void quartet_block_reg(const PairsDev &p, const RysDe
         const F b00 = F(0.5 * x * ipq);
         const F b10 = F((1.0 - fq) * i2p);
         const F b01 = F((1.0 - fp) * i2q);
+        // the weight and prefactor ride on the z table (its I(0,0|0,0))
+        const F pw = F(pref * rwt[r]);
         F I[3][T];
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
           rys_axis_table<C::LA, C::LB, C::LC, C::LD, F>(
               F((P[ax] - g.A[ax]) - fq * PQ[ax]), F(QC[ax] + fp * PQ[ax]), b10, b01, b00,
-              F(AB[ax]), F(CD[ax]), I[ax]);
-        const F pw = F(pref * rwt[r]);
+              F(AB[ax]), F(CD[ax]), I[ax], ax == 2 ? pw : F(1));
 #pragma unroll
-        for (int e = LO; e < HI; ++e)
-          acc[e - LO] += pw * (I[0][O.x[e]] * I[1][O.y[e]] * I[2][O.z[e]]);
+        for (int e = LO; e < HI; ++e) acc[e - LO] += I[0][O.x[e]] * I[1][O.y[e]] * I[2][O.z[e]];
       }
     }
   }

@@ -183,3 +183,13 @@ def test_batch_autotune_keeps_results():
             assert np.max(np.abs(K[i].cpu().numpy() - rk)) < TOL
     with pytest.raises(ig.UsageError):
         bc.autotune(dens, molecule=len(mols))
+    # a loaded schedule (every class on the team kernels) for the whole batch
+    sched = synthetic.to_basis(mols[0][0]).direct_contractor().schedule()
+    bc.set_schedule([dict(e, variant=0) for e in sched])
+    for t in J + K:
+        t.zero_()
+    bc.jk_device(dens, J, K, threshold=1e-12)
+    torch.cuda.synchronize()
+    for i, (rj, rk, _) in enumerate(ref):
+        assert np.max(np.abs(J[i].cpu().numpy() - rj)) < TOL
+        assert np.max(np.abs(K[i].cpu().numpy() - rk)) < TOL
