@@ -26,7 +26,8 @@ struct Cls {
   static constexpr int G = pick_team(N4);
   static constexpr int E = (N4 + G - 1) / G;
   static constexpr int TEAMS = 64 / G;
-  static constexpr int WORK = 2 * NR + 3 * NR * T;  // roots/weights + tables
+  // tables of one primitive quartet + the records of a chunk (quartet_block)
+  static constexpr int WORK = 3 * NR * T + (G < 16 ? G : 16) * (14 + 2 * NR);
 };
 
 // Packed per-element table offsets (x | y<<10 | z<<20) of element e.
@@ -72,16 +73,100 @@ __device__ __forceinline__ void load_geom(const BasisDev &b, int sA, int sB, int
   }
 }
 
+// Per-primitive-quartet records of a team in LDS, filled for a chunk of up
+// to PQ primitive quartets at once so the global loads (pair data, Rys
+// coefficients) of the whole chunk are in flight together:
+//   0 zeta_P, 1 zeta_Q, 2 1/(zeta_P+zeta_Q), 3 1/(2 zeta_P), 4 1/(2 zeta_Q),
+//   5-7 P, 8-10 Q, 11 prefactor, 12 T, 13 attenuation s, then NR roots and
+//   NR weights.
+constexpr int kRecHead = 14;
+__host__ __device__ constexpr int rec_size(int nr) { return kRecHead + 2 * nr; }
+__host__ __device__ constexpr int rec_chunk(int g) { return g < 16 ? g : 16; }
+
+template <int NR, int G>
+__device__ __forceinline__ void prim_records(const PairsDev &p, const RysDev &R, int ob, int ok,
+                                             int nkp, int c0, int nc, double omega, int tl,
+                                             double *__restrict__ rec) {
+  constexpr int RS = rec_size(NR);
+  for (int q = tl; q < nc; q += G) {
+    const int f = c0 + q, ib = f / nkp, ik = f - ib * nkp;
+    const double pz = p.zeta[ob + ib], qz = p.zeta[ok + ik];
+    const double P[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
+    const double Q[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
+    const double rs = rsqrt(pz + qz), ipq = rs * rs;
+    const double rho = pz * qz * ipq;
+    double r2 = 0.0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) r2 += (P[x] - Q[x]) * (P[x] - Q[x]);
+    double Tq = rho * r2;
+    double pref = kTwoPi52 * p.kz[ob + ib] * p.kz[ok + ik] * rs;
+    double s_att = 1.0;
+    if (omega > 0.0) {  // erf(w r)/r: roots/weights at sT, x -> s x, w -> sqrt(s) w
+      s_att = omega * omega / (rho + omega * omega);
+      Tq *= s_att;
+      pref *= sqrt(s_att);
+    }
+    double *r = rec + q * RS;
+    r[0] = pz;
+    r[1] = qz;
+    r[2] = ipq;
+    r[3] = p.i2z[ob + ib];
+    r[4] = p.i2z[ok + ik];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      r[5 + x] = P[x];
+      r[8 + x] = Q[x];
+    }
+    r[11] = pref;
+    r[12] = Tq;
+    r[13] = s_att;
+  }
+  team_sync();
+  for (int t = tl; t < nc * 2 * NR; t += G) {
+    const int q = t / (2 * NR), idx = t - q * (2 * NR);
+    double *r = rec + q * RS;
+    const double v = rys_series(R, NR, r[12], idx);
+    r[kRecHead + idx] = idx < NR ? v * r[13] : v;
+  }
+  team_sync();
+}
+
+// Coefficients of the (root, axis) 2D table of record r: the vertical
+// recurrence terms and the transfer distances (ab, cd), selected per axis.
+struct AxisTerms {
+  double c00, cp0, b10, b01, b00, g00;
+};
+template <int NR>
+__device__ __forceinline__ AxisTerms axis_terms(const double *r, int root, int ax,
+                                                const QuartetGeom &g) {
+  const double x = r[kRecHead + root];
+  const double pz = r[0], qz = r[1], ipq = r[2];
+  const double fq = x * qz * ipq, fp = x * pz * ipq;
+  const double Pa = r[5 + ax], Qa = r[8 + ax];
+  const double Aa = ax == 0 ? g.A[0] : (ax == 1 ? g.A[1] : g.A[2]);
+  const double Ca = ax == 0 ? g.C[0] : (ax == 1 ? g.C[1] : g.C[2]);
+  AxisTerms t;
+  t.c00 = (Pa - Aa) - fq * (Pa - Qa);
+  t.cp0 = (Qa - Ca) + fp * (Pa - Qa);
+  t.b10 = (1.0 - fq) * r[3];
+  t.b01 = (1.0 - fp) * r[4];
+  t.b00 = 0.5 * x * ipq;
+  // the z table carries the weight and prefactor
+  t.g00 = ax == 2 ? r[11] * r[kRecHead + NR + root] : 1.0;
+  return t;
+}
+
 // Contracted Cartesian block of quartet (pb | pk) accumulated into acc[]
-// (element tl + G*k).  `work` is this team's LDS area (C::WORK doubles).
+// (element tl + G*k).  `work` is this team's LDS area (C::WORK doubles):
+// the 3*NR tables of one primitive quartet, then the records of a chunk.
 template <class C>
 __device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R, int pb, int pk,
                                               const QuartetGeom &g, double omega, int tl,
                                               const int *offs, double *acc,
                                               double *__restrict__ work) {
-  constexpr int G = C::G, NR = C::NR, T = C::T;
-  double *rw = work;
-  double *tab = work + 2 * NR;
+  constexpr int G = C::G, NR = C::NR, T = C::T, PQ = rec_chunk(C::G), RS = rec_size(C::NR);
+  double *tab = work;
+  double *rec = work + 3 * NR * T;
   double AB[3], CD[3];
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
@@ -90,60 +175,26 @@ __device__ __forceinline__ void quartet_block(const PairsDev &p, const RysDev &R
   }
   const int ob = p.pp_off[pb], nbp = p.pp_n[pb];
   const int ok = p.pp_off[pk], nkp = p.pp_n[pk];
-  for (int ib = 0; ib < nbp; ++ib) {
-    const double pz = p.zeta[ob + ib];
-    const double P[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
-    const double kzp = kTwoPi52 * p.kz[ob + ib], i2p = p.i2z[ob + ib];
-    for (int ik = 0; ik < nkp; ++ik) {
-      const double qz = p.zeta[ok + ik];
-      const double Q[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
-      const double kzq = p.kz[ok + ik], i2q = p.i2z[ok + ik];
-      const double pq = pz + qz;
-      const double rs = rsqrt(pq), ipq = rs * rs;
-      const double rho = pz * qz * ipq;
-      double PQ[3], r2 = 0.0;
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        PQ[x] = P[x] - Q[x];
-        r2 += PQ[x] * PQ[x];
-      }
-      double Tq = rho * r2;
-      double pref = kzp * kzq * rs;
-      double s_att = 1.0;
-      if (omega > 0.0) {  // erf(w r)/r: roots/weights at sT, x -> s x, w -> sqrt(s) w
-        s_att = omega * omega / (rho + omega * omega);
-        Tq *= s_att;
-        pref *= sqrt(s_att);
-      }
-      // phase 1: roots and weights
-      for (int idx = tl; idx < 2 * NR; idx += G) {
-        double v = rys_series(R, NR, Tq, idx);
-        rw[idx] = (idx < NR) ? v * s_att : v;
-      }
-      team_sync();
-      // phase 2: per (root, axis) 2D tables
+  const int npq = nbp * nkp;
+  for (int c0 = 0; c0 < npq; c0 += PQ) {
+    const int nc = npq - c0 < PQ ? npq - c0 : PQ;
+    prim_records<NR, G>(p, R, ob, ok, nkp, c0, nc, omega, tl, rec);
+    for (int q = 0; q < nc; ++q) {
+      const double *r = rec + q * RS;
+      // per (root, axis) 2D tables
       for (int task = tl; task < 3 * NR; task += G) {
-        const int r = task / 3, ax = task - 3 * r;
-        // per-axis scalars by select, keeping the small arrays in registers
-        auto pick = [ax](const double *v) { return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]); };
-        const double x = rw[r];
-        const double fq = x * qz * ipq, fp = x * pz * ipq;
-        const double b00 = 0.5 * x * ipq;
-        const double b10 = (1.0 - fq) * i2p;
-        const double b01 = (1.0 - fp) * i2q;
-        const double pqa = pick(PQ);
-        const double c00 = (pick(P) - pick(g.A)) - fq * pqa;
-        const double cp0 = (pick(Q) - pick(g.C)) + fp * pqa;
-        // the z table carries the weight and prefactor
-        rys_axis_table<C::LA, C::LB, C::LC, C::LD>(c00, cp0, b10, b01, b00, pick(AB), pick(CD),
-                                                   tab + task * T,
-                                                   ax == 2 ? pref * rw[NR + r] : 1.0);
+        const int root = task / 3, ax = task - 3 * root;
+        const AxisTerms a = axis_terms<NR>(r, root, ax, g);
+        const double ab = ax == 0 ? AB[0] : (ax == 1 ? AB[1] : AB[2]);
+        const double cd = ax == 0 ? CD[0] : (ax == 1 ? CD[1] : CD[2]);
+        rys_axis_table<C::LA, C::LB, C::LC, C::LD>(a.c00, a.cp0, a.b10, a.b01, a.b00, ab, cd,
+                                                   tab + task * T, a.g00);
       }
       team_sync();
-      // phase 3: assembly, one root at a time so only 3E table reads are live
+      // assembly, one root at a time so only 3E table reads are live
 #pragma unroll 1
-      for (int r = 0; r < NR; ++r) {
-        const double *tx = tab + (3 * r) * T, *ty = tx + T, *tz = ty + T;
+      for (int root = 0; root < NR; ++root) {
+        const double *tx = tab + (3 * root) * T, *ty = tx + T, *tz = ty + T;
 #pragma unroll
         for (int k = 0; k < C::E; ++k) {
           if (tl + G * k < C::N4) {
@@ -295,8 +346,8 @@ __device__ __forceinline__ void init_offsets(int tl, int *offs) {
   X(2, 2, 0, 0) X(2, 2, 1, 0) X(2, 2, 1, 1) X(2, 2, 2, 0) X(2, 2, 2, 1) X(2, 2, 2, 2)
 
 // Class pairs with an f pair on the bra and blocks of at most 3600 Cartesian
-// elements: compiled J/K team kernels ((ff|fd) and (ff|ff) stay on the
-// runtime-l kernels, as do every other f kernel: ERI, Schwarz, gradient).
+// elements: compiled J/K team kernels (the other f kernels, ERI, Schwarz and
+// gradient, stay on the runtime-l kernels).
 #define IG_FOR_F_TEAM_CLASSES(X)                                                             \
   X(3, 0, 0, 0) X(3, 0, 1, 0) X(3, 0, 1, 1) X(3, 0, 2, 0) X(3, 0, 2, 1) X(3, 0, 2, 2)        \
   X(3, 0, 3, 0)                                                                              \
@@ -306,6 +357,13 @@ __device__ __forceinline__ void init_offsets(int tl, int *offs) {
   X(3, 2, 3, 0) X(3, 2, 3, 1) X(3, 2, 3, 2)                                                  \
   X(3, 3, 0, 0) X(3, 3, 1, 0) X(3, 3, 1, 1) X(3, 3, 2, 0) X(3, 3, 2, 1) X(3, 3, 2, 2)        \
   X(3, 3, 3, 0) X(3, 3, 3, 1)
+
+// Classes with only the team kernel with contracted-space transfers: none.
+// (ff|fd) and (ff|ff) built that way (blocks of 6000 and 10000 elements in
+// LDS) missed the 1e-12 bar on a small f basis (3.6e-12: the degree-3
+// transfers on both sides amplify the contracted intermediates), so they
+// stay on the runtime-l kernels, which transfer per root.
+#define IG_FOR_F_HRR_ONLY_CLASSES(X)
 
 // The f class pairs of at most 270 elements (four register slices) also have
 // the lane-per-quartet kernels (register, row sums; FP64 and FP32; batched).

@@ -134,8 +134,11 @@ bool class_supported(int la, int lb, int lc, int ld);
 bool jk_team_compiled(int la, int lb, int lc, int ld);
 bool jk_lane_compiled(int la, int lb, int lc, int ld);
 // the team kernel with contracted-space horizontal transfers (ig_jk_hrr.hip):
-// team classes with lb > 0 or ld > 0
+// team classes of more than two register slices with lb > 0 or ld > 0, and
+// (ff|fd), (ff|ff)
 bool jk_hrr_compiled(int la, int lb, int lc, int ld);
+// any compiled J/K kernel
+bool jk_compiled(int la, int lb, int lc, int ld);
 hipError_t launch_jk_hrr(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                          const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
                          unsigned long long *counters, hipStream_t s);

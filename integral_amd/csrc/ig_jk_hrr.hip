@@ -72,7 +72,7 @@ struct HrrCls {
   // LDS of one team: the vertical tables during the primitive loop; after it
   // R1 (the contracted [E|F], later the block) and R2 ((ab|F], or the block
   // when the bra needs no transfer), then the transfer coefficients
-  static constexpr int WORK = 2 * NR + 3 * NR * TV;
+  static constexpr int WORK = 3 * NR * TV + rec_chunk(G) * rec_size(NR);
   static constexpr int R1 = cmax(NEF, C::N4);
   static constexpr int R2 = LB > 0 ? NH1 : (LD > 0 ? C::N4 : 0);
   static constexpr int CO = NT * (C::nb + C::nd);
@@ -182,7 +182,8 @@ __global__ void __launch_bounds__(64) jk_hrr_kernel(BasisDev b, PairsDev p, RysD
   __shared__ double lds[H::TEAMS * H::PER];
   const int lane = threadIdx.x, team = lane / G, tl = lane % G;
   double *work = lds + team * H::PER;
-  double *rw = work, *tab = work + 2 * NR;
+  double *tab = work, *rec = work + 3 * NR * TV;
+  constexpr int PQ = rec_chunk(G), RS = rec_size(NR);
   double *r1 = work, *r2 = work + H::R1;
   double *bco = work + cmax(H::WORK, H::R1 + H::R2), *kco = bco + NT * nb;
   int offs[H::E];
@@ -219,50 +220,21 @@ __global__ void __launch_bounds__(64) jk_hrr_kernel(BasisDev b, PairsDev p, RysD
     for (int k = 0; k < H::E; ++k) acc[k] = 0.0;
     const int ob = p.pp_off[pb], nbp = p.pp_n[pb];
     const int ok = p.pp_off[pk], nkp = p.pp_n[pk];
-    for (int ib = 0; ib < nbp; ++ib) {
-      const double pz = p.zeta[ob + ib];
-      const double Pc[3] = {p.px[ob + ib], p.py[ob + ib], p.pz[ob + ib]};
-      const double kzp = kTwoPi52 * p.kz[ob + ib], i2p = p.i2z[ob + ib];
-      for (int ik = 0; ik < nkp; ++ik) {
-        const double qz = p.zeta[ok + ik];
-        const double Qc[3] = {p.px[ok + ik], p.py[ok + ik], p.pz[ok + ik]};
-        const double kzq = p.kz[ok + ik], i2q = p.i2z[ok + ik];
-        const double rs = rsqrt(pz + qz), ipq = rs * rs;
-        const double rho = pz * qz * ipq;
-        double PQ[3], r2sq = 0.0;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          PQ[x] = Pc[x] - Qc[x];
-          r2sq += PQ[x] * PQ[x];
-        }
-        double Tq = rho * r2sq;
-        double pref = kzp * kzq * rs;
-        double s_att = 1.0;
-        if (omega > 0.0) {  // erf(w r)/r: roots/weights at sT, x -> s x, w -> sqrt(s) w
-          s_att = omega * omega / (rho + omega * omega);
-          Tq *= s_att;
-          pref *= sqrt(s_att);
-        }
-        for (int idx = tl; idx < 2 * NR; idx += G) {
-          const double v = rys_series(R, NR, Tq, idx);
-          rw[idx] = (idx < NR) ? v * s_att : v;
-        }
-        team_sync();
+    const int npq = nbp * nkp;
+    for (int c0 = 0; c0 < npq; c0 += PQ) {
+      const int nc = npq - c0 < PQ ? npq - c0 : PQ;
+      prim_records<NR, G>(p, R, ob, ok, nkp, c0, nc, omega, tl, rec);
+      for (int q = 0; q < nc; ++q) {
+        const double *r = rec + q * RS;
         for (int task = tl; task < 3 * NR; task += G) {
-          const int r = task / 3, ax = task - 3 * r;
-          auto pick = [ax](const double *v) { return ax == 0 ? v[0] : (ax == 1 ? v[1] : v[2]); };
-          const double x = rw[r];
-          const double fq = x * qz * ipq, fp = x * pz * ipq;
-          const double pqa = pick(PQ);
-          rys_axis_vrr<H::NB, H::NK>((pick(Pc) - pick(g.A)) - fq * pqa,
-                                     (pick(Qc) - pick(g.C)) + fp * pqa, (1.0 - fq) * i2p,
-                                     (1.0 - fp) * i2q, 0.5 * x * ipq, tab + task * TV,
-                                     ax == 2 ? pref * rw[NR + r] : 1.0);
+          const int root = task / 3, ax = task - 3 * root;
+          const AxisTerms a = axis_terms<NR>(r, root, ax, g);
+          rys_axis_vrr<H::NB, H::NK>(a.c00, a.cp0, a.b10, a.b01, a.b00, tab + task * TV, a.g00);
         }
         team_sync();
 #pragma unroll 1
-        for (int r = 0; r < NR; ++r) {  // z tables carry the weight and prefactor
-          const double *tx = tab + (3 * r) * TV, *ty = tx + TV, *tz = ty + TV;
+        for (int root = 0; root < NR; ++root) {  // z tables carry the weight and prefactor
+          const double *tx = tab + (3 * root) * TV, *ty = tx + TV, *tz = ty + TV;
 #pragma unroll
           for (int k = 0; k < H::E; ++k) {
             if (tl + G * k < H::NEF) {
@@ -339,7 +311,7 @@ hipError_t jk_hrr_one(const BasisDev &b, const PairsDev &p, const RysDev &r, con
   if constexpr (!H::useful) {
     return hipErrorInvalidValue;
   } else {
-    static_assert(H::TEAMS * H::PER * sizeof(double) <= 64 * 1024, "team LDS over 64 KB");
+    static_assert(H::TEAMS * H::PER * sizeof(double) <= 160 * 1024, "team LDS over 160 KB");
     if (qs.total <= 0) return hipSuccess;
     long long blocks = (qs.total + H::TEAMS - 1) / H::TEAMS;
     if (blocks > 256 * 64) blocks = 256 * 64;
@@ -354,8 +326,16 @@ hipError_t jk_hrr_one(const BasisDev &b, const PairsDev &p, const RysDev &r, con
 // offered for the classes the team kernel is meant for (blocks over two
 // register slices); on smaller ones one lane per quartet is faster
 bool jk_hrr_compiled(int la, int lb, int lc, int ld) {
+#define IG_CASE(A, B, Cc, D) \
+  if (la == A && lb == B && lc == Cc && ld == D) return true;
+  IG_FOR_F_HRR_ONLY_CLASSES(IG_CASE)
+#undef IG_CASE
   return jk_team_compiled(la, lb, lc, ld) && (lb > 0 || ld > 0) &&
          ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) > 2 * kRegBlockMax;
+}
+
+bool jk_compiled(int la, int lb, int lc, int ld) {
+  return jk_team_compiled(la, lb, lc, ld) || jk_hrr_compiled(la, lb, lc, ld);
 }
 
 hipError_t launch_jk_hrr(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
@@ -366,6 +346,7 @@ hipError_t launch_jk_hrr(int la, int lb, int lc, int ld, const BasisDev &b, cons
     return jk_hrr_one<A, B, Cc, D>(b, p, r, qs, d, omega, ctr, s);
   IG_FOR_CLASSES(IG_CASE)
   IG_FOR_F_TEAM_CLASSES(IG_CASE)
+  IG_FOR_F_HRR_ONLY_CLASSES(IG_CASE)
 #undef IG_CASE
   return hipErrorInvalidValue;
 }

@@ -307,10 +307,12 @@ int jk_default_variant(int la, int lb, int lc, int ld) {
   // sums while a quartet needs at most two element slices (each slice
   // rebuilds the quartet's 2D tables), team kernels for larger blocks
   // (measured on s,p,d,f molecules: the contracted-space transfers win on
-  // the largest blocks, (fd|fd), (ff|dd), (fd|dd))
+  // the largest blocks with a d or f second ket shell, (fd|fd), (ff|dd),
+  // (fd|dd); the autotuner finds the others)
   const int n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
   if (n4 <= 2 * kRegBlockMax && jk_lane_compiled(la, lb, lc, ld)) return kJkRowAcc;
-  return n4 >= 2000 && jk_hrr_compiled(la, lb, lc, ld) ? kJkTeamHrr : kJkTeam;
+  if (!jk_team_compiled(la, lb, lc, ld)) return kJkTeamHrr;
+  return n4 >= 2000 && ld >= 2 && jk_hrr_compiled(la, lb, lc, ld) ? kJkTeamHrr : kJkTeam;
 }
 
 hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,

@@ -775,7 +775,7 @@ size_t big_need(const ClassRange &X, const ClassRange &Y, long long total, int t
 
 // the same for J/K, whose compiled kernels also cover most f classes
 size_t big_need_jk(const ClassRange &X, const ClassRange &Y, long long total) {
-  if (total <= 0 || jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) return 0;
+  if (total <= 0 || jk_compiled(X.l1, X.l2, Y.l1, Y.l2)) return 0;
   return big_scratch_elems(X.l1, X.l2, Y.l1, Y.l2, total, 0);
 }
 
@@ -791,13 +791,13 @@ hipError_t jk_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const Qu
                      const DensDev &dd, unsigned long long *ctr, hipStream_t s, int variant = -1,
                      bool fp32 = false) {
   RysDevice &R = rys_device();
-  if (jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) {
+  if (jk_compiled(X.l1, X.l2, Y.l1, Y.l2)) {
     const bool lane = jk_lane_compiled(X.l1, X.l2, Y.l1, Y.l2);
     int v = variant >= 0 ? variant : jk_variant(P, X, Y);
     const bool team = v == kJkTeam || v == kJkTeamHrr;
     if (fp32 && team && lane) v = kJkReg;
     if (!jk_variant_fits(v, X.l1, X.l2, Y.l1, Y.l2, P.bdev, dd.neff, qs.list != nullptr))
-      v = lane ? kJkReg : kJkTeam;
+      v = lane ? kJkReg : (jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2) ? kJkTeam : kJkTeamHrr);
     if (fp32 && v != kJkTeam && v != kJkTeamHrr)
       return launch_jk_lane_f32(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, dd, 0.0, v,
                                 ctr, s);
@@ -1118,7 +1118,7 @@ void autotune_plan(Plan &P, const double *d_dens, size_t ndens, double threshold
   try {
     for (auto &cp : P.cached_cps) {
       const ClassRange &X = P.cls[cp.x], &Y = P.cls[cp.y];
-      if (!cp.total || !jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) continue;
+      if (!cp.total || !jk_compiled(X.l1, X.l2, Y.l1, Y.l2)) continue;
       const QuartetSet qs = implicit_set(P, cp);
       int best = -1;
       float best_ms = 0.0f;
@@ -1182,7 +1182,7 @@ extern "C" int ig_direct_schedule_read(const ig_direct *d, ig_schedule_entry *ou
     for (int x = 0; x < kNumPairClass; ++x)
       for (int y = 0; y <= x; ++y) {
         const ClassRange &X = P.cls[x], &Y = P.cls[y];
-        if (!X.n || !Y.n || !jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2)) continue;
+        if (!X.n || !Y.n || !jk_compiled(X.l1, X.l2, Y.l1, Y.l2)) continue;
         if (k < cap) {
           const int key = x * kNumPairClass + y;
           out[k].la = X.l1;
@@ -1208,7 +1208,7 @@ int check_schedule(const ig_schedule_entry *in, size_t n) {
     const ig_schedule_entry &e = in[k];
     const bool ok_l = e.la >= 0 && e.lb >= 0 && e.lc >= 0 && e.ld >= 0 && e.la <= kMaxL &&
                       e.lc <= kMaxL && e.lb <= e.la && e.ld <= e.lc;
-    if (!ok_l || !jk_team_compiled(e.la, e.lb, e.lc, e.ld))
+    if (!ok_l || !jk_compiled(e.la, e.lb, e.lc, e.ld))
       return fail(IG_ERR_INVALID_ARGUMENT,
                   "schedule entry " + std::to_string(k) + " names a class without compiled kernels");
     if (e.variant < -1 || e.variant >= kJkNumVariants)

@@ -191,9 +191,9 @@ def test_direct_jk_every_kernel_variant(variant):
 @pytest.mark.parametrize("variant", [None, 0, 1, 2, 4])
 def test_direct_jk_f_classes(variant):
     """The compiled f-shell J/K kernels (team kernels for f class pairs up to
-    3600 elements, lane kernels up to 270) against the oracle at tau = 0 on
-    non-symmetric densities, spherical and Cartesian f shells; (ff|fd) and
-    (ff|ff) run on the runtime-l kernels."""
+    3600 elements, lane kernels up to 270, (ff|fd) and (ff|ff) on the team
+    kernel with contracted-space transfers only) against the oracle at tau = 0
+    on non-symmetric densities, spherical and Cartesian f shells."""
     shells, _ = synthetic.molecule(20260111, 3, 4.0, [(0, 2), (1, 2), (2, 1), (3, 1)],
                                    (0.1, 30.0), True)
     shells += [(3, (0.4, 1.1, -0.7), [0.8, 0.3], [0.6, 0.5], False),
@@ -227,7 +227,7 @@ def test_autotune_keeps_results():
     oJ, oK = O.Basis(shells).direct_jk([p], 1e-12)
     assert np.max(np.abs(J[0] - oJ[0])) < TOL and np.max(np.abs(K[0] - oK[0])) < TOL
     with pytest.raises(ig.IntegralError):
-        dc.set_schedule([{"cls": (3, 3, 3, 3), "variant": 0}])
+        dc.set_schedule([{"cls": (4, 4, 4, 4), "variant": 0}])
     with pytest.raises(ig.IntegralError):
         dc.set_schedule([{"cls": (4, 0, 0, 0), "variant": 0}])
 
