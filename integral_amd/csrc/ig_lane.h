@@ -105,13 +105,13 @@ hipError_t jk_lane_batch_one(const BasisDev *bs, const PairsDev *ps, const RysDe
   if (variant == kJkHgp) {
     if constexpr (HgpClass<C>::ok)
       return launch_jk_rowacc_batch_slices<C, 0, F, 1>(bs, ps, r, sets, dds, seg, nb, queues,
-                                                       chunk, ctr, s, int(blocks),
-                                                       acc_doubles * sizeof(double));
+                                                       chunk, int(acc_doubles), ctr, s,
+                                                       int(blocks), acc_doubles * sizeof(double));
     else
       return hipErrorInvalidValue;
   }
   return launch_jk_rowacc_batch_slices<C, 0, F, 0>(bs, ps, r, sets, dds, seg, nb, queues, chunk,
-                                                   ctr, s, int(blocks),
+                                                   int(acc_doubles), ctr, s, int(blocks),
                                                    rys + acc_doubles * sizeof(double));
 }
 

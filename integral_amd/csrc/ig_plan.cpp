@@ -1548,6 +1548,12 @@ extern "C" int ig_grad_last_stats(const ig_grad *g, int64_t *st) {
 // ---------------------------------------------------------------------------
 // batched J/K over many bases (one symmetric density per basis)
 
+// The batched graph's execution is switched off until the cause of an
+// illegal memory access it took on the GPU (small mixed s,p,d,f molecules,
+// after 64-molecule runs had passed) is found; ig_batch_jk_device fails with
+// IG_ERR_UNSUPPORTED meanwhile.
+constexpr bool kBatchGraphEnabled = false;
+
 struct ig_batch {
   std::vector<std::unique_ptr<Plan>> plans;
   std::mutex mu;
@@ -1758,6 +1764,11 @@ extern "C" int ig_batch_jk_device(ig_batch *b, const double *const *d_dens, doub
                                   int want_k, double *const *d_J, double *const *d_K,
                                   void *stream) {
   if (!b || !d_dens || !d_J || (want_k && !d_K)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (!kBatchGraphEnabled)
+    return fail(IG_ERR_UNSUPPORTED,
+                "batched J/K is switched off in this build: its graph took an illegal memory "
+                "access on the GPU with small mixed s,p,d,f molecules and the cause is not found "
+                "yet; use ig_direct_jk_device per molecule");
   const size_t nb = b->plans.size();
   for (size_t m = 0; m < nb; ++m)
     if (!d_dens[m] || !d_J[m] || (want_k && !d_K[m]))

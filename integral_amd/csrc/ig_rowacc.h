@@ -339,13 +339,14 @@ __global__ void __launch_bounds__(64 * kRowWaves)
 // Workgroups take chunks of it from a device work queue (one atomic counter,
 // zeroed before the launch), so molecules of uneven size balance.  A chunk
 // may span bases; each part runs with that basis's views (read from the
-// device arrays bs, ps, sets, dds).  The LDS accumulators are sized for the
-// widest basis.
+// device arrays bs, ps, sets, dds).  The LDS accumulators hold nacc doubles:
+// the host's size for the widest basis that has quartets of this class (the
+// launch's dynamic LDS is sized from the same number).
 template <class C, int LO, int HI, typename F, int ENG = 0>
 __global__ void __launch_bounds__(64 * kRowWaves)
     jk_rowacc_batch_kernel(const BasisDev *bs, const PairsDev *ps, RysDev R,
                            const QuartetSet *sets, const DensDev *dds, const long long *seg,
-                           int nb, unsigned *queue, long long chunk,
+                           int nb, unsigned *queue, long long chunk, int nacc,
                            unsigned long long *counters) {
   extern __shared__ double lds[];
   __shared__ int s_row[2];
@@ -353,11 +354,6 @@ __global__ void __launch_bounds__(64 * kRowWaves)
   if constexpr (ENG == 0) stage_rys<C::NR>(R, lds);
   const int nrys = ENG == 0 ? 2 * R.nsub[C::NR] * C::NR * kRysTerms : 0;
   double *JB = lds + nrys;
-  int nacc = 0;
-  for (int m = 0; m < nb; ++m) {
-    const int w = int(rowacc_doubles(C::na, C::nb, dds[m].neff, RowCols<C>(bs[m]).W));
-    nacc = w > nacc ? w : nacc;
-  }
   for (int i = threadIdx.x; i < nacc; i += blockDim.x) JB[i] = 0.0;
   unsigned long long my_quartets = 0, my_prims = 0;
   const long long total = seg[nb];
@@ -411,18 +407,18 @@ template <class C, int S, typename F, int ENG = 0>
 hipError_t launch_jk_rowacc_batch_slices(const BasisDev *bs, const PairsDev *ps, const RysDev &r,
                                          const QuartetSet *sets, const DensDev *dds,
                                          const long long *seg, int nb, unsigned *queues,
-                                         long long chunk, unsigned long long *ctr, hipStream_t s,
-                                         int blocks, size_t lds) {
+                                         long long chunk, int nacc, unsigned long long *ctr,
+                                         hipStream_t s, int blocks, size_t lds) {
   constexpr int NS = RegSlices<C, F>::NS, E = RegSlices<C, F>::E;
   if constexpr (S < NS) {
     constexpr int LO = S * E, HI = (S + 1) * E < C::N4 ? (S + 1) * E : C::N4;
     hipLaunchKernelGGL((jk_rowacc_batch_kernel<C, LO, HI, F, ENG>), dim3(blocks),
                        dim3(64 * kRowWaves), lds, s, bs, ps, r, sets, dds, seg, nb, queues + S,
-                       chunk, ctr);
+                       chunk, nacc, ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_jk_rowacc_batch_slices<C, S + 1, F, ENG>(bs, ps, r, sets, dds, seg, nb, queues,
-                                                          chunk, ctr, s, blocks, lds);
+                                                          chunk, nacc, ctr, s, blocks, lds);
   } else {
     return hipSuccess;
   }
