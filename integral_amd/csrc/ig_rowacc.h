@@ -230,26 +230,30 @@ __device__ __forceinline__ void rowacc_range(const BasisDev &b, const PairsDev &
                                              const RysDev &R, const QuartetSet &qs,
                                              const DensDev &dd, double omega, long long begin,
                                              long long end, const double *rys_lds, double *JB,
-                                             int *s_row, unsigned long long &my_quartets,
+                                             unsigned long long &my_quartets,
                                              unsigned long long &my_prims) {
   const RowCols<C> rc(b);
   const int nsh = b.nsh;
   const size_t xo = jk_copy_offset(dd);
-  RowCursor cur;
+  RowCursor cur, bcur;
   int racc = -1;  // accumulated row, uniform over the workgroup
-  int step = 0;
-  for (long long base = begin; base < end; base += blockDim.x, ++step) {
+  for (long long base = begin; base < end; base += blockDim.x) {
     const long long t = base + threadIdx.x;
     bool on = t < end;
     int P = 0, Q = 0;
     if (on) cur.next(qs, t, P, Q);
-    // s_row by step parity: a slow wavefront still reading step i's value
-    // cannot see step i+1's
-    if (threadIdx.x == 0) s_row[step & 1] = P;  // t = base < end: thread 0 has a quartet
-    __syncthreads();
-    const int row = s_row[step & 1];
+    // the step's row is that of its first quartet, which every thread
+    // tracks itself (a second cursor on `base`), so the workgroup agrees on
+    // it without exchanging it; wavefronts only meet at a row change, where
+    // the finished row's LDS sums are flushed (the sums themselves are LDS
+    // atomics, safe between wavefronts at different steps)
+    int row, Qb;
+    bcur.next(qs, base, row, Qb);
     if (row != racc) {
-      if (racc >= 0) rowacc_flush<C>(b, p, qs, dd, rc, racc, JB, xo);
+      if (racc >= 0) {
+        __syncthreads();
+        rowacc_flush<C>(b, p, qs, dd, rc, racc, JB, xo);
+      }
       racc = row;
     }
     const int pb = qs.x_off + P, pk = qs.y_off + Q;
@@ -318,19 +322,18 @@ __global__ void __launch_bounds__(64 * kRowWaves)
     jk_rowacc_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs, DensDev dd, double omega,
                      unsigned long long *counters) {
   extern __shared__ double lds[];
-  __shared__ int s_row[2];
   if constexpr (ENG == 0) stage_rys<C::NR>(R, lds);
   const int nrys = ENG == 0 ? 2 * R.nsub[C::NR] * C::NR * kRysTerms : 0;
   double *JB = lds + nrys;
   const int nacc = int(rowacc_doubles(C::na, C::nb, dd.neff, RowCols<C>(b).W));
   for (int i = threadIdx.x; i < nacc; i += blockDim.x) JB[i] = 0.0;
-  // the barrier at the top of the first step orders these stores
+  __syncthreads();  // zeros (and the staged Rys tables) before any use
   unsigned long long my_quartets = 0, my_prims = 0;
   const long long chunk = (qs.total + gridDim.x - 1) / gridDim.x;
   const long long begin = (long long)blockIdx.x * chunk;
   const long long end = begin + chunk < qs.total ? begin + chunk : qs.total;
-  rowacc_range<C, LO, HI, F, ENG>(b, p, R, qs, dd, omega, begin, end, lds, JB, s_row,
-                                  my_quartets, my_prims);
+  rowacc_range<C, LO, HI, F, ENG>(b, p, R, qs, dd, omega, begin, end, lds, JB, my_quartets,
+                                  my_prims);
   rowacc_count<LO>(my_quartets, my_prims, counters);
 }
 
@@ -349,7 +352,6 @@ __global__ void __launch_bounds__(64 * kRowWaves)
                            int nb, unsigned *queue, long long chunk, int nacc,
                            unsigned long long *counters) {
   extern __shared__ double lds[];
-  __shared__ int s_row[2];
   __shared__ long long s_item[2];
   if constexpr (ENG == 0) stage_rys<C::NR>(R, lds);
   const int nrys = ENG == 0 ? 2 * R.nsub[C::NR] * C::NR * kRysTerms : 0;
@@ -378,7 +380,7 @@ __global__ void __launch_bounds__(64 * kRowWaves)
       const QuartetSet qs = sets[lo];
       const DensDev dd = dds[lo];
       rowacc_range<C, LO, HI, F, ENG>(b, p, R, qs, dd, 0.0, g0 - seg[lo], e - seg[lo], lds, JB,
-                                      s_row, my_quartets, my_prims);
+                                      my_quartets, my_prims);
       g0 = e;
     }
   }
