@@ -112,6 +112,36 @@ int ig_eri_screened(const ig_basis *basis, int engine, double tau, double *out, 
 int ig_schwarz_bounds(const ig_basis *basis, int engine, double *out, size_t out_len,
                       void *stream);
 
+/* ---- packed symmetry-unique ERIs: PackedEriBuilder (packed_eri.rs) -------
+ * AO pairs (p, q), p >= q, numbered shell-pair blocked: canonical shell pairs
+ * (i >= j) row-major, inside a pair the function pairs (a, b) row-major with
+ * b <= a when i == j (packed_eri.rs:10-21, :99-110).  The output is the
+ * row-major lower triangle over pair indices: (t|u), u <= t, at t(t+1)/2 + u.
+ * Canonical bra pair k owns the contiguous rows of its AO pairs. */
+/* Number of AO pairs m (= nao(nao+1)/2) and PackedEriBuilder::output_len,
+ * m(m+1)/2 (packed_eri.rs:140-144). */
+size_t ig_packed_eri_n_ao_pairs(const ig_basis *basis);
+size_t ig_packed_eri_output_len(const ig_basis *basis);
+/* PackedEriBuilder::ao_pairs (packed_eri.rs:133-138): n must be the AO-pair
+ * count; pairs receives (p, q) as 2n ints in storage order. */
+int ig_packed_eri_ao_pairs(const ig_basis *basis, int32_t *pairs, size_t n);
+/* The slice [offset, offset+len) of the output owned by canonical bra pairs
+ * [bra_begin, bra_end) (bra_end < 0: all) -- PackedEriBuilder::partition
+ * (packed_eri.rs:146-171) for a range of tasks. */
+int ig_packed_eri_rows(const ig_basis *basis, int64_t bra_begin, int64_t bra_end, size_t *offset,
+                       size_t *len);
+/* PackedEriBuilder::fill for the bra pairs [bra_begin, bra_end)
+ * (packed_eri.rs:173-220): every canonical quartet with that bra and a ket
+ * pair <= it; with threshold > 0, quartets with Q_ij*Q_kl < threshold are
+ * skipped.  rows (host, len = the range's length from ig_packed_eri_rows) is
+ * overwritten: skipped and zero integrals are 0. */
+int ig_packed_eri_fill(const ig_basis *basis, int engine, int64_t bra_begin, int64_t bra_end,
+                       double threshold, double *rows, size_t len, void *stream);
+/* PackedEriBuilder::build (packed_eri.rs:222-230): all rows; out (host) has
+ * ig_packed_eri_output_len doubles, overwritten. */
+int ig_packed_eri_build(const ig_basis *basis, int engine, double threshold, double *out,
+                        size_t len, void *stream);
+
 /* ---- integral-direct J/K: DirectContractor (direct.rs:39-218) ------------ */
 typedef struct ig_direct ig_direct;
 

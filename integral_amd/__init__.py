@@ -196,6 +196,14 @@ def library():
     lib.ig_direct_schedule_read.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz,
                                             ctypes.POINTER(sz)]
     lib.ig_direct_schedule_write.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz]
+    lib.ig_packed_eri_n_ao_pairs.argtypes = [_vp]
+    lib.ig_packed_eri_n_ao_pairs.restype = sz
+    lib.ig_packed_eri_output_len.argtypes = [_vp]
+    lib.ig_packed_eri_output_len.restype = sz
+    lib.ig_packed_eri_ao_pairs.argtypes = [_vp, ctypes.POINTER(ctypes.c_int32), sz]
+    lib.ig_packed_eri_rows.argtypes = [_vp, i64, i64, ctypes.POINTER(sz), ctypes.POINTER(sz)]
+    lib.ig_packed_eri_fill.argtypes = [_vp, ctypes.c_int, i64, i64, ctypes.c_double, _dp, sz, _vp]
+    lib.ig_packed_eri_build.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, sz, _vp]
     lib.ig_batch_create.argtypes = [ctypes.POINTER(_vp), sz, ctypes.POINTER(_vp)]
     lib.ig_batch_destroy.argtypes = [_vp]
     lib.ig_batch_destroy.restype = None
@@ -390,8 +398,91 @@ class Basis:
     def direct_contractor(self):
         return DirectContractor(self)
 
+    def packed_eri_builder(self):
+        return PackedEriBuilder(self)
+
     def direct_grad_contractor(self):
         return DirectGradContractor(self)
+
+
+class PackedEriWorkspace:
+    """packed_eri.rs:34-47.  The device plan owns all scratch; kept for API parity."""
+
+
+class PackedBraFill:
+    """packed_eri.rs:51-63: the packed rows owned by one canonical bra pair."""
+
+    def __init__(self, bra, rows):
+        self._bra = bra
+        self.rows = rows
+
+    def bra(self):
+        return self._bra
+
+
+class PackedEriBuilder:
+    """PackedEriBuilder (packed_eri.rs:67-240): the nao^4/8 symmetry-unique
+    ERIs over shell-pair-blocked AO pairs, (t|u) at t(t+1)/2 + u."""
+
+    def __init__(self, basis, engine=Engine.Auto):
+        self._basis = basis
+        self._engine = int(engine)
+        nsh = len(basis._shells)
+        self._pairs = [(i, j) for i in range(nsh) for j in range(i + 1)]
+
+    @classmethod
+    def with_engine(cls, basis, engine):
+        return cls(basis, engine)
+
+    def bra_pairs(self):
+        return list(self._pairs)
+
+    def ao_pairs(self):
+        lib = library()
+        n = int(lib.ig_packed_eri_n_ao_pairs(self._basis._h))
+        buf = (ctypes.c_int32 * max(2 * n, 1))()
+        _raise(lib.ig_packed_eri_ao_pairs(self._basis._h, buf, n))
+        return [(buf[2 * k], buf[2 * k + 1]) for k in range(n)]
+
+    def output_len(self):
+        return int(library().ig_packed_eri_output_len(self._basis._h))
+
+    def _rows(self, b0, b1):
+        off, ln = ctypes.c_size_t(), ctypes.c_size_t()
+        _raise(library().ig_packed_eri_rows(self._basis._h, int(b0), int(b1), ctypes.byref(off),
+                                            ctypes.byref(ln)))
+        return off.value, ln.value
+
+    def partition(self, out):
+        """One task per bra pair over disjoint slices of `out` (a zeroed
+        float64 array of output_len()); packed_eri.rs:146-171."""
+        if not (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous
+                and out.size == self.output_len()):
+            raise UsageError("packed ERI output buffer must hold %d elements" % self.output_len())
+        tasks = []
+        for bra in range(len(self._pairs)):
+            off, ln = self._rows(bra, bra + 1)
+            tasks.append(PackedBraFill(bra, out[off:off + ln]))
+        return tasks
+
+    def fill(self, workspace, task, threshold=0.0, stream=None):
+        """Evaluate one bra pair's quartets into its rows (packed_eri.rs:173-220)."""
+        rows = task.rows
+        _raise(library().ig_packed_eri_fill(self._basis._h, self._engine, task.bra(),
+                                            task.bra() + 1, float(threshold), _ptr(rows),
+                                            rows.size, _stream_ptr(stream)))
+
+    def fill_range(self, bra_begin, bra_end, rows, threshold=0.0, stream=None):
+        """fill() for the contiguous bra pairs [bra_begin, bra_end) at once."""
+        _raise(library().ig_packed_eri_fill(self._basis._h, self._engine, int(bra_begin),
+                                            int(bra_end), float(threshold), _ptr(rows), rows.size,
+                                            _stream_ptr(stream)))
+
+    def build(self, threshold=0.0, stream=None):
+        out = np.zeros(self.output_len())
+        _raise(library().ig_packed_eri_build(self._basis._h, self._engine, float(threshold),
+                                             _ptr(out), out.size, _stream_ptr(stream)))
+        return out
 
 
 class DirectWorkspace:

@@ -375,8 +375,9 @@ __device__ __constant__ int kBigPerms8[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 
 
 __global__ void __launch_bounds__(kBigThreads)
     eri_big_kernel(int la, int lb, int lc, int ld, int rb, BasisDev b, PairsDev p, RysDev R,
-                   QuartetSet qs, double *out, const unsigned long long *eoff, double *scratch,
+                   QuartetSet qs, EriOut eo, const unsigned long long *eoff, double *scratch,
                    size_t slot) {
+  double *out = eo.out;
   __shared__ BigStatic S;
   extern __shared__ double dyn[];
   const BigClass K(la, lb, lc, ld, 0);
@@ -413,6 +414,16 @@ __global__ void __launch_bounds__(kBigThreads)
       __syncthreads();
     }
     const int nf = dims[0] * dims[1] * dims[2] * dims[3];
+    if (eo.start) {  // packed: the canonical element only
+      for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+        const int id = e % dims[3], r1 = e / dims[3], ic = r1 % dims[2], r2 = r1 / dims[2],
+                  ibb = r2 % dims[1], ia = r2 / dims[1];
+        packed_store(b, eo.start, eo.row0, out, sh[0], ia, sh[1], ibb, sh[2], ic, sh[3], id,
+                     c0[e]);
+      }
+      __syncthreads();
+      continue;
+    }
     int seen[8][4];
     int nseen = 0;
     for (int pi = 0; pi < 8; ++pi) {
@@ -690,7 +701,7 @@ hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, cons
 }
 
 hipError_t launch_eri_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
-                          const RysDev &r, const QuartetSet &qs, double *out, double *scratch,
+                          const RysDev &r, const QuartetSet &qs, const EriOut &o, double *scratch,
                           hipStream_t s) {
   if (qs.total <= 0) return hipSuccess;
   const size_t slot = 2 * size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
@@ -699,7 +710,7 @@ hipError_t launch_eri_big(int la, int lb, int lc, int ld, const BasisDev &b, con
   root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
   double *slots = big_prepare(la, lb, lc, ld, scratch, s);
   hipLaunchKernelGGL(eri_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la,
-                     lb, lc, ld, rb, b, p, r, qs, out,
+                     lb, lc, ld, rb, b, p, r, qs, o,
                      reinterpret_cast<const unsigned long long *>(scratch), slots, slot);
   return hipGetLastError();
 }

@@ -21,6 +21,44 @@ __device__ __forceinline__ size_t jk_copy_offset(const DensDev &d) {
 // J'/K' accumulation (device-scope FP64 atomic add).
 __device__ __forceinline__ void jk_add(const DensDev &, double *p, double v) { atomicAdd(p, v); }
 
+// Packed storage of one function-space element (ab|cd) of shells (sA, sB |
+// sC, sD) with function indices fa..fd (packed_eri.rs:10-21 layout): each
+// AO pair is put in canonical order (shell i >= j; inside one shell a >= b),
+// numbered start[i(i+1)/2 + j] + (a(a+1)/2 + b on a diagonal shell pair,
+// a*n_j + b otherwise), and (t|u) stored at t(t+1)/2 + u with u <= t.
+// Where a block holds one packed integral twice -- (ab| and (ba| of one
+// shell, or (t|u) and (u|t) of one shell pair -- only the copy the
+// reference stores is written (a >= b, c >= d, u <= t; store_block,
+// packed_eri.rs:300-330), so the result is exactly that element of the block.
+__device__ __forceinline__ long long packed_ao_pair(const BasisDev &b, const long long *start,
+                                                    int si, int a, int sj, int bb) {
+  if (si < sj || (si == sj && a < bb)) {
+    const int s = si, x = a;
+    si = sj;
+    a = bb;
+    sj = s;
+    bb = x;
+  }
+  const int nj = b.kind[sj] ? 2 * b.l[sj] + 1 : ncart(b.l[sj]);
+  return start[(long long)si * (si + 1) / 2 + sj] +
+         (si == sj ? (long long)a * (a + 1) / 2 + bb : (long long)a * nj + bb);
+}
+__device__ __forceinline__ void packed_store(const BasisDev &b, const long long *start,
+                                             long long row0, double *out, int sA, int fa, int sB,
+                                             int fb, int sC, int fc, int sD, int fd, double v) {
+  if ((sA == sB && fa < fb) || (sC == sD && fc < fd)) return;
+  long long t = packed_ao_pair(b, start, sA, fa, sB, fb);
+  long long u = packed_ao_pair(b, start, sC, fc, sD, fd);
+  const bool same_pair = (sA == sC && sB == sD) || (sA == sD && sB == sC);
+  if (same_pair && t < u) return;
+  if (t < u) {
+    const long long x = t;
+    t = u;
+    u = x;
+  }
+  out[t * (t + 1) / 2 + u - row0] = v;
+}
+
 // Lanes of one team always sit in one wavefront, so ordering LDS traffic
 // between phases only needs a wavefront-scope fence plus a scheduling barrier.
 __device__ __forceinline__ void team_sync() {

@@ -211,8 +211,18 @@ hipError_t launch_jk_batch_f32_fcls(int la, int lb, int lc, int ld, const BasisD
                                     long long total, unsigned *queues, int variant,
                                     size_t acc_doubles, unsigned long long *counters,
                                     hipStream_t s);
+// Output of the ERI kernels: the dense nao^4 tensor (every distinct
+// permutation of each canonical quartet) or, with `start` set, the packed
+// symmetry-unique triangle over shell-pair-blocked AO pairs (packed_eri.rs:
+// 10-21): start[i(i+1)/2+j] is the first AO-pair index of canonical shell
+// pair (i, j); element (t|u), u <= t, goes to t(t+1)/2 + u - row0.
+struct EriOut {
+  double *out;
+  const long long *start;  // null: dense
+  long long row0;
+};
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
-                      const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s);
+                      const RysDev &r, const QuartetSet &qs, const EriOut &o, hipStream_t s);
 // lane-per-quartet adjoint gradient (ig_grad_lane.hip); natom sizes its LDS
 // copy of the gradient
 hipError_t launch_grad_lane(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
@@ -230,7 +240,7 @@ hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, cons
                          const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
                          double *scratch, unsigned long long *counters, hipStream_t s);
 hipError_t launch_eri_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
-                          const RysDev &r, const QuartetSet &qs, double *out, double *scratch,
+                          const RysDev &r, const QuartetSet &qs, const EriOut &o, double *scratch,
                           hipStream_t s);
 hipError_t launch_grad_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                            const RysDev &r, const QuartetSet &qs, const GradDev &g,

@@ -150,7 +150,8 @@ __device__ __constant__ int kPerms8[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 
 
 template <class C>
 __global__ void __launch_bounds__(64) eri_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs,
-                                                 double *out) {
+                                                 EriOut eo) {
+  double *out = eo.out;
   constexpr int G = C::G, PER = cmax(C::WORK, 2 * C::N4);
   __shared__ double lds[C::TEAMS * PER];
   const int lane = threadIdx.x, team = lane / G, tl = lane % G;
@@ -197,6 +198,16 @@ __global__ void __launch_bounds__(64) eri_kernel(BasisDev b, PairsDev p, RysDev 
       team_sync();
     }
     const int nf = dims[0] * dims[1] * dims[2] * dims[3];
+    if (eo.start) {  // packed: the canonical element only
+      for (int e = tl; e < nf; e += G) {
+        const int id = e % dims[3], r1 = e / dims[3], ic = r1 % dims[2], r2 = r1 / dims[2],
+                  ibb = r2 % dims[1], ia = r2 / dims[1];
+        packed_store(b, eo.start, eo.row0, out, sh[0], ia, sh[1], ibb, sh[2], ic, sh[3], id,
+                     cur[e]);
+      }
+      team_sync();
+      continue;
+    }
     int offf[4];
     for (int x = 0; x < 4; ++x) offf[x] = b.off_func[sh[x]];
     int seen[8][4];
@@ -259,11 +270,11 @@ hipError_t jk_team_one(const BasisDev &b, const PairsDev &p, const RysDev &r, co
 
 template <int LA, int LB, int LC, int LD>
 hipError_t eri_one(const BasisDev &b, const PairsDev &p, const RysDev &r, const QuartetSet &qs,
-                   double *out, hipStream_t s) {
+                   const EriOut &o, hipStream_t s) {
   using C = Cls<LA, LB, LC, LD>;
   if (qs.total <= 0) return hipSuccess;
   hipLaunchKernelGGL(eri_kernel<C>, dim3(grid_for<LA, LB, LC, LD>(qs.total)), dim3(64), 0, s, b,
-                     p, r, qs, out);
+                     p, r, qs, o);
   return hipGetLastError();
 }
 
@@ -332,9 +343,9 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, const BasisDev &b, const Pa
 }
 
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
-                      const RysDev &r, const QuartetSet &qs, double *out, hipStream_t s) {
+                      const RysDev &r, const QuartetSet &qs, const EriOut &o, hipStream_t s) {
 #define IG_CASE(A, B, Cc, D) \
-  if (la == A && lb == B && lc == Cc && ld == D) return eri_one<A, B, Cc, D>(b, p, r, qs, out, s);
+  if (la == A && lb == B && lc == Cc && ld == D) return eri_one<A, B, Cc, D>(b, p, r, qs, o, s);
   IG_FOR_CLASSES(IG_CASE)
 #undef IG_CASE
   return hipErrorInvalidValue;

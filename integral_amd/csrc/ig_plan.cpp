@@ -1322,17 +1322,17 @@ static int eri_impl(const ig_basis *b, double tau, double *out, size_t len, int6
     for (auto &cp : P->cached_cps)
       big = std::max(big, big_need(P->cls[cp.x], P->cls[cp.y], cp.total, 1));
     P->s_big.reserve(big * sizeof(double));
+    const EriOut eo{dout.as<double>(), nullptr, 0};
     for (auto &cp : P->cached_cps) {
       if (!cp.total) continue;
       QuartetSet qs = implicit_set(*P, cp);
       const ClassRange &X = P->cls[cp.x], &Y = P->cls[cp.y];
       if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
-        check(launch_eri(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, dout.as<double>(),
-                         s),
+        check(launch_eri(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, eo, s),
               "eri launch");
       else
-        check(launch_eri_big(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs,
-                             dout.as<double>(), P->s_big.as<double>(), s),
+        check(launch_eri_big(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, eo,
+                             P->s_big.as<double>(), s),
               "eri launch");
     }
     check(hipMemcpyAsync(out, dout.p, len * sizeof(double), hipMemcpyDeviceToHost, s), "copy eri");
@@ -1363,6 +1363,153 @@ extern "C" int ig_eri_screened(const ig_basis *b, int engine, double tau, double
   if (!b || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
   if (int rc = check_engine(engine)) return rc;
   return eri_impl(b, tau, out, len, stats, stream);
+}
+
+// ---------------------------------------------------------------------------
+// packed symmetry-unique ERIs: PackedEriBuilder (packed_eri.rs:67-240)
+namespace {
+
+int shell_nfunc(const HostShell &s) { return s.nfunc(); }
+size_t tri_count(size_t n) { return n * (n + 1) / 2; }
+
+// First AO-pair index of every canonical shell pair (i >= j, row-major), then
+// the total (packed_eri.rs:99-110): inside one pair the function pairs (a, b)
+// row-major, b <= a on a diagonal pair.
+std::vector<long long> packed_pair_start(const ig_basis &b) {
+  const int nsh = int(b.shells.size());
+  std::vector<long long> start;
+  start.reserve(size_t(nsh) * (nsh + 1) / 2 + 1);
+  long long acc = 0;
+  for (int i = 0; i < nsh; ++i)
+    for (int j = 0; j <= i; ++j) {
+      start.push_back(acc);
+      const long long ni = shell_nfunc(b.shells[i]), nj = shell_nfunc(b.shells[j]);
+      acc += i == j ? ni * (ni + 1) / 2 : ni * nj;
+    }
+  start.push_back(acc);
+  return start;
+}
+
+// Rows of canonical bra pairs [b0, b1) of the packed triangle into `rows`
+// (len = its length), quartets with Q_ij Q_kl < tau skipped (tau > 0).
+int packed_impl(const ig_basis *b, double tau, int64_t b0, int64_t b1, double *rows, size_t len,
+                void *stream) {
+  const std::vector<long long> start = packed_pair_start(*b);
+  const int64_t np = int64_t(start.size()) - 1;
+  if (b1 < 0) b1 = np;
+  if (b0 < 0 || b0 > b1 || b1 > np) return fail(IG_ERR_INVALID_ARGUMENT, "bra pair range");
+  const size_t row0 = tri_count(size_t(start[b0])), row1 = tri_count(size_t(start[b1]));
+  if (len != row1 - row0) return fail(IG_ERR_LENGTH_MISMATCH, "packed ERI rows length");
+  if (!(tau >= 0.0) || std::isinf(tau)) return fail(IG_ERR_INVALID_ARGUMENT, "threshold");
+  return guarded([&] {
+    hipStream_t s = as_stream(stream);
+    auto P = build_plan(*b, s);
+    DBuf dstart, dout;
+    dstart.upload(start);
+    dout.reserve(std::max<size_t>(len, 1) * sizeof(double));
+    if (len) check(launch_fill(dout.as<double>(), len, 0.0, s), "zero packed eri");
+    const EriOut eo{dout.as<double>(), dstart.as<long long>(), (long long)row0};
+    RysDevice &R = rys_device();
+    auto launch = [&](const ClassRange &X, const ClassRange &Y, const QuartetSet &qs) {
+      if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
+        check(launch_eri(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, eo, s),
+              "packed eri launch");
+      else
+        check(launch_eri_big(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, eo,
+                             P->s_big.as<double>(), s),
+              "packed eri launch");
+    };
+    {  // explicit lists of the bra range, kets <= bra (packed_eri.rs:183): a
+       // whole build and the fills of its parts evaluate every quartet in the
+       // same orientation, so they agree bit for bit (packed_eri.rs:222-230)
+      ExplicitSets ex = explicit_sets(*P, b0, b1, tau);
+      std::vector<int2> flat;
+      for (auto &l : ex.lists) flat.insert(flat.end(), l.begin(), l.end());
+      P->s_list.upload(flat);
+      size_t big = 0;
+      for (size_t c = 0; c < ex.cls.size(); ++c)
+        big = std::max(big, big_need(P->cls[ex.cls[c].first], P->cls[ex.cls[c].second],
+                                     (long long)ex.lists[c].size(), 1));
+      P->s_big.reserve(big * sizeof(double));
+      size_t off = 0;
+      for (size_t c = 0; c < ex.cls.size(); ++c) {
+        const ClassRange &X = P->cls[ex.cls[c].first], &Y = P->cls[ex.cls[c].second];
+        QuartetSet qs{};
+        qs.x_off = X.off;
+        qs.x_n = X.n;
+        qs.y_off = Y.off;
+        qs.y_n = Y.n;
+        qs.same = ex.cls[c].first == ex.cls[c].second;
+        qs.list = P->s_list.as<int2>() + off;
+        qs.total = (long long)ex.lists[c].size();
+        off += ex.lists[c].size();
+        launch(X, Y, qs);
+      }
+    }
+    if (len)
+      check(hipMemcpyAsync(rows, dout.p, len * sizeof(double), hipMemcpyDeviceToHost, s),
+            "copy packed eri");
+    check(hipStreamSynchronize(s), "packed eri sync");
+    return int(IG_OK);
+  });
+}
+
+}  // namespace
+
+extern "C" size_t ig_packed_eri_n_ao_pairs(const ig_basis *b) {
+  return b ? size_t(packed_pair_start(*b).back()) : 0;
+}
+
+extern "C" size_t ig_packed_eri_output_len(const ig_basis *b) {
+  return b ? tri_count(ig_packed_eri_n_ao_pairs(b)) : 0;
+}
+
+extern "C" int ig_packed_eri_ao_pairs(const ig_basis *b, int32_t *pairs, size_t n) {
+  if (!b || (n && !pairs)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  const int nsh = int(b->shells.size());
+  size_t k = 0;
+  for (int i = 0; i < nsh; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const HostShell &si = b->shells[i], &sj = b->shells[j];
+      for (int a = 0; a < si.nfunc(); ++a)
+        for (int c = 0; c < (i == j ? a + 1 : sj.nfunc()); ++c) {
+          if (k < n) {
+            pairs[2 * k] = si.off_func + a;
+            pairs[2 * k + 1] = sj.off_func + c;
+          }
+          ++k;
+        }
+    }
+  if (k != n) return fail(IG_ERR_LENGTH_MISMATCH, "ao pair count is " + std::to_string(k));
+  return IG_OK;
+}
+
+extern "C" int ig_packed_eri_rows(const ig_basis *b, int64_t bra_begin, int64_t bra_end,
+                                  size_t *offset, size_t *len) {
+  if (!b || !offset || !len) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  const std::vector<long long> start = packed_pair_start(*b);
+  const int64_t np = int64_t(start.size()) - 1;
+  if (bra_end < 0) bra_end = np;
+  if (bra_begin < 0 || bra_begin > bra_end || bra_end > np)
+    return fail(IG_ERR_INVALID_ARGUMENT, "bra pair range");
+  *offset = tri_count(size_t(start[bra_begin]));
+  *len = tri_count(size_t(start[bra_end])) - *offset;
+  return IG_OK;
+}
+
+extern "C" int ig_packed_eri_fill(const ig_basis *b, int engine, int64_t bra_begin,
+                                  int64_t bra_end, double threshold, double *rows, size_t len,
+                                  void *stream) {
+  if (!b || (len && !rows)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  return packed_impl(b, threshold, bra_begin, bra_end, rows, len, stream);
+}
+
+extern "C" int ig_packed_eri_build(const ig_basis *b, int engine, double threshold, double *out,
+                                   size_t len, void *stream) {
+  if (!b || (len && !out)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  return packed_impl(b, threshold, 0, -1, out, len, stream);
 }
 
 // ---------------------------------------------------------------------------

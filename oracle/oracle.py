@@ -153,6 +153,54 @@ class Basis:
         f(self._h, engine, i, j, k, l, _ptr(out))
         return out.reshape(dims)
 
+    def packed_ao_pairs(self):
+        """Shell-pair-blocked AO pairs (p, q), p >= q (packed_eri.rs:99-110):
+        canonical shell pairs (i >= j) row-major, inside one pair (a, b)
+        row-major with b <= a when i == j.  Returns (pairs, pair_start)."""
+        off = np.concatenate([[0], np.cumsum([self.nfunc(s) for s in range(self.nsh)])])
+        pairs, start = [], []
+        for i in range(self.nsh):
+            for j in range(i + 1):
+                start.append(len(pairs))
+                for a in range(self.nfunc(i)):
+                    for b in range(a + 1 if i == j else self.nfunc(j)):
+                        pairs.append((int(off[i] + a), int(off[j] + b)))
+        start.append(len(pairs))
+        return pairs, start
+
+    def packed_eri(self, threshold=0.0, engine=ENGINE_AUTO):
+        """PackedEriBuilder::build (packed_eri.rs:173-230, store_block
+        :300-330): every canonical quartet (ket pair <= bra pair) evaluated
+        once and stored at t(t+1)/2 + u; with threshold > 0, quartets with
+        Q_ij Q_kl < threshold are skipped (left zero)."""
+        pairs, start = self.packed_ao_pairs()
+        m = len(pairs)
+        out = np.zeros(m * (m + 1) // 2)
+        canon = [(i, j) for i in range(self.nsh) for j in range(i + 1)]
+        q = None
+        if threshold > 0.0:
+            qm = self.schwarz(engine).reshape(self.nsh, self.nsh)
+            q = [qm[i, j] for i, j in canon]
+        for bra, (i, j) in enumerate(canon):
+            for ket in range(bra + 1):
+                k, l = canon[ket]
+                if q is not None and q[bra] * q[ket] < threshold:
+                    continue
+                blk = self.eri_block(i, j, k, l, engine)
+                na, nb, nc, nd = blk.shape
+                t = start[bra]
+                for a in range(na):
+                    for b in range(a + 1 if i == j else nb):
+                        u = start[ket]
+                        for c in range(nc):
+                            for d in range(c + 1 if k == l else nd):
+                                if u > t:
+                                    break
+                                out[t * (t + 1) // 2 + u] = blk[a, b, c, d]
+                                u += 1
+                        t += 1
+        return out
+
     def eri(self, engine=ENGINE_AUTO, threads=None):
         n = self.nao
         out = np.zeros(n ** 4)
