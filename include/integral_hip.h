@@ -108,6 +108,16 @@ int ig_eri_dense(const ig_basis *basis, int engine, double *out, size_t out_len,
  * stats[1]=skipped (may be NULL). */
 int ig_eri_screened(const ig_basis *basis, int engine, double tau, double *out, size_t out_len,
                     int64_t *stats, void *stream);
+/* EriBuilder::fill (eri_builder.rs:337-454) for the canonical bra pairs
+ * [bra_begin, bra_end) (bra_end < 0: all): out is the caller's nao^4 host
+ * tensor (zeroed, as EriBuilder::partition requires); only the rows (mu,nu)
+ * those bra pairs own -- mu in shell i and nu in shell j, or the reverse --
+ * are written, every element equal to Basis::eri's to round-off (the
+ * reference's contract, eri_builder.rs:419-446).  Filling every range gives
+ * the whole tensor; EriBuilder::build is ig_eri_dense.  Coulomb operator
+ * only (EriBuilder::kernel(Erf) has no GPU counterpart here). */
+int ig_eri_builder_fill(const ig_basis *basis, int engine, int64_t bra_begin, int64_t bra_end,
+                        double *out, size_t out_len, void *stream);
 /* Basis::schwarz_bounds_with, integrals.rs:842-862: nshell^2 doubles, host. */
 int ig_schwarz_bounds(const ig_basis *basis, int engine, double *out, size_t out_len,
                       void *stream);

@@ -196,6 +196,7 @@ def library():
     lib.ig_direct_schedule_read.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz,
                                             ctypes.POINTER(sz)]
     lib.ig_direct_schedule_write.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz]
+    lib.ig_eri_builder_fill.argtypes = [_vp, ctypes.c_int, i64, i64, _dp, sz, _vp]
     lib.ig_packed_eri_n_ao_pairs.argtypes = [_vp]
     lib.ig_packed_eri_n_ao_pairs.restype = sz
     lib.ig_packed_eri_output_len.argtypes = [_vp]
@@ -401,8 +402,72 @@ class Basis:
     def packed_eri_builder(self):
         return PackedEriBuilder(self)
 
+    def eri_builder(self):
+        return EriBuilder(self)
+
     def direct_grad_contractor(self):
         return DirectGradContractor(self)
+
+
+class BraPairFill:
+    """eri_builder.rs BraPairFill: one canonical bra pair's task over the
+    caller's nao^4 buffer (the GPU fill writes only the rows it owns)."""
+
+    def __init__(self, bra, out):
+        self._bra = bra
+        self.out = out
+
+    def bra(self):
+        return self._bra
+
+
+class EriBuilder:
+    """EriBuilder (eri_builder.rs:101-454): the dense nao^4 tensor filled by
+    canonical bra-pair tasks that own disjoint output rows."""
+
+    def __init__(self, basis, engine=Engine.Auto):
+        self._basis = basis
+        self._engine = int(engine)
+        nsh = len(basis._shells)
+        self._pairs = [(i, j) for i in range(nsh) for j in range(i + 1)]
+
+    @classmethod
+    def with_engine(cls, basis, engine):
+        return cls(basis, engine)
+
+    def kernel(self, k):
+        if k.omega is not None:
+            raise UnsupportedError("EriBuilder with the erf kernel has no GPU path in this build")
+        return self
+
+    def bra_pairs(self):
+        return list(self._pairs)
+
+    def output_len(self):
+        return self._basis.nao() ** 4
+
+    def _check(self, out):
+        if not (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous
+                and out.size == self.output_len()):
+            raise UsageError("ERI output buffer must hold %d elements" % self.output_len())
+
+    def partition(self, out):
+        """One task per canonical bra pair (eri_builder.rs:337-399)."""
+        self._check(out)
+        return [BraPairFill(k, out) for k in range(len(self._pairs))]
+
+    def fill(self, task, stream=None):
+        self.fill_range(task.bra(), task.bra() + 1, task.out, stream)
+
+    def fill_range(self, bra_begin, bra_end, out, stream=None):
+        """The rows of the canonical bra pairs [bra_begin, bra_end) at once."""
+        self._check(out)
+        _raise(library().ig_eri_builder_fill(self._basis._h, self._engine, int(bra_begin),
+                                             int(bra_end), _ptr(out), out.size,
+                                             _stream_ptr(stream)))
+
+    def build(self):
+        return self._basis.eri_with(self._engine)
 
 
 class PackedEriWorkspace:

@@ -436,6 +436,11 @@ __global__ void __launch_bounds__(kBigThreads)
       if (dup) continue;
       for (int m = 0; m < 4; ++m) seen[nseen][m] = tup[m];
       ++nseen;
+      if (eo.rows_end >= 0) {  // rows owned by the bra-pair range only
+        const long long hi = tup[0] > tup[1] ? tup[0] : tup[1], lo = tup[0] + tup[1] - hi;
+        const long long cp = hi * (hi + 1) / 2 + lo;
+        if (cp < eo.rows_begin || cp >= eo.rows_end) continue;
+      }
       // position m of the permuted index takes source axis kBigPerms8[pi][m]
       size_t stride[4], base = 0;
       for (int m = 0; m < 4; ++m) {

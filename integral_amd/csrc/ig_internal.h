@@ -216,10 +216,14 @@ hipError_t launch_jk_batch_f32_fcls(int la, int lb, int lc, int ld, const BasisD
 // symmetry-unique triangle over shell-pair-blocked AO pairs (packed_eri.rs:
 // 10-21): start[i(i+1)/2+j] is the first AO-pair index of canonical shell
 // pair (i, j); element (t|u), u <= t, goes to t(t+1)/2 + u - row0.
+// With rows_end >= 0 the dense scatter only writes the permutations whose
+// first index pair (the output row) belongs to a canonical shell pair in
+// [rows_begin, rows_end): the rows an EriBuilder bra-pair range owns.
 struct EriOut {
   double *out;
   const long long *start;  // null: dense
   long long row0;
+  long long rows_begin = 0, rows_end = -1;
 };
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                       const RysDev &r, const QuartetSet &qs, const EriOut &o, hipStream_t s);

@@ -222,6 +222,11 @@ __global__ void __launch_bounds__(64) eri_kernel(BasisDev b, PairsDev p, RysDev 
       if (dup) continue;
       for (int q = 0; q < 4; ++q) seen[nseen][q] = tup[q];
       ++nseen;
+      if (eo.rows_end >= 0) {  // rows owned by the bra-pair range only
+        const long long hi = tup[0] > tup[1] ? tup[0] : tup[1], lo = tup[0] + tup[1] - hi;
+        const long long cp = hi * (hi + 1) / 2 + lo;
+        if (cp < eo.rows_begin || cp >= eo.rows_end) continue;
+      }
       size_t stride[4];
       size_t base = 0;
       for (int q = 0; q < 4; ++q) {

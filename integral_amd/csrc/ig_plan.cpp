@@ -1456,6 +1456,110 @@ int packed_impl(const ig_basis *b, double tau, int64_t b0, int64_t b1, double *r
 
 }  // namespace
 
+// EriBuilder fill (eri_builder.rs:337-454): the dense rows owned by the
+// canonical bra pairs [b0, b1).  Each unordered pair of shell pairs {P, Q}
+// with P in the range is evaluated once -- Q <= P, or Q above the range --
+// and its block written to every permutation slot whose row pair is owned.
+static int eri_builder_impl(const ig_basis *b, int64_t b0, int64_t b1, double *out, size_t len,
+                            void *stream) {
+  const size_t nao = size_t(b->nao);
+  if (len != nao * nao * nao * nao) return fail(IG_ERR_LENGTH_MISMATCH, "eri length");
+  const int nsh = int(b->shells.size());
+  const int64_t np = int64_t(nsh) * (nsh + 1) / 2;
+  if (b1 < 0) b1 = np;
+  if (b0 < 0 || b0 > b1 || b1 > np) return fail(IG_ERR_INVALID_ARGUMENT, "bra pair range");
+  return guarded([&] {
+    hipStream_t s = as_stream(stream);
+    auto P = build_plan(*b, s);
+    DBuf dout;
+    dout.reserve(len * sizeof(double));
+    check(launch_fill(dout.as<double>(), len, 0.0, s), "zero eri");
+    // quartet lists per (class, class), oriented as the kernels need
+    int index[kNumPairClass][kNumPairClass];
+    for (auto &r : index)
+      for (int &v : r) v = -1;
+    std::vector<std::pair<int, int>> cls;
+    std::vector<std::vector<int2>> lists;
+    for (int64_t bc = b0; bc < b1; ++bc) {
+      int i, j;
+      pair_of(bc, i, j);
+      if (P->schwarz[size_t(i) * nsh + j] == 0.0) continue;  // no primitive pairs: zero
+      const int pb = P->canon_to_pos[bc];
+      for (int64_t kc = 0; kc < np; ++kc) {
+        if (kc > bc && kc < b1) continue;  // evaluated with kc as the bra
+        int k, l;
+        pair_of(kc, k, l);
+        if (P->schwarz[size_t(k) * nsh + l] == 0.0) continue;
+        const int pk = P->canon_to_pos[kc];
+        const int cb = pair_class(P->sh_l[P->p_sh1[pb]], P->sh_l[P->p_sh2[pb]]);
+        const int ck = pair_class(P->sh_l[P->p_sh1[pk]], P->sh_l[P->p_sh2[pk]]);
+        int x = cb, y = ck, px = pb, py = pk;
+        if (cb < ck) {
+          std::swap(x, y);
+          std::swap(px, py);
+        }
+        if (index[x][y] < 0) {
+          index[x][y] = int(cls.size());
+          cls.push_back({x, y});
+          lists.emplace_back();
+        }
+        lists[index[x][y]].push_back(int2{px - P->cls[x].off, py - P->cls[y].off});
+      }
+    }
+    std::vector<int2> flat;
+    for (auto &l : lists) flat.insert(flat.end(), l.begin(), l.end());
+    P->s_list.upload(flat);
+    size_t big = 0;
+    for (size_t c = 0; c < cls.size(); ++c)
+      big = std::max(big, big_need(P->cls[cls[c].first], P->cls[cls[c].second],
+                                   (long long)lists[c].size(), 1));
+    P->s_big.reserve(big * sizeof(double));
+    EriOut eo{dout.as<double>(), nullptr, 0};
+    eo.rows_begin = b0;
+    eo.rows_end = b1;
+    RysDevice &R = rys_device();
+    size_t off = 0;
+    for (size_t c = 0; c < cls.size(); ++c) {
+      const ClassRange &X = P->cls[cls[c].first], &Y = P->cls[cls[c].second];
+      QuartetSet qs{};
+      qs.x_off = X.off;
+      qs.x_n = X.n;
+      qs.y_off = Y.off;
+      qs.y_n = Y.n;
+      qs.same = cls[c].first == cls[c].second;
+      qs.list = P->s_list.as<int2>() + off;
+      qs.total = (long long)lists[c].size();
+      off += lists[c].size();
+      if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
+        check(launch_eri(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, eo, s),
+              "eri builder launch");
+      else
+        check(launch_eri_big(X.l1, X.l2, Y.l1, Y.l2, P->bdev, P->pdev, R.view, qs, eo,
+                             P->s_big.as<double>(), s),
+              "eri builder launch");
+    }
+    // owned rows back to the caller: for (i, j), rows (mu in i, nu in j) and
+    // (nu, mu); for fixed mu the rows over nu are one contiguous run
+    const size_t n2 = nao * nao;
+    for (int64_t bc = b0; bc < b1; ++bc) {
+      int i, j;
+      pair_of(bc, i, j);
+      const HostShell &si = b->shells[i], &sj = b->shells[j];
+      for (int pass = 0; pass < (i == j ? 1 : 2); ++pass) {
+        const HostShell &r = pass ? sj : si, &c = pass ? si : sj;
+        for (int a = 0; a < r.nfunc(); ++a) {
+          const size_t first = (size_t(r.off_func + a) * nao + c.off_func) * n2;
+          check(hipMemcpyAsync(out + first, dout.as<double>() + first,
+                               size_t(c.nfunc()) * n2 * sizeof(double), hipMemcpyDeviceToHost, s),
+                "copy eri rows");
+        }
+      }
+    }
+    check(hipStreamSynchronize(s), "eri builder sync");
+    return int(IG_OK);
+  });
+}
+
 extern "C" size_t ig_packed_eri_n_ao_pairs(const ig_basis *b) {
   return b ? size_t(packed_pair_start(*b).back()) : 0;
 }
@@ -1503,6 +1607,13 @@ extern "C" int ig_packed_eri_fill(const ig_basis *b, int engine, int64_t bra_beg
   if (!b || (len && !rows)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
   if (int rc = check_engine(engine)) return rc;
   return packed_impl(b, threshold, bra_begin, bra_end, rows, len, stream);
+}
+
+extern "C" int ig_eri_builder_fill(const ig_basis *b, int engine, int64_t bra_begin,
+                                   int64_t bra_end, double *out, size_t len, void *stream) {
+  if (!b || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  return eri_builder_impl(b, bra_begin, bra_end, out, len, stream);
 }
 
 extern "C" int ig_packed_eri_build(const ig_basis *b, int engine, double threshold, double *out,

@@ -106,3 +106,39 @@ def test_packed_spdf_molecule_and_bad_arguments():
         pb.partition(np.zeros(3))
     with pytest.raises(ig.IntegralError):
         pb.fill_range(5, 2, np.zeros(1))
+
+
+@pytest.mark.parametrize("sph", [False, True])
+def test_eri_builder_fills_own_rows(sph):
+    """EriBuilder (eri_builder.rs): tasks filled in reverse order assemble
+    the eri() tensor to round-off, and one task writes only the rows of its
+    bra pair (mu in i, nu in j, or the reverse)."""
+    shells = ref_basis(sph)
+    note("eri builder sph=%s" % sph)
+    basis = synthetic.to_basis(shells)
+    eb = basis.eri_builder()
+    dense = basis.eri()
+    assert np.max(np.abs(dense - O.Basis(shells).eri())) < TOL
+    out = np.zeros(eb.output_len())
+    tasks = eb.partition(out)
+    assert len(tasks) == len(eb.bra_pairs())
+    for task in reversed(tasks):
+        eb.fill(task)
+    assert np.max(np.abs(out - dense)) < 1e-14
+    n = basis.nao()
+    offs = np.concatenate([[0], np.cumsum([2 * s[0] + 1 if sph else (s[0] + 1) * (s[0] + 2) // 2
+                                           for s in shells])])
+    for k in (0, 4, len(tasks) - 1):
+        i, j = eb.bra_pairs()[k]
+        one = np.zeros(eb.output_len())
+        eb.fill(eb.partition(one)[k])
+        t = one.reshape(n, n, n, n)
+        own = np.zeros((n, n), dtype=bool)
+        own[offs[i]:offs[i + 1], offs[j]:offs[j + 1]] = True
+        own[offs[j]:offs[j + 1], offs[i]:offs[i + 1]] = True
+        assert np.all(t[~own] == 0.0)
+        assert np.max(np.abs(t[own] - dense.reshape(n, n, n, n)[own])) < 1e-14
+    with pytest.raises(ig.UsageError):
+        eb.partition(np.zeros(5))
+    with pytest.raises(ig.UnsupportedError):
+        eb.kernel(ig.EriKernel.erf(0.4))
