@@ -122,6 +122,24 @@ int ig_eri_builder_fill(const ig_basis *basis, int engine, int64_t bra_begin, in
 int ig_schwarz_bounds(const ig_basis *basis, int engine, double *out, size_t out_len,
                       void *stream);
 
+/* ---- density fitting (df.rs): 2- and 3-center Coulomb integrals ----------
+ * Evaluated by the 4-center kernels with a unit-s dummy (zero exponent,
+ * coefficient 1: the constant function) on the aux index (df.rs:1-39). */
+/* Basis::eri_2c_with (df.rs:56-90): the [naux, naux] metric over `aux`,
+ * exactly symmetric (each canonical aux shell pair once, mirrored). */
+int ig_eri_2c(const ig_basis *aux, int engine, double *out, size_t len, void *stream);
+/* Basis::schwarz_aux_bounds_with (df.rs:257-280): sqrt(max |(P|P)|) per aux
+ * shell, len = number of aux shells. */
+int ig_schwarz_aux_bounds(const ig_basis *aux, int engine, double *out, size_t len,
+                          void *stream);
+/* Eri3cBuilder::fill (df.rs:463-512) for the canonical main-basis bra pairs
+ * [bra_begin, bra_end) (bra_end < 0: all, i.e. Eri3cBuilder::build): out is
+ * the caller's [nao, nao, naux] host tensor, P fastest (zeroed, as
+ * Eri3cBuilder::partition requires); the rows (mu in i, nu in j) and, for
+ * i != j, (nu, mu) of those pairs are written. */
+int ig_eri_3c_fill(const ig_basis *main, const ig_basis *aux, int engine, int64_t bra_begin,
+                   int64_t bra_end, double *out, size_t len, void *stream);
+
 /* ---- packed symmetry-unique ERIs: PackedEriBuilder (packed_eri.rs) -------
  * AO pairs (p, q), p >= q, numbered shell-pair blocked: canonical shell pairs
  * (i >= j) row-major, inside a pair the function pairs (a, b) row-major with

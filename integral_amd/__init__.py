@@ -197,6 +197,9 @@ def library():
                                             ctypes.POINTER(sz)]
     lib.ig_direct_schedule_write.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz]
     lib.ig_eri_builder_fill.argtypes = [_vp, ctypes.c_int, i64, i64, _dp, sz, _vp]
+    lib.ig_eri_2c.argtypes = [_vp, ctypes.c_int, _dp, sz, _vp]
+    lib.ig_schwarz_aux_bounds.argtypes = [_vp, ctypes.c_int, _dp, sz, _vp]
+    lib.ig_eri_3c_fill.argtypes = [_vp, _vp, ctypes.c_int, i64, i64, _dp, sz, _vp]
     lib.ig_packed_eri_n_ao_pairs.argtypes = [_vp]
     lib.ig_packed_eri_n_ao_pairs.restype = sz
     lib.ig_packed_eri_output_len.argtypes = [_vp]
@@ -405,6 +408,43 @@ class Basis:
     def eri_builder(self):
         return EriBuilder(self)
 
+    # density fitting (df.rs): self as the auxiliary basis for 2c, as the
+    # main basis for 3c
+    def eri_2c_with(self, engine):
+        n = self.nao()
+        out = np.zeros(n * n)
+        _raise(library().ig_eri_2c(self._h, int(engine), _ptr(out), out.size, None))
+        return out
+
+    def eri_2c(self):
+        return self.eri_2c_with(Engine.Auto)
+
+    def schwarz_aux_bounds_with(self, engine):
+        out = np.zeros(len(self._shells))
+        _raise(library().ig_schwarz_aux_bounds(self._h, int(engine), _ptr(out), out.size, None))
+        return out
+
+    def schwarz_aux_bounds(self):
+        return self.schwarz_aux_bounds_with(Engine.Auto)
+
+    def eri_3c_builder(self, aux):
+        return Eri3cBuilder(self, aux)
+
+    def eri_3c_block_with(self, engine, aux, ish, jsh, psh):
+        """One (ij|P) block, row-major [n_i, n_j, n_p] (df.rs:148-199)."""
+        b = Eri3cBuilder(self, aux, engine)
+        hi, lo = (ish, jsh) if ish >= jsh else (jsh, ish)
+        out = np.zeros(b.output_len())
+        b.fill_range(hi * (hi + 1) // 2 + lo, hi * (hi + 1) // 2 + lo + 1, out)
+        n, naux = self.nao(), aux.nao()
+        off = np.concatenate([[0], np.cumsum([s.n_func() for s in self._shells])])
+        aoff = np.concatenate([[0], np.cumsum([s.n_func() for s in aux._shells])])
+        t = out.reshape(n, n, naux)
+        return t[off[ish]:off[ish + 1], off[jsh]:off[jsh + 1], aoff[psh]:aoff[psh + 1]].ravel()
+
+    def eri_3c_block(self, aux, ish, jsh, psh):
+        return self.eri_3c_block_with(Engine.Auto, aux, ish, jsh, psh)
+
     def direct_grad_contractor(self):
         return DirectGradContractor(self)
 
@@ -468,6 +508,63 @@ class EriBuilder:
 
     def build(self):
         return self._basis.eri_with(self._engine)
+
+
+class Bra3cFill:
+    """df.rs:531-548: the (mu, nu) rows one canonical bra pair owns."""
+
+    def __init__(self, bra, out):
+        self._bra = bra
+        self.out = out
+
+    def bra(self):
+        return self._bra
+
+
+class Eri3cBuilder:
+    """Eri3cBuilder (df.rs:325-530): the dense [nao, nao, naux] 3-center
+    tensor, P fastest, filled by canonical main-basis bra-pair tasks."""
+
+    def __init__(self, main, aux, engine=Engine.Auto):
+        self._main, self._aux, self._engine = main, aux, int(engine)
+        nsh = len(main._shells)
+        self._pairs = [(i, j) for i in range(nsh) for j in range(i + 1)]
+
+    @classmethod
+    def with_engine(cls, main, aux, engine):
+        return cls(main, aux, engine)
+
+    def bra_pairs(self):
+        return list(self._pairs)
+
+    def output_len(self):
+        return self._main.nao() ** 2 * self._aux.nao()
+
+    def _check(self, out):
+        if not (isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous
+                and out.size == self.output_len()):
+            raise UsageError("3c output buffer must be nao^2*naux = %d elements"
+                             % self.output_len())
+
+    def partition(self, out):
+        self._check(out)
+        return [Bra3cFill(p, out) for p in self._pairs]
+
+    def fill(self, task, stream=None):
+        i, j = task.bra()
+        k = i * (i + 1) // 2 + j
+        self.fill_range(k, k + 1, task.out, stream)
+
+    def fill_range(self, bra_begin, bra_end, out, stream=None):
+        self._check(out)
+        _raise(library().ig_eri_3c_fill(self._main._h, self._aux._h, self._engine,
+                                        int(bra_begin), int(bra_end), _ptr(out), out.size,
+                                        _stream_ptr(stream)))
+
+    def build(self):
+        out = np.zeros(self.output_len())
+        self.fill_range(0, -1, out)
+        return out
 
 
 class PackedEriWorkspace:

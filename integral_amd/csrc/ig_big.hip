@@ -414,6 +414,16 @@ __global__ void __launch_bounds__(kBigThreads)
       __syncthreads();
     }
     const int nf = dims[0] * dims[1] * dims[2] * dims[3];
+    if (eo.mode == kDf3c || eo.mode == kDf2c) {
+      for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+        const int id = e % dims[3], r1 = e / dims[3], ic = r1 % dims[2], r2 = r1 / dims[2],
+                  ibb = r2 % dims[1], ia = r2 / dims[1];
+        const int idx[4] = {ia, ibb, ic, id};
+        df_store(b, eo, sh, idx, c0[e]);
+      }
+      __syncthreads();
+      continue;
+    }
     if (eo.start) {  // packed: the canonical element only
       for (int e = threadIdx.x; e < nf; e += blockDim.x) {
         const int id = e % dims[3], r1 = e / dims[3], ic = r1 % dims[2], r2 = r1 / dims[2],

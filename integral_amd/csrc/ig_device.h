@@ -59,6 +59,28 @@ __device__ __forceinline__ void packed_store(const BasisDev &b, const long long 
   out[t * (t + 1) / 2 + u - row0] = v;
 }
 
+// Density-fitting stores of one function-space element of a (.. | ..) block
+// of the combined basis (modes kDf3c, kDf2c; see EriOut): sh are the four
+// shells, idx the element's function index in each.
+__device__ __forceinline__ void df_store(const BasisDev &b, const EriOut &eo, const int *sh,
+                                         const int *idx, double v) {
+  if (eo.mode == kDf3c) {
+    const int m = (sh[0] < eo.nmain_sh && sh[1] < eo.nmain_sh) ? 0 : 2, x = 2 - m;
+    const int ap = sh[x] >= eo.dummy_sh0 ? x + 1 : x;
+    const long long mu = b.off_func[sh[m]] + idx[m], nu = b.off_func[sh[m + 1]] + idx[m + 1];
+    const long long p = b.off_func[sh[ap]] - eo.nao_m + idx[ap];
+    eo.out[(mu * eo.nao_m + nu) * eo.naux + p] = v;
+    if (sh[m] != sh[m + 1]) eo.out[(nu * eo.nao_m + mu) * eo.naux + p] = v;
+  } else {  // kDf2c: (P, 1s | Q, 1s), mirrored; a diagonal shell pair keeps P >= Q
+    const int p1 = sh[0] >= eo.dummy_sh0 ? 1 : 0, p2 = sh[2] >= eo.dummy_sh0 ? 3 : 2;
+    if (sh[p1] == sh[p2] && idx[p1] < idx[p2]) return;
+    const long long P = b.off_func[sh[p1]] - eo.nao_m + idx[p1];
+    const long long Q = b.off_func[sh[p2]] - eo.nao_m + idx[p2];
+    eo.out[P * eo.naux + Q] = v;
+    eo.out[Q * eo.naux + P] = v;
+  }
+}
+
 // Lanes of one team always sit in one wavefront, so ordering LDS traffic
 // between phases only needs a wavefront-scope fence plus a scheduling barrier.
 __device__ __forceinline__ void team_sync() {

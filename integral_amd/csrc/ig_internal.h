@@ -219,11 +219,21 @@ hipError_t launch_jk_batch_f32_fcls(int la, int lb, int lc, int ld, const BasisD
 // With rows_end >= 0 the dense scatter only writes the permutations whose
 // first index pair (the output row) belongs to a canonical shell pair in
 // [rows_begin, rows_end): the rows an EriBuilder bra-pair range owns.
+// Density fitting (df.rs) runs the same kernels on a combined basis: main
+// shells [0, nmain_sh), aux shells, then one unit-s dummy per aux shell from
+// dummy_sh0 on (zero exponent, coefficient 1: the constant function).  Mode
+// kDf3c stores (main pair | aux, dummy) blocks as [nao_m, nao_m, naux] with
+// both bra orders; kDf2c stores (aux, dummy | aux, dummy) as the symmetric
+// [naux, naux] metric (aux functions start at function index nao_m).
+enum EriMode { kEriDense = 0, kEriPacked = 1, kDf3c = 2, kDf2c = 3 };
 struct EriOut {
   double *out;
-  const long long *start;  // null: dense
+  const long long *start;  // packed: AO-pair starts (null otherwise)
   long long row0;
   long long rows_begin = 0, rows_end = -1;
+  int mode = kEriDense;
+  int nmain_sh = 0, dummy_sh0 = 0;
+  long long nao_m = 0, naux = 0;
 };
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                       const RysDev &r, const QuartetSet &qs, const EriOut &o, hipStream_t s);

@@ -198,6 +198,16 @@ __global__ void __launch_bounds__(64) eri_kernel(BasisDev b, PairsDev p, RysDev 
       team_sync();
     }
     const int nf = dims[0] * dims[1] * dims[2] * dims[3];
+    if (eo.mode == kDf3c || eo.mode == kDf2c) {
+      for (int e = tl; e < nf; e += G) {
+        const int id = e % dims[3], r1 = e / dims[3], ic = r1 % dims[2], r2 = r1 / dims[2],
+                  ibb = r2 % dims[1], ia = r2 / dims[1];
+        const int idx[4] = {ia, ibb, ic, id};
+        df_store(b, eo, sh, idx, cur[e]);
+      }
+      team_sync();
+      continue;
+    }
     if (eo.start) {  // packed: the canonical element only
       for (int e = tl; e < nf; e += G) {
         const int id = e % dims[3], r1 = e / dims[3], ic = r1 % dims[2], r2 = r1 / dims[2],

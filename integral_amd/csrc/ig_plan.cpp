@@ -480,6 +480,10 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
     for (int p1 = 0; p1 < s1->nprim; ++p1)
       for (int p2 = 0; p2 < s2->nprim; ++p2) {
         double e1 = s1->exps[p1], e2 = s2->exps[p2], z = e1 + e2;
+        // two unit-s dummies (zero exponents): a constant product function
+        // that no density-fitting quartet uses (df.rs:1-39 pairs a dummy
+        // with its aux shell only)
+        if (z == 0.0) continue;
         double K = std::exp(-(e1 * e2 / z) * d2);
         double w = K * s1->eff[p1] * s2->eff[p2];
         if (K * std::fabs(s1->eff[p1] * s2->eff[p2]) < 1e-32) continue;  // os_eri.rs:211
@@ -1556,6 +1560,215 @@ static int eri_builder_impl(const ig_basis *b, int64_t b0, int64_t b1, double *o
       }
     }
     check(hipStreamSynchronize(s), "eri builder sync");
+    return int(IG_OK);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// density fitting (df.rs): 3-center (ij|P) and 2-center (P|Q) through the
+// 4-center kernels with a unit-s dummy on the aux index (df.rs:1-39)
+namespace {
+
+// main shells, then the aux shells, then one unit-s dummy per aux shell
+ig_basis df_combined(const ig_basis *main, const ig_basis &aux) {
+  ig_basis c;
+  std::vector<std::array<double, 3>> atoms;
+  auto add = [&](HostShell h) {
+    h.off_func = c.nao;
+    h.off_cart = c.nao_cart;
+    c.nao += h.nfunc();
+    c.nao_cart += ncart(h.l);
+    int a = -1;
+    for (size_t k = 0; k < atoms.size(); ++k)
+      if (std::memcmp(atoms[k].data(), h.center, sizeof(h.center)) == 0) a = int(k);
+    if (a < 0) {
+      atoms.push_back({h.center[0], h.center[1], h.center[2]});
+      a = int(atoms.size()) - 1;
+    }
+    h.atom = a;
+    c.shells.push_back(std::move(h));
+  };
+  if (main)
+    for (const HostShell &s : main->shells) add(s);
+  for (const HostShell &s : aux.shells) add(s);
+  for (const HostShell &s : aux.shells) {
+    HostShell d;  // unit_s (df.rs:33-37): exponent 0, coefficient 1, exempt from normalisation
+    d.l = 0;
+    d.kind = IG_CARTESIAN;
+    d.nprim = 1;
+    std::memcpy(d.center, s.center, sizeof(d.center));
+    d.exps = {0.0};
+    d.coefs = {1.0};
+    d.eff = {1.0};
+    add(std::move(d));
+  }
+  c.natom = int(atoms.size());
+  return c;
+}
+
+// One DF evaluation on the combined basis: quartets (bra pair | ket pair)
+// given as canonical pair indices of that basis, stored by mode.
+void df_run(Plan &P, const std::vector<std::pair<int64_t, int64_t>> &quartets, EriOut eo,
+            hipStream_t s) {
+  int index[kNumPairClass][kNumPairClass];
+  for (auto &r : index)
+    for (int &v : r) v = -1;
+  std::vector<std::pair<int, int>> cls;
+  std::vector<std::vector<int2>> lists;
+  const int nsh = P.nsh;
+  for (const auto &q : quartets) {
+    int i, j, k, l;
+    pair_of(q.first, i, j);
+    pair_of(q.second, k, l);
+    if (P.schwarz[size_t(i) * nsh + j] == 0.0 || P.schwarz[size_t(k) * nsh + l] == 0.0)
+      continue;  // no primitive pairs: the integrals are 0
+    const int pb = P.canon_to_pos[q.first], pk = P.canon_to_pos[q.second];
+    const int cb = pair_class(P.sh_l[P.p_sh1[pb]], P.sh_l[P.p_sh2[pb]]);
+    const int ck = pair_class(P.sh_l[P.p_sh1[pk]], P.sh_l[P.p_sh2[pk]]);
+    int x = cb, y = ck, px = pb, py = pk;
+    if (cb < ck) {
+      std::swap(x, y);
+      std::swap(px, py);
+    }
+    if (index[x][y] < 0) {
+      index[x][y] = int(cls.size());
+      cls.push_back({x, y});
+      lists.emplace_back();
+    }
+    lists[index[x][y]].push_back(int2{px - P.cls[x].off, py - P.cls[y].off});
+  }
+  std::vector<int2> flat;
+  for (auto &l : lists) flat.insert(flat.end(), l.begin(), l.end());
+  P.s_list.upload(flat);
+  size_t big = 0;
+  for (size_t c = 0; c < cls.size(); ++c)
+    big = std::max(big, big_need(P.cls[cls[c].first], P.cls[cls[c].second],
+                                 (long long)lists[c].size(), 1));
+  P.s_big.reserve(big * sizeof(double));
+  RysDevice &R = rys_device();
+  size_t off = 0;
+  for (size_t c = 0; c < cls.size(); ++c) {
+    const ClassRange &X = P.cls[cls[c].first], &Y = P.cls[cls[c].second];
+    QuartetSet qs{};
+    qs.x_off = X.off;
+    qs.x_n = X.n;
+    qs.y_off = Y.off;
+    qs.y_n = Y.n;
+    qs.same = cls[c].first == cls[c].second;
+    qs.list = P.s_list.as<int2>() + off;
+    qs.total = (long long)lists[c].size();
+    off += lists[c].size();
+    if (class_supported(X.l1, X.l2, Y.l1, Y.l2))
+      check(launch_eri(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, eo, s), "df launch");
+    else
+      check(launch_eri_big(X.l1, X.l2, Y.l1, Y.l2, P.bdev, P.pdev, R.view, qs, eo,
+                           P.s_big.as<double>(), s),
+            "df launch");
+  }
+}
+
+int64_t canon_index(int64_t i, int64_t j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+}  // namespace
+
+extern "C" int ig_eri_2c(const ig_basis *aux, int engine, double *out, size_t len, void *stream) {
+  if (!aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  const size_t naux = size_t(aux->nao);
+  if (len != naux * naux) return fail(IG_ERR_LENGTH_MISMATCH, "2c metric length");
+  return guarded([&] {
+    hipStream_t s = as_stream(stream);
+    const ig_basis comb = df_combined(nullptr, *aux);
+    auto P = build_plan(comb, s);
+    const int na = int(aux->shells.size());
+    std::vector<std::pair<int64_t, int64_t>> qs;  // canonical aux pairs p >= q (df.rs:66-67)
+    for (int p = 0; p < na; ++p)
+      for (int q = 0; q <= p; ++q) qs.push_back({canon_index(na + p, p), canon_index(na + q, q)});
+    DBuf dout;
+    dout.reserve(std::max<size_t>(len, 1) * sizeof(double));
+    if (len) check(launch_fill(dout.as<double>(), len, 0.0, s), "zero 2c");
+    EriOut eo{dout.as<double>(), nullptr, 0};
+    eo.mode = kDf2c;
+    eo.nmain_sh = 0;
+    eo.dummy_sh0 = na;
+    eo.nao_m = 0;
+    eo.naux = (long long)naux;
+    df_run(*P, qs, eo, s);
+    if (len)
+      check(hipMemcpyAsync(out, dout.p, len * sizeof(double), hipMemcpyDeviceToHost, s),
+            "copy 2c");
+    check(hipStreamSynchronize(s), "2c sync");
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_schwarz_aux_bounds(const ig_basis *aux, int engine, double *out, size_t len,
+                                     void *stream) {
+  if (!aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  const size_t naux = size_t(aux->nao), na = aux->shells.size();
+  if (len != na) return fail(IG_ERR_LENGTH_MISMATCH, "aux bound length");
+  std::vector<double> m(naux * naux);
+  if (int rc = ig_eri_2c(aux, engine, m.data(), m.size(), stream)) return rc;
+  // sqrt(max_mu |(P mu|P mu)|) per aux shell (df.rs:257-280)
+  for (size_t p = 0; p < na; ++p) {
+    const HostShell &sp = aux->shells[p];
+    double mx = 0.0;
+    for (int a = 0; a < sp.nfunc(); ++a) {
+      const size_t d = size_t(sp.off_func + a);
+      mx = std::max(mx, std::fabs(m[d * naux + d]));
+    }
+    out[p] = std::sqrt(mx);
+  }
+  return IG_OK;
+}
+
+extern "C" int ig_eri_3c_fill(const ig_basis *main, const ig_basis *aux, int engine,
+                              int64_t bra_begin, int64_t bra_end, double *out, size_t len,
+                              void *stream) {
+  if (!main || !aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  const size_t nao = size_t(main->nao), naux = size_t(aux->nao);
+  if (len != nao * nao * naux) return fail(IG_ERR_LENGTH_MISMATCH, "3c tensor length");
+  const int nm = int(main->shells.size()), na = int(aux->shells.size());
+  const int64_t np = int64_t(nm) * (nm + 1) / 2;
+  if (bra_end < 0) bra_end = np;
+  if (bra_begin < 0 || bra_begin > bra_end || bra_end > np)
+    return fail(IG_ERR_INVALID_ARGUMENT, "bra pair range");
+  return guarded([&] {
+    hipStream_t s = as_stream(stream);
+    const ig_basis comb = df_combined(main, *aux);
+    auto P = build_plan(comb, s);
+    std::vector<std::pair<int64_t, int64_t>> qs;  // every aux shell per bra pair (df.rs:475-500)
+    for (int64_t bc = bra_begin; bc < bra_end; ++bc)
+      for (int p = 0; p < na; ++p) qs.push_back({bc, canon_index(nm + na + p, nm + p)});
+    DBuf dout;
+    dout.reserve(std::max<size_t>(len, 1) * sizeof(double));
+    if (len) check(launch_fill(dout.as<double>(), len, 0.0, s), "zero 3c");
+    EriOut eo{dout.as<double>(), nullptr, 0};
+    eo.mode = kDf3c;
+    eo.nmain_sh = nm;
+    eo.dummy_sh0 = nm + na;
+    eo.nao_m = (long long)nao;
+    eo.naux = (long long)naux;
+    df_run(*P, qs, eo, s);
+    // the rows of the range's bra pairs: (mu in i, nu in j) and (nu, mu),
+    // each mu's run over nu contiguous
+    for (int64_t bc = bra_begin; bc < bra_end; ++bc) {
+      int i, j;
+      pair_of(bc, i, j);
+      const HostShell &si = main->shells[i], &sj = main->shells[j];
+      for (int pass = 0; pass < (i == j ? 1 : 2); ++pass) {
+        const HostShell &r = pass ? sj : si, &c = pass ? si : sj;
+        for (int a = 0; a < r.nfunc(); ++a) {
+          const size_t first = (size_t(r.off_func + a) * nao + c.off_func) * naux;
+          check(hipMemcpyAsync(out + first, dout.as<double>() + first,
+                               size_t(c.nfunc()) * naux * sizeof(double), hipMemcpyDeviceToHost,
+                               s),
+                "copy 3c rows");
+        }
+      }
+    }
+    check(hipStreamSynchronize(s), "3c sync");
     return int(IG_OK);
   });
 }

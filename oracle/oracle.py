@@ -273,3 +273,67 @@ class Basis:
         if rc != 0:
             raise ValueError(lib().or_last_error().decode())
         return out.reshape(self.natom, 3)
+
+
+# ---------------------------------------------------------------------------
+# density fitting (df.rs): 2- and 3-center Coulomb integrals through the
+# 4-center blocks with a unit-s dummy on the aux index (df.rs:1-39)
+
+def _unit_s(shell):
+    """unit_s (df.rs:33-37): one zero-exponent primitive, coefficient 1."""
+    return (0, tuple(shell[1]), [0.0], [1.0], False)
+
+
+def df_eri_2c(aux, engine=ENGINE_AUTO):
+    """Basis::eri_2c_with (df.rs:56-90): (P 1s|Q 1s) over canonical aux shell
+    pairs, each evaluated once and mirrored; row-major [naux, naux]."""
+    na = len(aux)
+    ob = Basis(list(aux) + [_unit_s(s) for s in aux])
+    nf = [ob.nfunc(p) for p in range(na)]
+    off = np.concatenate([[0], np.cumsum(nf)]).astype(int)
+    n = int(off[-1])
+    out = np.zeros((n, n))
+    for p in range(na):
+        for q in range(p + 1):
+            blk = ob.eri_block(p, na + p, q, na + q, engine).reshape(nf[p], nf[q])
+            for a in range(nf[p]):
+                for b in range(nf[q]):
+                    out[off[p] + a, off[q] + b] = blk[a, b]
+                    out[off[q] + b, off[p] + a] = blk[a, b]
+    return out.ravel()
+
+
+def df_eri_3c(main, aux, engine=ENGINE_AUTO):
+    """Eri3cBuilder::build (df.rs:463-530): (ij|P 1s) for every canonical main
+    pair and aux shell, written to the (i, j) band and, for i != j, the (j, i)
+    band; row-major [nao, nao, naux], P fastest."""
+    nm, na = len(main), len(aux)
+    ob = Basis(list(main) + list(aux) + [_unit_s(s) for s in aux])
+    nf = [ob.nfunc(s) for s in range(nm + na)]
+    off = np.concatenate([[0], np.cumsum(nf[:nm])]).astype(int)
+    aoff = np.concatenate([[0], np.cumsum(nf[nm:])]).astype(int)
+    n, naux = int(off[-1]), int(aoff[-1])
+    out = np.zeros((n, n, naux))
+    for i in range(nm):
+        for j in range(i + 1):
+            for p in range(na):
+                blk = ob.eri_block(i, j, nm + p, nm + na + p, engine)
+                blk = blk.reshape(nf[i], nf[j], nf[nm + p])
+                out[off[i]:off[i + 1], off[j]:off[j + 1], aoff[p]:aoff[p + 1]] = blk
+                if i != j:
+                    out[off[j]:off[j + 1], off[i]:off[i + 1], aoff[p]:aoff[p + 1]] = \
+                        blk.transpose(1, 0, 2)
+    return out.ravel()
+
+
+def df_schwarz_aux(aux, engine=ENGINE_AUTO):
+    """Basis::schwarz_aux_bounds_with (df.rs:257-280): sqrt(max |(P|P)|) per
+    aux shell from the diagonal of its (P 1s|P 1s) block."""
+    na = len(aux)
+    ob = Basis(list(aux) + [_unit_s(s) for s in aux])
+    out = np.zeros(na)
+    for p in range(na):
+        n = ob.nfunc(p)
+        blk = ob.eri_block(p, na + p, p, na + p, engine).reshape(n, n)
+        out[p] = np.sqrt(np.max(np.abs(np.diag(blk))))
+    return out
