@@ -329,13 +329,14 @@ bool jk_lane_compiled(int la, int lb, int lc, int ld) {
 }
 
 int jk_default_variant(int la, int lb, int lc, int ld) {
-  // the autotuner's usual choice on s,p,d bases: register kernels with row
-  // sums while a quartet needs at most two element slices (each slice
-  // rebuilds the quartet's 2D tables), team kernels for larger blocks
+  // the autotuner's usual choice: register kernels with row sums on s,p,d
+  // classes up to eight element slices (each slice rebuilds the quartet's 2D
+  // tables; (dd|dd) goes to the team kernel), and on f classes up to two
   // (measured on s,p,d,f molecules: the contracted-space transfers win on
   // the largest blocks with a d or f second ket shell, (fd|fd), (ff|dd),
   // (fd|dd); the autotuner finds the others)
   const int n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+  if (class_supported(la, lb, lc, ld) && n4 <= 8 * kRegBlockMax) return kJkRowAcc;
   if (n4 <= 2 * kRegBlockMax && jk_lane_compiled(la, lb, lc, ld)) return kJkRowAcc;
   if (!jk_team_compiled(la, lb, lc, ld)) return kJkTeamHrr;
   return n4 >= 2000 && ld >= 2 && jk_hrr_compiled(la, lb, lc, ld) ? kJkTeamHrr : kJkTeam;
