@@ -4,13 +4,12 @@
 //
 // Per lane and element slice [LO, HI): the weight W_e of every Cartesian
 // element (Coulomb and exchange terms, direct_grad.rs:490-547) in registers;
-// then per primitive quartet and Rys root, for each differentiated centre
-// X = A, B, C in turn, the three 2D tables raised by one on X only (so they
-// fit in registers) and the sums
-//   dE/dX_x += sum_e W_e w_r (2 alpha_X I_x[x_X+1] - x_X I_x[x_X-1]) I_y I_z
-// (and cyclically for y, z).  D follows by translational invariance.  The 12
-// per-atom sums of a quartet go to an LDS copy of the gradient (natom x 3)
-// with LDS atomics; each workgroup adds its copy to global memory once.
+// then per primitive quartet and Rys root the slice's weights are contracted
+// with the unraised 2D tables into per-axis power-tuple sums, and those sums
+// with the derivative tables of centres A, B (one table raised on A) and C
+// (one raised on C).  D follows by translational invariance.  The 12 per-atom
+// sums of a quartet go to an LDS copy of the gradient (natom x 3) with LDS
+// atomics; each workgroup adds its copy to global memory once.
 #include "ig_cls.h"
 
 namespace ig {
@@ -20,8 +19,9 @@ namespace ig {
 namespace {
 
 constexpr int kGradWaves = 4;
-constexpr int kGradSlice = 27;  // elements per slice: W + one raised table set + sums in VGPRs
-constexpr int kGradLdsMax = 64 * 1024;
+constexpr int kGradThreads = 64 * kGradWaves;
+constexpr int kGradSlice = 27;  // elements per slice (weights in LDS, sums in VGPRs)
+constexpr int kGradLdsMax = 80 * 1024;  // two workgroups per CU
 
 // Cartesian powers of the four shells of every element of class C.
 template <class C>
@@ -51,53 +51,131 @@ __host__ __device__ constexpr int toff(int a, int b, int c, int d, int EB, int E
   return ((a * EB + b) * EC + c) * ED + d;
 }
 
-// Derivative sums of centre X (0 A, 1 B, 2 C) for one root: tables raised on X.
-template <class C, int LO, int HI, int X>
-__device__ __forceinline__ void grad_center(const double (&c00)[3], const double (&cp0)[3],
-                                            double b10, double b01, double b00,
-                                            const double (&AB)[3], const double (&CD)[3],
-                                            double twoexp, double pw, const double *W,
-                                            double *g) {
-  constexpr GradPow<C> Pw{};
-  constexpr int RA = C::LA + (X == 0), RB = C::LB + (X == 1), RC = C::LC + (X == 2), RD = C::LD;
-  constexpr int EB = RB + 1, EC = RC + 1, ED = RD + 1;
-  constexpr int T = (RA + 1) * EB * EC * ED;
-  constexpr int S = X == 0 ? EB * EC * ED : (X == 1 ? EC * ED : ED);  // stride of X's power
-  double I[3][T];
-#pragma unroll
-  for (int ax = 0; ax < 3; ++ax)
-    rys_axis_table<RA, RB, RC, RD>(c00[ax], cp0[ax], b10, b01, b00, AB[ax], CD[ax], I[ax]);
-  double gx = 0.0, gy = 0.0, gz = 0.0;
-#pragma unroll
-  for (int e = LO; e < HI; ++e) {
-    double v[3], dv[3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      const int o = toff(Pw.a[e][ax], Pw.b[e][ax], Pw.c[e][ax], Pw.d[e][ax], EB, EC, ED);
-      const int px = X == 0 ? Pw.a[e][ax] : (X == 1 ? Pw.b[e][ax] : Pw.c[e][ax]);
-      v[ax] = I[ax][o];
-      dv[ax] = twoexp * I[ax][o + S];
-      if (px > 0) dv[ax] -= double(px) * I[ax][o - S];
-    }
-    const double s = W[e - LO];
-    gx += s * dv[0] * v[1] * v[2];
-    gy += s * v[0] * dv[1] * v[2];
-    gz += s * v[0] * v[1] * dv[2];
+// Where each element of the slice [LO, HI) sits in the unraised per-axis
+// table (extents (LA+1, LB+1, LC+1, LD+1)), and which table entries (power
+// tuples (a_x, b_x, c_x, d_x) of one axis) some element of the slice uses.
+template <class C, int LO, int HI>
+struct GradMap {
+  static constexpr int EA = C::LA + 1, EB = C::LB + 1, EC = C::LC + 1, ED = C::LD + 1;
+  static constexpr int T0 = EA * EB * EC * ED;
+  int o[HI - LO][3];
+  bool used[3][T0];
+  constexpr GradMap() : o(), used() {
+    GradPow<C> Pw{};
+    for (int e = LO; e < HI; ++e)
+      for (int ax = 0; ax < 3; ++ax) {
+        const int t = toff(Pw.a[e][ax], Pw.b[e][ax], Pw.c[e][ax], Pw.d[e][ax], EB, EC, ED);
+        o[e - LO][ax] = t;
+        used[ax][t] = true;
+      }
   }
-  g[3 * X + 0] += pw * gx;
-  g[3 * X + 1] += pw * gy;
-  g[3 * X + 2] += pw * gz;
+};
+
+// Derivative sums of one Rys root.  With Z_x[t] = sum over the slice's
+// elements e whose x powers are the tuple t of W_e I_y[e] I_z[e] (and
+// cyclically), the centre-X derivative along x is
+//   sum_t Z_x[t] (2 alpha_X I_x[t + 1_X] - t_X I_x[t - 1_X]),
+// so the elements are visited once per root, not once per centre.  The A and
+// B derivatives both come from the table raised on A, using the transfer
+// I(a, b+1) = I(a+1, b) + (A-B) I(a, b); the C derivative from the table
+// raised on C.  D follows by translational invariance.
+template <class C, int LO, int HI, int WS>
+__device__ __forceinline__ void grad_root(const double (&c00)[3], const double (&cp0)[3],
+                                          double b10, double b01, double b00,
+                                          const double (&AB)[3], const double (&CD)[3],
+                                          double ta, double tb, double tc, double pw,
+                                          const double *W, double *g) {
+  using M = GradMap<C, LO, HI>;
+  constexpr M Mp{};
+  constexpr int T0 = M::T0, EB = M::EB, EC = M::EC, ED = M::ED;
+  constexpr int SA = EB * EC * ED, SB = EC * ED;  // strides of a and b (both layouts)
+  double Z[3][T0];
+  {
+    double I[3][T0];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+      rys_axis_table<C::LA, C::LB, C::LC, C::LD>(c00[ax], cp0[ax], b10, b01, b00, AB[ax],
+                                                 CD[ax], I[ax]);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+      for (int t = 0; t < T0; ++t) Z[ax][t] = 0.0;
+#pragma unroll
+    for (int e = LO; e < HI; ++e) {
+      const int ox = Mp.o[e - LO][0], oy = Mp.o[e - LO][1], oz = Mp.o[e - LO][2];
+      const double w = W[(e - LO) * WS];
+      const double wx = w * I[0][ox];
+      Z[0][ox] += (w * I[1][oy]) * I[2][oz];
+      Z[1][oy] += wx * I[2][oz];
+      Z[2][oz] += wx * I[1][oy];
+    }
+  }
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    double sa = 0.0, sb = 0.0, sc = 0.0;
+    {
+      double R[(C::LA + 2) * SA];  // raised on A: same offsets as the unraised layout
+      rys_axis_table<C::LA + 1, C::LB, C::LC, C::LD>(c00[ax], cp0[ax], b10, b01, b00, AB[ax],
+                                                     CD[ax], R);
+#pragma unroll
+      for (int t = 0; t < T0; ++t) {
+        if (!Mp.used[ax][t]) continue;
+        const int a = t / SA, b = (t / SB) % EB;
+        const double up = R[t + SA];
+        double da = ta * up, db = tb * (up + AB[ax] * R[t]);
+        if (a > 0) da -= double(a) * R[t - SA];
+        if (b > 0) db -= double(b) * R[t - SB];
+        sa += Z[ax][t] * da;
+        sb += Z[ax][t] * db;
+      }
+    }
+    {
+      constexpr int RC = C::LC + 2;
+      double R[(C::LA + 1) * EB * RC * ED];  // raised on C
+      rys_axis_table<C::LA, C::LB, C::LC + 1, C::LD>(c00[ax], cp0[ax], b10, b01, b00, AB[ax],
+                                                     CD[ax], R);
+#pragma unroll
+      for (int t = 0; t < T0; ++t) {
+        if (!Mp.used[ax][t]) continue;
+        const int a = t / SA, b = (t / SB) % EB, c = (t / ED) % EC, d = t % ED;
+        const int o = toff(a, b, c, d, EB, RC, ED);
+        double dc = tc * R[o + ED];
+        if (c > 0) dc -= double(c) * R[o - ED];
+        sc += Z[ax][t] * dc;
+      }
+    }
+    g[ax] += pw * sa;
+    g[3 + ax] += pw * sb;
+    g[6 + ax] += pw * sc;
+  }
 }
 
-template <class C, int LO, int HI>
-__global__ void __launch_bounds__(64 * kGradWaves)
+// Where the weights of a lane's quartet live.  For total angular momentum 3
+// and 4, and 5 with at most one d shell, they go to LDS (slot stride
+// kGradThreads, so a wavefront's reads are conflict-free), which keeps the
+// kernel at two wavefronts per SIMD without spills; elsewhere (few elements,
+// or tables too large for two wavefronts) they stay in VGPRs at one
+// wavefront per SIMD.  Chosen from per-class kernel times of both forms.
+template <class C>
+constexpr bool grad_w_in_lds() {
+  constexpr int L = C::LA + C::LB + C::LC + C::LD;
+  constexpr int nd = (C::LA == 2) + (C::LB == 2) + (C::LC == 2) + (C::LD == 2);
+  return (L >= 3 && L <= 4) || (L == 5 && nd <= 1);
+}
+
+// With the weights in LDS, the slot offset passes through an empty asm
+// statement at every root so that the reads stay inside the root loop
+// instead of being hoisted back into registers.
+template <class C, int LO, int HI, bool WL = grad_w_in_lds<C>()>
+__global__ void __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(WL ? 2 : 1)))
     grad_lane_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs, GradDev gd,
                      unsigned long long *counters, int lds_grad) {
   constexpr int NR = (C::LA + C::LB + C::LC + C::LD + 1) / 2 + 1;
   extern __shared__ double lds[];
   stage_rys<NR>(R, lds);  // ends with a barrier
   const int nrys = 2 * R.nsub[NR] * NR * kRysTerms;
-  double *G = lds + nrys;  // natom x 3 (when lds_grad)
+  double *Wl = lds + nrys;  // WL: (HI - LO) slots of stride kGradThreads
+  double *G = lds + nrys + (WL ? (HI - LO) * kGradThreads : 0);  // natom x 3 (when lds_grad)
   if (lds_grad) {
     for (int i = threadIdx.x; i < lds_grad; i += blockDim.x) G[i] = 0.0;
     __syncthreads();
@@ -139,9 +217,9 @@ __global__ void __launch_bounds__(64 * kGradWaves)
     }
     if (p.q[pb] * p.q[pk] * deg * bound < gd.tau) continue;
     // weights of the slice's elements
+    double Wr[WL ? 1 : HI - LO];
     const size_t a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
                  d0 = b.off_cart[g.sD];
-    double W[HI - LO];
 #pragma unroll
     for (int e = LO; e < HI; ++e) {
       const int id = e % C::nd, r1 = e / C::nd, ic = r1 % C::nc, r2 = r1 / C::nc;
@@ -158,7 +236,10 @@ __global__ void __launch_bounds__(64 * kGradWaves)
                (Lc[A * ncrt + Cc] * Rc[B * ncrt + D] + Lc[B * ncrt + Cc] * Rc[A * ncrt + D] +
                 Lc[A * ncrt + D] * Rc[B * ncrt + Cc] + Lc[B * ncrt + D] * Rc[A * ncrt + Cc]);
       }
-      W[e - LO] = w;
+      if constexpr (WL)
+        Wl[(e - LO) * kGradThreads + threadIdx.x] = w;
+      else
+        Wr[e - LO] = w;
     }
     double gs[9];
 #pragma unroll
@@ -207,9 +288,15 @@ __global__ void __launch_bounds__(64 * kGradWaves)
             cp0[ax] = (Qv[ax] - g.C[ax]) + fp * PQ[ax];
           }
           const double pw = pref * rwt[r];
-          grad_center<C, LO, HI, 0>(c00, cp0, b10, b01, b00, AB, CD, 2.0 * al, pw, W, gs);
-          grad_center<C, LO, HI, 1>(c00, cp0, b10, b01, b00, AB, CD, 2.0 * be, pw, W, gs);
-          grad_center<C, LO, HI, 2>(c00, cp0, b10, b01, b00, AB, CD, 2.0 * gam, pw, W, gs);
+          if constexpr (WL) {
+            int wo = threadIdx.x;
+            asm volatile("" : "+v"(wo));
+            grad_root<C, LO, HI, kGradThreads>(c00, cp0, b10, b01, b00, AB, CD, 2.0 * al,
+                                               2.0 * be, 2.0 * gam, pw, Wl + wo, gs);
+          } else {
+            grad_root<C, LO, HI, 1>(c00, cp0, b10, b01, b00, AB, CD, 2.0 * al, 2.0 * be,
+                                    2.0 * gam, pw, Wr, gs);
+          }
         }
       }
     }
@@ -259,7 +346,8 @@ hipError_t launch_grad_slices(const BasisDev &b, const PairsDev &p, const RysDev
   constexpr int NS = GradSlices<C>::NS, E = GradSlices<C>::E;
   if constexpr (S < NS) {
     constexpr int LO = S * E, HI = (S + 1) * E < C::N4 ? (S + 1) * E : C::N4;
-    hipLaunchKernelGGL((grad_lane_kernel<C, LO, HI>), dim3(blocks), dim3(64 * kGradWaves), lds, s,
+    const size_t wl = grad_w_in_lds<C>() ? size_t(HI - LO) * kGradThreads * sizeof(double) : 0;
+    hipLaunchKernelGGL((grad_lane_kernel<C, LO, HI>), dim3(blocks), dim3(kGradThreads), lds + wl, s,
                        b, p, r, qs, g, ctr, lds_grad);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -278,8 +366,10 @@ hipError_t grad_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r,
   constexpr int NR = (LA + LB + LC + LD + 1) / 2 + 1;
   const size_t rys = size_t(2) * r.nsub[NR] * NR * kRysTerms * sizeof(double);
   const size_t gl = size_t(3) * natom * sizeof(double);
-  const int lds_grad = rys + gl <= size_t(kGradLdsMax) ? 3 * natom : 0;
-  long long blocks = (qs.total + 64 * kGradWaves - 1) / (64 * kGradWaves);
+  const size_t wl =  // widest slice
+      grad_w_in_lds<C>() ? size_t(GradSlices<C>::E) * kGradThreads * sizeof(double) : 0;
+  const int lds_grad = rys + wl + gl <= size_t(kGradLdsMax) ? 3 * natom : 0;
+  long long blocks = (qs.total + kGradThreads - 1) / kGradThreads;
   if (blocks > 2048) blocks = 2048;
   return launch_grad_slices<C, 0>(b, p, r, qs, g, ctr, s, int(blocks),
                                   rys + (lds_grad ? gl : 0), lds_grad);
