@@ -346,6 +346,20 @@ int ig_grad_contract(const ig_grad *grad, const ig_grad_density *density, int ke
                      double omega, double threshold, double *out, size_t out_len, void *stream);
 int ig_grad_last_stats(const ig_grad *grad, int64_t *stats2);
 
+/* ---- dense two-particle-density gradient: Basis::eri_grad_contract ------ */
+/* Basis::eri_grad_contract / eri_grad_contract_with / eri_grad_contract_kernel
+ * (grad.rs:417-466, 493-540): per atom F_c = sum over all ordered quartets of
+ * gamma[mu,nu,lam,sig] * d(mu nu|lam sig)/dR_c.  d_gamma: the dense nao^4
+ * row-major two-particle density on the device (gamma_len must be nao^4, else
+ * IG_ERR_GAMMA_LENGTH); any gamma, symmetric or not.  out: natom*3 doubles
+ * (host), overwritten, atoms in Basis::atoms order.  Fails with
+ * IG_ERR_L_TOO_HIGH_FOR_GRADIENT for l > 5 and IG_ERR_INVALID_OMEGA as
+ * ig_grad_contract.  Allocates two nao_cart^4 device buffers (the Cartesian
+ * gamma); synchronises `stream`. */
+int ig_eri_grad_contract_device(const ig_basis *basis, int engine, const double *d_gamma,
+                                size_t gamma_len, int kernel, double omega, double *out,
+                                size_t out_len, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

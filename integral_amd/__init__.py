@@ -235,6 +235,8 @@ def library():
     lib.ig_grad_contract.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      _dp, sz, _vp]
     lib.ig_grad_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
+    lib.ig_eri_grad_contract_device.argtypes = [_vp, ctypes.c_int, _vp, sz, ctypes.c_int,
+                                                ctypes.c_double, _dp, sz, _vp]
     _lib = lib
     return lib
 
@@ -444,6 +446,35 @@ class Basis:
 
     def eri_3c_block(self, aux, ish, jsh, psh):
         return self.eri_3c_block_with(Engine.Auto, aux, ish, jsh, psh)
+
+    def eri_grad_contract_with(self, engine, gamma, kernel=None, stream=None):
+        """Basis::eri_grad_contract_with (grad.rs:426-466): per atom
+        F_c = sum over all ordered quartets of gamma[mu,nu,lam,sig] *
+        d(mu nu|lam sig)/dR_c, returned as (natom, 3) in atoms() order.
+        gamma is the dense nao^4 row-major two-particle density, a float64
+        CUDA tensor (used in place) or a host array (copied to the device).
+        `kernel` as in eri_grad_contract_kernel (grad.rs:493-540)."""
+        import torch
+        if isinstance(gamma, torch.Tensor):
+            g = gamma
+            if not (g.is_cuda and g.dtype == torch.float64 and g.is_contiguous()):
+                raise UsageError("gamma must be a contiguous float64 CUDA tensor")
+        else:
+            g = torch.from_numpy(np.ascontiguousarray(gamma, dtype=np.float64).ravel()).to("cuda")
+        kernel = kernel or EriKernel.coulomb()
+        out = np.zeros((len(self.atoms()), 3))
+        _raise(library().ig_eri_grad_contract_device(
+            self._h, int(engine), _vp(g.data_ptr()), g.numel(), *kernel._args(), _ptr(out),
+            out.size, _stream_ptr(stream)))
+        return out
+
+    def eri_grad_contract(self, gamma):
+        """Basis::eri_grad_contract (grad.rs:417-419)."""
+        return self.eri_grad_contract_with(Engine.Auto, gamma)
+
+    def eri_grad_contract_kernel(self, gamma, k):
+        """Basis::eri_grad_contract_kernel (grad.rs:493-540): Coulomb or erf."""
+        return self.eri_grad_contract_with(Engine.Auto, gamma, kernel=k)
 
     def direct_grad_contractor(self):
         return DirectGradContractor(self)

@@ -21,6 +21,32 @@ __device__ __forceinline__ size_t jk_copy_offset(const DensDev &d) {
 // J'/K' accumulation (device-scope FP64 atomic add).
 __device__ __forceinline__ void jk_add(const DensDev &, double *p, double v) { atomicAdd(p, v); }
 
+// Weight of the Cartesian element (A B | C D) of a canonical shell quartet
+// against a dense two-particle density G (nc^4, row-major): G summed over the
+// ordered quartets the canonical one stands for, i.e. over the distinct
+// shell-level permutations (swap within the bra when its shells differ,
+// within the ket likewise, bra with ket when the pairs differ), so a
+// canonical-quartet sweep gives the all-ordered-quartets contraction of
+// grad.rs:426-466 for any G, symmetric or not.
+__device__ __forceinline__ double gamma_weight(const double *__restrict__ G, size_t nc, size_t A,
+                                               size_t B, size_t C, size_t D, bool ab, bool cd,
+                                               bool pq) {
+  auto at = [&](size_t i, size_t j, size_t k, size_t l) {
+    return G[((i * nc + j) * nc + k) * nc + l];
+  };
+  double w = at(A, B, C, D);
+  if (ab) w += at(B, A, C, D);
+  if (cd) w += at(A, B, D, C);
+  if (ab && cd) w += at(B, A, D, C);
+  if (pq) {
+    w += at(C, D, A, B);
+    if (cd) w += at(D, C, A, B);
+    if (ab) w += at(C, D, B, A);
+    if (ab && cd) w += at(D, C, B, A);
+  }
+  return w;
+}
+
 // Packed storage of one function-space element (ab|cd) of shells (sA, sB |
 // sC, sD) with function indices fa..fd (packed_eri.rs:10-21 layout): each
 // AO pair is put in canonical order (shell i >= j; inside one shell a >= b),

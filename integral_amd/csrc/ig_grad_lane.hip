@@ -226,6 +226,8 @@ __global__ void __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_
       const int ib = r2 % C::nb, ia = r2 / C::nb;
       const size_t A = a0 + ia, B = b0 + ib, Cc = c0 + ic, D = d0 + id;
       double w = 0.0;
+      if (gd.gamma)
+        w = gamma_weight(gd.gamma, size_t(ncrt), A, B, Cc, D, g.sA != g.sB, g.sC != g.sD, pb != pk);
       for (int tm = 0; tm < nterm; ++tm) {
         const double *Lc = gd.mats + (size_t)(2 * tm) * nc2, *Rc = Lc + nc2;
         if (tm < gd.ncoul)

@@ -123,6 +123,9 @@ struct GradDev {
   double *out;           // natom x 3 accumulator
   double tau;
   double omega;          // <= 0: Coulomb
+  // dense two-particle density (nao_cart^4 Cartesian, row-major) instead of
+  // the density terms (ncoul = nexch = 0 then), or null
+  const double *gamma;
 };
 
 // ---------------------------------------------------------------------------
@@ -274,6 +277,10 @@ hipError_t launch_block_max(const BasisDev &b, const double *m, int nmats, doubl
                             hipStream_t s);
 hipError_t launch_to_cart(const BasisDev &b, const double *m_func, int nmats, double *m_cart,
                           hipStream_t s);
+// dense nao^4 two-particle density -> nao_cart^4 (t0, t1: nao_cart^4 doubles
+// each; result in t1)
+hipError_t launch_gamma_to_cart(const BasisDev &b, const double *gamma, double *t0, double *t1,
+                                hipStream_t s);
 hipError_t launch_jk_finish(const BasisDev &b, const DensDev &d, double *d_J, double *d_K,
                             hipStream_t s);
 // sum the ncopy per-XCD accumulator copies of d into the first one

@@ -522,6 +522,50 @@ hipError_t launch_to_cart(const BasisDev &b, const double *mf, int nmats, double
   return hipGetLastError();
 }
 
+// One index of a 4-index tensor from function to Cartesian space:
+// out[p, I, q] = sum_mu M_S(mu, I) in[p, off_func(S) + mu, q], S the shell of
+// Cartesian index I and M_S its c2s rows (the identity on a Cartesian
+// shell); in is [pre, nao, post], out [pre, nao_cart, post].
+__global__ void axis_to_cart_kernel(BasisDev b, const double *__restrict__ in,
+                                    double *__restrict__ out, size_t pre, size_t post) {
+  const size_t nc = (size_t)b.nao_cart, total = pre * nc * post;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t q = t % post, r = t / post, p = r / nc;
+    const int I = int(r % nc);
+    const int S = shell_of(b.off_cart, b.nsh, I);
+    const int a = I - b.off_cart[S], lS = b.l[S];
+    const double *src = in + (p * (size_t)b.nao + b.off_func[S]) * post + q;
+    double v;
+    if (!b.kind[S]) {
+      v = src[(size_t)a * post];
+    } else {
+      const double *MS = b.c2s + b.c2s_off[lS];
+      v = 0.0;
+      for (int mu = 0; mu < 2 * lS + 1; ++mu) v += MS[mu * ncart(lS) + a] * src[(size_t)mu * post];
+    }
+    out[t] = v;
+  }
+}
+
+// Dense two-particle density (nao^4, function space) to Cartesian space, one
+// index per pass, last index first.  t0 and t1 hold nao_cart^4 doubles each;
+// the result is left in t1.
+hipError_t launch_gamma_to_cart(const BasisDev &b, const double *g, double *t0, double *t1,
+                                hipStream_t s) {
+  const size_t n = (size_t)b.nao, c = (size_t)b.nao_cart;
+  const double *src[4] = {g, t0, t1, t0};
+  double *dst[4] = {t0, t1, t0, t1};
+  const size_t pre[4] = {n * n * n, n * n, n, 1}, post[4] = {1, c, c * c, c * c * c};
+  for (int k = 0; k < 4; ++k) {
+    hipLaunchKernelGGL(axis_to_cart_kernel, dim3(grid1d(pre[k] * c * post[k])), dim3(256), 0, s,
+                       b, src[k], dst[k], pre[k], post[k]);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 __global__ void asym_flag_kernel(const double *__restrict__ d, int n, int ndens, int *flag) {
   const size_t n2 = (size_t)n * n;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n2 * ndens;

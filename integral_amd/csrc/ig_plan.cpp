@@ -2017,6 +2017,71 @@ extern "C" int ig_grad_last_stats(const ig_grad *g, int64_t *st) {
 }
 
 // ---------------------------------------------------------------------------
+// dense two-particle density: Basis::eri_grad_contract / _with / _kernel
+// (grad.rs:417-466, 493-540).  The nao^4 gamma goes to Cartesian space once
+// (four one-index passes), then every canonical quartet with Q > 0 runs
+// through the gradient kernels with its weights read from gamma (summed over
+// the ordered quartets it stands for) instead of the density terms.
+extern "C" int ig_eri_grad_contract_device(const ig_basis *basis, int engine,
+                                           const double *d_gamma, size_t gamma_len, int kernel,
+                                           double omega, double *out, size_t out_len,
+                                           void *stream) {
+  if (!basis || !d_gamma || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  for (size_t sh = 0; sh < basis->shells.size(); ++sh)
+    if (basis->shells[sh].l > kMaxGradL)
+      return fail(IG_ERR_L_TOO_HIGH_FOR_GRADIENT,
+                  "shell " + std::to_string(sh) + ": l=" + std::to_string(basis->shells[sh].l) +
+                      " exceeds the gradient maximum 5");
+  const size_t nao = size_t(basis->nao), n4 = nao * nao * nao * nao;
+  if (gamma_len != n4)
+    return fail(IG_ERR_GAMMA_LENGTH, "gamma must hold nao^4 = " + std::to_string(n4) +
+                                         " elements, got " + std::to_string(gamma_len));
+  if (kernel != IG_KERNEL_COULOMB && kernel != IG_KERNEL_ERF)
+    return fail(IG_ERR_INVALID_ARGUMENT, "unknown kernel");
+  if (kernel == IG_KERNEL_ERF && !(std::isfinite(omega) && omega > 0.0))
+    return fail(IG_ERR_INVALID_OMEGA, "EriKernel::Erf requires a finite omega > 0");
+  return guarded([&] {
+    std::unique_ptr<Plan> Pp = build_plan(*basis, nullptr);
+    Plan &P = *Pp;
+    if (out_len != size_t(P.natom) * 3)
+      return fail(IG_ERR_LENGTH_MISMATCH, "out must hold natom*3");
+    hipStream_t s = as_stream(stream);
+    const size_t nc = size_t(P.nao_cart), c4 = nc * nc * nc * nc;
+    DBuf t0, t1, dout;
+    t0.reserve(c4 * sizeof(double));
+    t1.reserve(c4 * sizeof(double));
+    dout.reserve(out_len * sizeof(double));
+    check(launch_gamma_to_cart(P.bdev, d_gamma, t0.as<double>(), t1.as<double>(), s),
+          "gamma to Cartesian");
+    check(launch_fill(dout.as<double>(), out_len, 0.0, s), "zero grad");
+    P.s_counters.reserve(2 * sizeof(unsigned long long));
+    check(launch_fill(P.s_counters.as<double>(), 2, 0.0, s), "zero counters");
+    GradDev gd{};
+    gd.out = dout.as<double>();
+    gd.tau = 0.0;  // every quartet, as the dense sweep
+    gd.omega = kernel == IG_KERNEL_ERF ? omega : 0.0;
+    gd.gamma = t1.as<double>();
+    auto *ctr = P.s_counters.as<unsigned long long>();
+    build_rowptrs(P, 0.0);
+    size_t big = 0;
+    for (auto &cp : P.cached_cps) big = std::max(big, big_need(P.cls[cp.x], P.cls[cp.y], cp.total, 0));
+    P.s_big.reserve(big * sizeof(double));
+    for (auto &cp : P.cached_cps) {
+      if (!cp.total) continue;
+      check(grad_launch(P, P.cls[cp.x], P.cls[cp.y], implicit_set(P, cp), gd, ctr, s),
+            "grad launch");
+    }
+    std::vector<double> h(out_len);
+    check(hipMemcpyAsync(h.data(), dout.p, out_len * sizeof(double), hipMemcpyDeviceToHost, s),
+          "copy grad");
+    check(hipStreamSynchronize(s), "grad sync");
+    std::copy(h.begin(), h.end(), out);
+    return int(IG_OK);
+  });
+}
+
+// ---------------------------------------------------------------------------
 // batched J/K over many bases (one symmetric density per basis)
 
 // The batched graph's execution is switched off until the cause of an

@@ -93,6 +93,24 @@ def test_gradient_rejects_l6_before_any_device_work():
         b.direct_grad_contractor()
 
 
+def test_dense_gamma_gradient_checks_before_any_device_work():
+    """ig_eri_grad_contract_device validates l, gamma length and omega before
+    it touches the (here dummy, never dereferenced) device pointer."""
+    lib = ig.library()
+    fake = ctypes.c_void_p(256)
+    out = np.zeros(3)
+    b6 = synthetic.to_basis([(6, (0.0, 0.0, 0.0), [1.0], [1.0], True)])
+    assert lib.ig_eri_grad_contract_device(b6._h, 0, fake, 13 ** 4, 0, 0.0, ig._ptr(out), 3,
+                                           None) == ig.E_LGRAD
+    b = synthetic.to_basis([(1, (0.0, 0.0, 0.0), [1.0], [1.0], False)])
+    assert lib.ig_eri_grad_contract_device(b._h, 0, fake, 80, 0, 0.0, ig._ptr(out), 3,
+                                           None) == ig.E_GAMMA
+    assert lib.ig_eri_grad_contract_device(b._h, 0, fake, 81, 1, float("nan"), ig._ptr(out), 3,
+                                           None) == 15
+    assert lib.ig_eri_grad_contract_device(b._h, 0, None, 81, 0, 0.0, ig._ptr(out), 3,
+                                           None) == 16
+
+
 def test_invalid_engine_and_null_arguments():
     b = synthetic.to_basis([(0, (0.0, 0.0, 0.0), [1.0], [1.0], False)])
     h = ctypes.c_void_p()
