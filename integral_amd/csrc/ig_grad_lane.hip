@@ -20,8 +20,8 @@ namespace {
 
 constexpr int kGradWaves = 4;
 constexpr int kGradThreads = 64 * kGradWaves;
-constexpr int kGradSlice = 27;  // elements per slice (weights in LDS, sums in VGPRs)
-constexpr int kGradLdsMax = 80 * 1024;  // two workgroups per CU
+constexpr int kGradSlice = 27;     // elements per slice
+constexpr int kGradSliceWide = 54;  // elements per slice of classes with >= 216 elements
 
 // Cartesian powers of the four shells of every element of class C.
 template <class C>
@@ -150,24 +150,45 @@ __device__ __forceinline__ void grad_root(const double (&c00)[3], const double (
   }
 }
 
-// Where the weights of a lane's quartet live.  For total angular momentum 3
-// and 4, and 5 with at most one d shell, they go to LDS (slot stride
-// kGradThreads, so a wavefront's reads are conflict-free), which keeps the
-// kernel at two wavefronts per SIMD without spills; elsewhere (few elements,
-// or tables too large for two wavefronts) they stay in VGPRs at one
-// wavefront per SIMD.  Chosen from per-class kernel times of both forms.
+// Elements per slice: 27, or 54 for the classes of 216 elements and more
+// (eight slices or more at 27), which halves the per-slice work (weights,
+// screening, roots and tables are redone in every slice).
 template <class C>
-constexpr bool grad_w_in_lds() {
+constexpr int grad_slice_cap() {
+  return C::N4 >= 216 ? kGradSliceWide : kGradSlice;
+}
+
+// Wavefronts per SIMD the kernel is built for: two for total angular
+// momentum 3 and 4, and 5 with at most one d shell (their weights in LDS
+// keep them there without spills); one elsewhere (few elements, or tables
+// too large for two wavefronts).  Chosen from per-class kernel times.
+template <class C>
+constexpr int grad_waves() {
   constexpr int L = C::LA + C::LB + C::LC + C::LD;
   constexpr int nd = (C::LA == 2) + (C::LB == 2) + (C::LC == 2) + (C::LD == 2);
-  return (L >= 3 && L <= 4) || (L == 5 && nd <= 1);
+  return (L >= 3 && L <= 4) || (L == 5 && nd <= 1) ? 2 : 1;
+}
+
+// Where the weights of a lane's quartet live: in LDS (slot stride
+// kGradThreads, so a wavefront's reads are conflict-free) for the
+// two-wavefront classes and the wide slices, in VGPRs otherwise.
+template <class C>
+constexpr bool grad_w_in_lds() {
+  return grad_waves<C>() == 2 || grad_slice_cap<C>() == kGradSliceWide;
+}
+
+// LDS a workgroup may take: two workgroups per CU at two wavefronts per
+// SIMD, the whole CU at one.
+template <class C>
+constexpr size_t grad_lds_max() {
+  return grad_waves<C>() == 2 ? 80 * 1024 : 160 * 1024;
 }
 
 // With the weights in LDS, the slot offset passes through an empty asm
 // statement at every root so that the reads stay inside the root loop
 // instead of being hoisted back into registers.
-template <class C, int LO, int HI, bool WL = grad_w_in_lds<C>()>
-__global__ void __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(WL ? 2 : 1)))
+template <class C, int LO, int HI, bool WL = grad_w_in_lds<C>(), int WPE = grad_waves<C>()>
+__global__ void __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
     grad_lane_kernel(BasisDev b, PairsDev p, RysDev R, QuartetSet qs, GradDev gd,
                      unsigned long long *counters, int lds_grad) {
   constexpr int NR = (C::LA + C::LB + C::LC + C::LD + 1) / 2 + 1;
@@ -337,7 +358,7 @@ __global__ void __launch_bounds__(kGradThreads) __attribute__((amdgpu_waves_per_
 
 template <class C>
 struct GradSlices {
-  static constexpr int NS = (C::N4 + kGradSlice - 1) / kGradSlice;
+  static constexpr int NS = (C::N4 + grad_slice_cap<C>() - 1) / grad_slice_cap<C>();
   static constexpr int E = (C::N4 + NS - 1) / NS;
 };
 
@@ -370,7 +391,7 @@ hipError_t grad_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r,
   const size_t gl = size_t(3) * natom * sizeof(double);
   const size_t wl =  // widest slice
       grad_w_in_lds<C>() ? size_t(GradSlices<C>::E) * kGradThreads * sizeof(double) : 0;
-  const int lds_grad = rys + wl + gl <= size_t(kGradLdsMax) ? 3 * natom : 0;
+  const int lds_grad = rys + wl + gl <= grad_lds_max<C>() ? 3 * natom : 0;
   long long blocks = (qs.total + kGradThreads - 1) / kGradThreads;
   if (blocks > 2048) blocks = 2048;
   return launch_grad_slices<C, 0>(b, p, r, qs, g, ctr, s, int(blocks),
