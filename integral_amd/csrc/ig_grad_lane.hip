@@ -21,7 +21,7 @@ namespace {
 constexpr int kGradWaves = 4;
 constexpr int kGradThreads = 64 * kGradWaves;
 constexpr int kGradSlice = 27;     // elements per slice
-constexpr int kGradSliceWide = 54;  // elements per slice of classes with >= 216 elements
+constexpr int kGradSliceWide = 54;  // elements per slice of the large one-wavefront classes
 
 // Cartesian powers of the four shells of every element of class C.
 template <class C>
@@ -150,14 +150,6 @@ __device__ __forceinline__ void grad_root(const double (&c00)[3], const double (
   }
 }
 
-// Elements per slice: 27, or 54 for the classes of 216 elements and more
-// (eight slices or more at 27), which halves the per-slice work (weights,
-// screening, roots and tables are redone in every slice).
-template <class C>
-constexpr int grad_slice_cap() {
-  return C::N4 >= 216 ? kGradSliceWide : kGradSlice;
-}
-
 // Wavefronts per SIMD the kernel is built for: two for total angular
 // momentum 3 and 4, and 5 with at most one d shell (their weights in LDS
 // keep them there without spills); one elsewhere (few elements, or tables
@@ -167,6 +159,15 @@ constexpr int grad_waves() {
   constexpr int L = C::LA + C::LB + C::LC + C::LD;
   constexpr int nd = (C::LA == 2) + (C::LB == 2) + (C::LC == 2) + (C::LD == 2);
   return (L >= 3 && L <= 4) || (L == 5 && nd <= 1) ? 2 : 1;
+}
+
+// Elements per slice: 27, or 54 for the one-wavefront classes of 108
+// elements and more, which halves the per-slice work (weights, screening,
+// roots and tables are redone in every slice).  The two-wavefront (d p|p p)
+// keeps 27: 54 would cost it its second wavefront (measured 10% slower).
+template <class C>
+constexpr int grad_slice_cap() {
+  return C::N4 >= 108 && grad_waves<C>() == 1 ? kGradSliceWide : kGradSlice;
 }
 
 // Where the weights of a lane's quartet live: in LDS (slot stride
