@@ -114,10 +114,21 @@ int ig_eri_screened(const ig_basis *basis, int engine, double tau, double *out, 
  * those bra pairs own -- mu in shell i and nu in shell j, or the reverse --
  * are written, every element equal to Basis::eri's to round-off (the
  * reference's contract, eri_builder.rs:419-446).  Filling every range gives
- * the whole tensor; EriBuilder::build is ig_eri_dense.  Coulomb operator
- * only (EriBuilder::kernel(Erf) has no GPU counterpart here). */
+ * the whole tensor; EriBuilder::build is ig_eri_dense.  Coulomb operator;
+ * ig_eri_builder_fill_kernel takes the EriKernel. */
 int ig_eri_builder_fill(const ig_basis *basis, int engine, int64_t bra_begin, int64_t bra_end,
                         double *out, size_t out_len, void *stream);
+/* EriBuilder::kernel(k) then fill (eri_builder.rs:280-305, 459-470): as
+ * ig_eri_builder_fill with the operator `kernel` (IG_KERNEL_COULOMB, or
+ * IG_KERNEL_ERF: erf(omega r)/r by the attenuated Rys transform, omega finite
+ * and > 0, else IG_ERR_INVALID_OMEGA). */
+int ig_eri_builder_fill_kernel(const ig_basis *basis, int engine, int kernel, double omega,
+                               int64_t bra_begin, int64_t bra_end, double *out, size_t out_len,
+                               void *stream);
+/* Basis::eri_kernel (integrals.rs:773-830): the dense nao^4 tensor of the
+ * operator `kernel` (Coulomb: ig_eri_dense), host memory, overwritten. */
+int ig_eri_kernel(const ig_basis *basis, int kernel, double omega, double *out, size_t out_len,
+                  void *stream);
 /* Basis::schwarz_bounds_with, integrals.rs:842-862: nshell^2 doubles, host. */
 int ig_schwarz_bounds(const ig_basis *basis, int engine, double *out, size_t out_len,
                       void *stream);
@@ -139,6 +150,15 @@ int ig_schwarz_aux_bounds(const ig_basis *aux, int engine, double *out, size_t l
  * i != j, (nu, mu) of those pairs are written. */
 int ig_eri_3c_fill(const ig_basis *main, const ig_basis *aux, int engine, int64_t bra_begin,
                    int64_t bra_end, double *out, size_t len, void *stream);
+/* Basis::eri_2c_kernel (df.rs:95-128): ig_eri_2c with the operator `kernel`
+ * (erf: omega finite and > 0, else IG_ERR_INVALID_OMEGA). */
+int ig_eri_2c_kernel(const ig_basis *aux, int kernel, double omega, double *out, size_t len,
+                     void *stream);
+/* Basis::eri_3c_block_kernel (df.rs:190-240) over bra-pair ranges: ig_eri_3c_fill
+ * with the operator `kernel`. */
+int ig_eri_3c_fill_kernel(const ig_basis *main, const ig_basis *aux, int engine, int kernel,
+                          double omega, int64_t bra_begin, int64_t bra_end, double *out,
+                          size_t len, void *stream);
 
 /* ---- packed symmetry-unique ERIs: PackedEriBuilder (packed_eri.rs) -------
  * AO pairs (p, q), p >= q, numbered shell-pair blocked: canonical shell pairs

@@ -16,6 +16,7 @@ fallback.  Importing works without a GPU (so the CPU test suite can check
 the boundary), but any compute call then raises DeviceError.
 """
 import ctypes
+import math
 import enum
 import os
 
@@ -198,6 +199,12 @@ def library():
     lib.ig_direct_schedule_write.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz]
     lib.ig_eri_builder_fill.argtypes = [_vp, ctypes.c_int, i64, i64, _dp, sz, _vp]
     lib.ig_eri_2c.argtypes = [_vp, ctypes.c_int, _dp, sz, _vp]
+    lib.ig_eri_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, sz, _vp]
+    lib.ig_eri_builder_fill_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                               i64, i64, _dp, sz, _vp]
+    lib.ig_eri_2c_kernel.argtypes = [_vp, ctypes.c_int, ctypes.c_double, _dp, sz, _vp]
+    lib.ig_eri_3c_fill_kernel.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          i64, i64, _dp, sz, _vp]
     lib.ig_schwarz_aux_bounds.argtypes = [_vp, ctypes.c_int, _dp, sz, _vp]
     lib.ig_eri_3c_fill.argtypes = [_vp, _vp, ctypes.c_int, i64, i64, _dp, sz, _vp]
     lib.ig_packed_eri_n_ao_pairs.argtypes = [_vp]
@@ -380,6 +387,14 @@ class Basis:
     def eri(self):
         return self.eri_with(Engine.Auto)
 
+    def eri_kernel(self, k):
+        """Basis::eri_kernel (integrals.rs:773-830): the dense tensor of the
+        Coulomb or erf(omega r)/r operator."""
+        n = self.nao()
+        out = np.zeros(n ** 4)
+        _raise(library().ig_eri_kernel(self._h, *k._args(), _ptr(out), out.size, None))
+        return out
+
     def eri_screened_with(self, engine, tau):
         n = self.nao()
         out = np.zeros(n ** 4)
@@ -421,6 +436,13 @@ class Basis:
     def eri_2c(self):
         return self.eri_2c_with(Engine.Auto)
 
+    def eri_2c_kernel(self, k):
+        """Basis::eri_2c_kernel (df.rs:95-128)."""
+        n = self.nao()
+        out = np.zeros(n * n)
+        _raise(library().ig_eri_2c_kernel(self._h, *k._args(), _ptr(out), out.size, None))
+        return out
+
     def schwarz_aux_bounds_with(self, engine):
         out = np.zeros(len(self._shells))
         _raise(library().ig_schwarz_aux_bounds(self._h, int(engine), _ptr(out), out.size, None))
@@ -432,9 +454,10 @@ class Basis:
     def eri_3c_builder(self, aux):
         return Eri3cBuilder(self, aux)
 
-    def eri_3c_block_with(self, engine, aux, ish, jsh, psh):
+    def eri_3c_block_with(self, engine, aux, ish, jsh, psh, kernel=None):
         """One (ij|P) block, row-major [n_i, n_j, n_p] (df.rs:148-199)."""
         b = Eri3cBuilder(self, aux, engine)
+        b._kernel = kernel or EriKernel.coulomb()
         hi, lo = (ish, jsh) if ish >= jsh else (jsh, ish)
         out = np.zeros(b.output_len())
         b.fill_range(hi * (hi + 1) // 2 + lo, hi * (hi + 1) // 2 + lo + 1, out)
@@ -446,6 +469,10 @@ class Basis:
 
     def eri_3c_block(self, aux, ish, jsh, psh):
         return self.eri_3c_block_with(Engine.Auto, aux, ish, jsh, psh)
+
+    def eri_3c_block_kernel(self, aux, ish, jsh, psh, k):
+        """Basis::eri_3c_block_kernel (df.rs:190-240)."""
+        return self.eri_3c_block_with(Engine.Auto, aux, ish, jsh, psh, kernel=k)
 
     def eri_grad_contract_with(self, engine, gamma, kernel=None, stream=None):
         """Basis::eri_grad_contract_with (grad.rs:426-466): per atom
@@ -499,6 +526,7 @@ class EriBuilder:
     def __init__(self, basis, engine=Engine.Auto):
         self._basis = basis
         self._engine = int(engine)
+        self._kernel = EriKernel.coulomb()
         nsh = len(basis._shells)
         self._pairs = [(i, j) for i in range(nsh) for j in range(i + 1)]
 
@@ -507,8 +535,12 @@ class EriBuilder:
         return cls(basis, engine)
 
     def kernel(self, k):
-        if k.omega is not None:
-            raise UnsupportedError("EriBuilder with the erf kernel has no GPU path in this build")
+        """EriBuilder::kernel (eri_builder.rs:280-305): the operator of every
+        later fill/build; erf needs a finite omega > 0 (checked here, as the
+        reference's panic)."""
+        if k.omega is not None and not (math.isfinite(k.omega) and k.omega > 0.0):
+            raise UsageError("EriKernel::Erf requires a finite omega > 0")
+        self._kernel = k
         return self
 
     def bra_pairs(self):
@@ -533,12 +565,15 @@ class EriBuilder:
     def fill_range(self, bra_begin, bra_end, out, stream=None):
         """The rows of the canonical bra pairs [bra_begin, bra_end) at once."""
         self._check(out)
-        _raise(library().ig_eri_builder_fill(self._basis._h, self._engine, int(bra_begin),
-                                             int(bra_end), _ptr(out), out.size,
-                                             _stream_ptr(stream)))
+        _raise(library().ig_eri_builder_fill_kernel(self._basis._h, self._engine,
+                                                    *self._kernel._args(), int(bra_begin),
+                                                    int(bra_end), _ptr(out), out.size,
+                                                    _stream_ptr(stream)))
 
     def build(self):
-        return self._basis.eri_with(self._engine)
+        if self._kernel.omega is None:
+            return self._basis.eri_with(self._engine)
+        return self._basis.eri_kernel(self._kernel)
 
 
 class Bra3cFill:
@@ -558,6 +593,7 @@ class Eri3cBuilder:
 
     def __init__(self, main, aux, engine=Engine.Auto):
         self._main, self._aux, self._engine = main, aux, int(engine)
+        self._kernel = EriKernel.coulomb()
         nsh = len(main._shells)
         self._pairs = [(i, j) for i in range(nsh) for j in range(i + 1)]
 
@@ -588,9 +624,10 @@ class Eri3cBuilder:
 
     def fill_range(self, bra_begin, bra_end, out, stream=None):
         self._check(out)
-        _raise(library().ig_eri_3c_fill(self._main._h, self._aux._h, self._engine,
-                                        int(bra_begin), int(bra_end), _ptr(out), out.size,
-                                        _stream_ptr(stream)))
+        _raise(library().ig_eri_3c_fill_kernel(self._main._h, self._aux._h, self._engine,
+                                               *self._kernel._args(), int(bra_begin),
+                                               int(bra_end), _ptr(out), out.size,
+                                               _stream_ptr(stream)))
 
     def build(self):
         out = np.zeros(self.output_len())

@@ -388,7 +388,7 @@ __global__ void __launch_bounds__(kBigThreads)
     next_quartet(qs, t, pb, pk);
     BigGeom g;
     g.load(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk]);
-    big_block(K, rb, p, R, pb, pk, g, 0.0, S, dyn, eoff, half0);
+    big_block(K, rb, p, R, pb, pk, g, eo.omega, S, dyn, eoff, half0);
     __syncthreads();
     double *c0 = half0, *c1 = half1;
     const int sh[4] = {g.sA, g.sB, g.sC, g.sD};

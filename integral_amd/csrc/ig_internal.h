@@ -237,6 +237,7 @@ struct EriOut {
   int mode = kEriDense;
   int nmain_sh = 0, dummy_sh0 = 0;
   long long nao_m = 0, naux = 0;
+  double omega = 0.0;  // the launch's operator: erf(omega r)/r when > 0, else 1/r
 };
 hipError_t launch_eri(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                       const RysDev &r, const QuartetSet &qs, const EriOut &o, hipStream_t s);

@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(64) eri_kernel(BasisDev b, PairsDev p, RysDev 
     double acc[C::E];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) acc[k] = 0.0;
-    quartet_block<C>(p, R, pb, pk, g, 0.0, tl, offs, acc, work);
+    quartet_block<C>(p, R, pb, pk, g, eo.omega, tl, offs, acc, work);
     double *cur = work, *nxt = work + C::N4;
 #pragma unroll
     for (int k = 0; k < C::E; ++k)

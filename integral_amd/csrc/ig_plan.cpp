@@ -1311,7 +1311,7 @@ extern "C" int ig_schwarz_bounds(const ig_basis *b, int engine, double *out, siz
 }
 
 static int eri_impl(const ig_basis *b, double tau, double *out, size_t len, int64_t *stats,
-                    void *stream) {
+                    void *stream, double omega = 0.0) {
   const size_t nao = size_t(b->nao);
   if (len != nao * nao * nao * nao) return fail(IG_ERR_LENGTH_MISMATCH, "eri length");
   return guarded([&] {
@@ -1326,7 +1326,8 @@ static int eri_impl(const ig_basis *b, double tau, double *out, size_t len, int6
     for (auto &cp : P->cached_cps)
       big = std::max(big, big_need(P->cls[cp.x], P->cls[cp.y], cp.total, 1));
     P->s_big.reserve(big * sizeof(double));
-    const EriOut eo{dout.as<double>(), nullptr, 0};
+    EriOut eo{dout.as<double>(), nullptr, 0};
+    eo.omega = omega;
     for (auto &cp : P->cached_cps) {
       if (!cp.total) continue;
       QuartetSet qs = implicit_set(*P, cp);
@@ -1465,7 +1466,7 @@ int packed_impl(const ig_basis *b, double tau, int64_t b0, int64_t b1, double *r
 // with P in the range is evaluated once -- Q <= P, or Q above the range --
 // and its block written to every permutation slot whose row pair is owned.
 static int eri_builder_impl(const ig_basis *b, int64_t b0, int64_t b1, double *out, size_t len,
-                            void *stream) {
+                            void *stream, double omega = 0.0) {
   const size_t nao = size_t(b->nao);
   if (len != nao * nao * nao * nao) return fail(IG_ERR_LENGTH_MISMATCH, "eri length");
   const int nsh = int(b->shells.size());
@@ -1521,6 +1522,7 @@ static int eri_builder_impl(const ig_basis *b, int64_t b0, int64_t b1, double *o
     EriOut eo{dout.as<double>(), nullptr, 0};
     eo.rows_begin = b0;
     eo.rows_end = b1;
+    eo.omega = omega;
     RysDevice &R = rys_device();
     size_t off = 0;
     for (size_t c = 0; c < cls.size(); ++c) {
@@ -1671,9 +1673,7 @@ int64_t canon_index(int64_t i, int64_t j) { return i >= j ? i * (i + 1) / 2 + j 
 
 }  // namespace
 
-extern "C" int ig_eri_2c(const ig_basis *aux, int engine, double *out, size_t len, void *stream) {
-  if (!aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
-  if (int rc = check_engine(engine)) return rc;
+static int eri_2c_impl(const ig_basis *aux, double *out, size_t len, void *stream, double omega) {
   const size_t naux = size_t(aux->nao);
   if (len != naux * naux) return fail(IG_ERR_LENGTH_MISMATCH, "2c metric length");
   return guarded([&] {
@@ -1693,6 +1693,7 @@ extern "C" int ig_eri_2c(const ig_basis *aux, int engine, double *out, size_t le
     eo.dummy_sh0 = na;
     eo.nao_m = 0;
     eo.naux = (long long)naux;
+    eo.omega = omega;
     df_run(*P, qs, eo, s);
     if (len)
       check(hipMemcpyAsync(out, dout.p, len * sizeof(double), hipMemcpyDeviceToHost, s),
@@ -1700,6 +1701,12 @@ extern "C" int ig_eri_2c(const ig_basis *aux, int engine, double *out, size_t le
     check(hipStreamSynchronize(s), "2c sync");
     return int(IG_OK);
   });
+}
+
+extern "C" int ig_eri_2c(const ig_basis *aux, int engine, double *out, size_t len, void *stream) {
+  if (!aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  return eri_2c_impl(aux, out, len, stream, 0.0);
 }
 
 extern "C" int ig_schwarz_aux_bounds(const ig_basis *aux, int engine, double *out, size_t len,
@@ -1722,11 +1729,8 @@ extern "C" int ig_schwarz_aux_bounds(const ig_basis *aux, int engine, double *ou
   return IG_OK;
 }
 
-extern "C" int ig_eri_3c_fill(const ig_basis *main, const ig_basis *aux, int engine,
-                              int64_t bra_begin, int64_t bra_end, double *out, size_t len,
-                              void *stream) {
-  if (!main || !aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
-  if (int rc = check_engine(engine)) return rc;
+static int eri_3c_impl(const ig_basis *main, const ig_basis *aux, int64_t bra_begin,
+                       int64_t bra_end, double *out, size_t len, void *stream, double omega) {
   const size_t nao = size_t(main->nao), naux = size_t(aux->nao);
   if (len != nao * nao * naux) return fail(IG_ERR_LENGTH_MISMATCH, "3c tensor length");
   const int nm = int(main->shells.size()), na = int(aux->shells.size());
@@ -1750,6 +1754,7 @@ extern "C" int ig_eri_3c_fill(const ig_basis *main, const ig_basis *aux, int eng
     eo.dummy_sh0 = nm + na;
     eo.nao_m = (long long)nao;
     eo.naux = (long long)naux;
+    eo.omega = omega;
     df_run(*P, qs, eo, s);
     // the rows of the range's bra pairs: (mu in i, nu in j) and (nu, mu),
     // each mu's run over nu contiguous
@@ -1771,6 +1776,71 @@ extern "C" int ig_eri_3c_fill(const ig_basis *main, const ig_basis *aux, int eng
     check(hipStreamSynchronize(s), "3c sync");
     return int(IG_OK);
   });
+}
+
+extern "C" int ig_eri_3c_fill(const ig_basis *main, const ig_basis *aux, int engine,
+                              int64_t bra_begin, int64_t bra_end, double *out, size_t len,
+                              void *stream) {
+  if (!main || !aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  return eri_3c_impl(main, aux, bra_begin, bra_end, out, len, stream, 0.0);
+}
+
+// EriKernel argument of the *_kernel entry points: Coulomb, or erf with a
+// finite omega > 0 (integrals.rs:316-321); returns the launch omega (0 for
+// Coulomb) or a status code through rc.
+static double kernel_omega(int kernel, double omega, int &rc) {
+  rc = IG_OK;
+  if (kernel == IG_KERNEL_COULOMB) return 0.0;
+  if (kernel != IG_KERNEL_ERF) {
+    rc = fail(IG_ERR_INVALID_ARGUMENT, "unknown kernel");
+    return 0.0;
+  }
+  if (!(std::isfinite(omega) && omega > 0.0)) {
+    rc = fail(IG_ERR_INVALID_OMEGA, "EriKernel::Erf requires a finite omega > 0");
+    return 0.0;
+  }
+  return omega;
+}
+
+extern "C" int ig_eri_kernel(const ig_basis *b, int kernel, double omega, double *out, size_t len,
+                             void *stream) {
+  if (!b || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  int rc;
+  const double w = kernel_omega(kernel, omega, rc);
+  if (rc) return rc;
+  return eri_impl(b, 0.0, out, len, nullptr, stream, w);
+}
+
+extern "C" int ig_eri_builder_fill_kernel(const ig_basis *b, int engine, int kernel, double omega,
+                                          int64_t bra_begin, int64_t bra_end, double *out,
+                                          size_t len, void *stream) {
+  if (!b || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  int rc;
+  const double w = kernel_omega(kernel, omega, rc);
+  if (rc) return rc;
+  return eri_builder_impl(b, bra_begin, bra_end, out, len, stream, w);
+}
+
+extern "C" int ig_eri_2c_kernel(const ig_basis *aux, int kernel, double omega, double *out,
+                                size_t len, void *stream) {
+  if (!aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  int rc;
+  const double w = kernel_omega(kernel, omega, rc);
+  if (rc) return rc;
+  return eri_2c_impl(aux, out, len, stream, w);
+}
+
+extern "C" int ig_eri_3c_fill_kernel(const ig_basis *main, const ig_basis *aux, int engine,
+                                     int kernel, double omega, int64_t bra_begin,
+                                     int64_t bra_end, double *out, size_t len, void *stream) {
+  if (!main || !aux || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (int rc = check_engine(engine)) return rc;
+  int rc;
+  const double w = kernel_omega(kernel, omega, rc);
+  if (rc) return rc;
+  return eri_3c_impl(main, aux, bra_begin, bra_end, out, len, stream, w);
 }
 
 extern "C" size_t ig_packed_eri_n_ao_pairs(const ig_basis *b) {

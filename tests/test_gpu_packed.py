@@ -140,5 +140,5 @@ def test_eri_builder_fills_own_rows(sph):
         assert np.max(np.abs(t[own] - dense.reshape(n, n, n, n)[own])) < 1e-14
     with pytest.raises(ig.UsageError):
         eb.partition(np.zeros(5))
-    with pytest.raises(ig.UnsupportedError):
-        eb.kernel(ig.EriKernel.erf(0.4))
+    with pytest.raises(ig.UsageError):  # omega must be finite and > 0
+        eb.kernel(ig.EriKernel.erf(-0.4))
