@@ -154,9 +154,11 @@ hipError_t launch_schwarz(int lcls1, int lcls2, const BasisDev &b, const PairsDe
 // block from the Boys function instead of Rys roots
 enum JkVariant { kJkTeam = 0, kJkReg = 1, kJkRowAcc = 2, kJkHgp = 3, kJkTeamHrr = 4, kJkNumVariants = 5 };
 // dynamic LDS of one row-accumulating workgroup: the device's per-workgroup
-// limit (160 KB on gfx950), read once; these kernels hold one workgroup per
-// CU at most anyway (over 256 VGPRs per lane)
+// limit (160 KB on gfx950), read once (the register-heavy classes hold one
+// workgroup per CU anyway; small accumulators leave room for several)
 size_t rowacc_lds_max();
+// compute units of the current device (sizes the row-sum kernels' grids)
+int device_cu_count();
 int jk_default_variant(int la, int lb, int lc, int ld);
 // width of the row accumulators' compact column space for ket momenta lc >= ld
 int rowacc_width(const BasisDev &b, int lc, int ld);

@@ -34,6 +34,18 @@ size_t rowacc_lds_max() {
   return v;
 }
 
+int device_cu_count() {
+  static const int v = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n < 1)
+      return 256;
+    return n;
+  }();
+  return v;
+}
+
 size_t rowacc_acc_doubles(const BasisDev &b, int la, int lb, int lc, int ld, int neff) {
   return rowacc_doubles(ncart(la), ncart(lb), neff, rowacc_width(b, lc, ld));
 }
@@ -69,7 +81,6 @@ hipError_t jk_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r, co
       const size_t acc = rowacc_doubles(C::na, C::nb, d.neff, rowacc_width(b, LC, LD));
       if (qs.list || acc * sizeof(double) > rowacc_lds_max()) return hipErrorInvalidValue;
       long long blocks = (qs.total + 64 * kRowWaves - 1) / (64 * kRowWaves);
-      if (blocks > 1024) blocks = 1024;
       return launch_jk_rowacc_slices<C, 0, F, 1>(b, p, r, qs, d, omega, ctr, s, int(blocks),
                                                  acc * sizeof(double));
     } else {
@@ -80,7 +91,6 @@ hipError_t jk_lane_one(const BasisDev &b, const PairsDev &p, const RysDev &r, co
     const size_t acc = rowacc_doubles(C::na, C::nb, d.neff, rowacc_width(b, LC, LD));
     if (qs.list || lds + acc * sizeof(double) > rowacc_lds_max()) return hipErrorInvalidValue;
     long long blocks = (qs.total + 64 * kRowWaves - 1) / (64 * kRowWaves);
-    if (blocks > 1024) blocks = 1024;
     return launch_jk_rowacc_slices<C, 0, F>(b, p, r, qs, d, omega, ctr, s, int(blocks),
                                             lds + acc * sizeof(double));
   }

@@ -395,7 +395,17 @@ hipError_t launch_jk_rowacc_slices(const BasisDev &b, const PairsDev &p, const R
   constexpr int NS = RegSlices<C, F>::NS, E = RegSlices<C, F>::E;
   if constexpr (S < NS) {
     constexpr int LO = S * E, HI = (S + 1) * E < C::N4 ? (S + 1) * E : C::N4;
-    hipLaunchKernelGGL((jk_rowacc_kernel<C, LO, HI, F, ENG>), dim3(blocks), dim3(64 * kRowWaves),
+    // about four rounds of the workgroups that fit at once: one workgroup per
+    // CU for the register-heavy classes, several for the small ones
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&jk_rowacc_kernel<C, LO, HI, F, ENG>),
+            64 * kRowWaves, lds) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    const long long cap = 4LL * device_cu_count() * per_cu;
+    const int grid = int(blocks < cap ? blocks : cap);
+    hipLaunchKernelGGL((jk_rowacc_kernel<C, LO, HI, F, ENG>), dim3(grid), dim3(64 * kRowWaves),
                        lds, s, b, p, r, qs, d, omega, ctr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
