@@ -64,41 +64,84 @@ def kernel_name(cls, variant):
                                                                    "s" if slices > 1 else "")
 
 
+def host_cpu():
+    """Host CPU facts recorded beside the CPU baseline: the model name, the
+    online CPUs (nproc) and the ones this process may run on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def cpu_threads():
+    """All the host cores this job may use: the process's CPU share
+    (OMP_NUM_THREADS, 16 per GPU on the GPU hosts), not every core the host
+    shows."""
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(n, share) if share > 0 else min(n, 16))
+
+
 def cpu_baseline(shells, dens, tau, seconds):
-    """Oracle on the host cores over a contiguous sample of canonical bra pairs
-    from the middle of the list, grown until it takes about `seconds`."""
+    """Oracle J/K on the host cores over bra-pair samples spread over the whole
+    canonical list (8 equal chunks at evenly spaced starts, grown together
+    until the run takes about `seconds`), on all the job's cores and then on
+    one core (a sample sized for about seconds/2)."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     ob = O.Basis(shells)
     q = ob.schwarz(threads=threads)
     bounds = ob.density_shell_bounds(dens)
     npairs = ob.nsh * (ob.nsh + 1) // 2
-    b0 = npairs // 2
+    starts = [int(npairs * (k + 0.5) / 8) for k in range(8)]
+
+    def run(m, nt):
+        t0 = time.perf_counter()
+        nq = 0
+        for b0 in starts:
+            ob.direct_jk(dens, tau, schwarz=q, bounds=bounds,
+                         bra_range=(b0, min(npairs, b0 + m)), threads=nt)
+            nq += ob.last_stats[0]
+        return nq, time.perf_counter() - t0
+
     m = 4
     while True:
-        t0 = time.perf_counter()
-        ob.direct_jk(dens, tau, schwarz=q, bounds=bounds, bra_range=(b0, b0 + m), threads=threads)
-        dt = time.perf_counter() - t0
-        nq = ob.last_stats[0]
-        log("cpu sample %d bra pairs: %.2f s" % (m, dt))
-        if dt >= seconds or b0 + m >= npairs:
+        nq, dt = run(m, threads)
+        log("cpu sample 8 x %d bra pairs on %d threads: %.2f s" % (m, threads, dt))
+        if dt >= seconds or m >= npairs // 8:
             break
         grow = max(2.0, min(8.0, seconds / max(dt, 1e-3)))
-        m = min(npairs - b0, int(m * grow) + 1)
-    return {"value": nq / dt, "unit": "quartets/s", "cores": threads, "kind": "port",
-            "sample": "canonical bra pairs [%d, %d) of %d (%d quartets, %.1f s), both densities, "
-                      "J and K, tau=1e-12" % (b0, b0 + m, npairs, nq, dt)}
+        m = min(npairs // 8, int(m * grow) + 1)
+    rate = nq / dt
+    m1 = max(1, int(m * min(1.0, 0.5 * seconds / max(dt * threads, 1e-3))))
+    nq1, dt1 = run(m1, 1)
+    log("cpu sample 8 x %d bra pairs on 1 thread: %.2f s" % (m1, dt1))
+    return {"value": rate, "unit": "quartets/s", "cores": threads, "kind": "port",
+            "one_core": nq1 / dt1, "host": host_cpu(),
+            "sample": "8 chunks of %d canonical bra pairs at evenly spaced starts of %d "
+                      "(%d quartets, %.1f s on %d threads; one core: 8 x %d pairs, %d quartets, "
+                      "%.1f s), both densities, J and K, tau=1e-12"
+                      % (m, npairs, nq, dt, threads, m1, nq1, dt1)}
 
 
-def bench_grad(args):
+def measure_grad(steps, warmup, with_cpu=False, cpu_seconds=15.0):
     """Seconds per geometry of forward J/K energy + backward forces on the
     30-atom s,s,p,p,d molecule (nao 390): plan build at the geometry, one J/K
     build from the device density, the energy, the gradient density and the
     adjoint gradient contraction.  Geometries are the starting one displaced by
-    0.01 bohr Gaussian noise per step, as in a geometry optimisation."""
+    0.01 bohr Gaussian noise per step, as in a geometry optimisation.  Also
+    the gradient kernels' per-class times (HIP events, one extra contraction)
+    and the roofline of the dominant class."""
     import torch
     import integral_amd as ig
-    from integral_amd import energy as EN, synthetic
+    from integral_amd import energy as EN, flops, synthetic
     cj, ck, tau = 1.0, -0.5, 1e-12
     shells, D = synthetic.gradient_config()
     tmpl, pos0 = EN.templates_from_shells(shells)
@@ -106,6 +149,7 @@ def bench_grad(args):
     stream = torch.cuda.current_stream(dev)
     Dd = torch.tensor(D[None], dtype=torch.float64, device=dev).contiguous()
     rng = np.random.default_rng(20260004)
+    keep = {}
 
     def geometry(p):
         t0 = time.perf_counter()
@@ -115,34 +159,64 @@ def bench_grad(args):
         t1 = time.perf_counter()
         J = torch.zeros_like(Dd)
         K = torch.zeros_like(Dd)
-        dc.jk_device(Dd, J, K, threshold=tau, stream=stream)
+        dc.jk_device(Dd, J, K, threshold=tau, stream=stream, symmetric=True)
         e = float((0.5 * cj * (Dd * J).sum() + 0.5 * ck * (Dd * K).sum()).item())
         t2 = time.perf_counter()
         dens = gc.density([ig.GradDensityTerm(0.5 * cj, D, D)], [ig.GradDensityTerm(0.5 * ck, D, D)])
         f = gc.contract(dens, threshold=tau)
         t3 = time.perf_counter()
+        keep["gc"], keep["dens"] = gc, dens
         return e, f, (t1 - t0, t2 - t1, t3 - t2), gc.last_stats()
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         geometry(pos0)
     torch.cuda.synchronize(dev)
     parts = np.zeros(3)
     t0 = time.perf_counter()
     p = pos0.copy()
-    for _ in range(args.steps):
+    for _ in range(steps):
         e, f, dt, gstats = geometry(p)
         parts += dt
         p = p + 0.01 * rng.standard_normal(p.shape)
     torch.cuda.synchronize(dev)
-    per = (time.perf_counter() - t0) / args.steps
-    parts /= args.steps
+    per = (time.perf_counter() - t0) / steps
+    parts /= steps
     log("grad: %.1f ms/geometry (plans %.1f, J/K energy %.1f, forces %.1f)"
         % (per * 1e3, parts[0] * 1e3, parts[1] * 1e3, parts[2] * 1e3))
+    gc, dens = keep["gc"], keep["dens"]
+    gc.set_profiling(True)
+    gc.contract(dens, threshold=tau)
+    recs = [r for r in gc.profile() if r["quartets"]]
+    gc.set_profiling(False)
+    for r in recs:
+        r["flops"] = r["quartets"] * flops.grad_quartet_flops(
+            r["cls"], r["prim_quartets"] / r["quartets"], nterms=2)
+    recs.sort(key=lambda r: -r["ms"])
+    dom = recs[0]
+    kern_ms = sum(r["ms"] for r in recs)
+    achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    roofline = {"bound": "valu", "pipe": "FP64 vector ALU (78.6 TFLOP/s)",
+                "kernel": "grad_lane_kernel<Cls<%s>, ...>" % ",".join(str(l) for l in dom["cls"]),
+                "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                "launch_ms": round(dom["ms"], 4), "launch_flops": dom["flops"],
+                "all_classes_ms": round(kern_ms, 4),
+                "all_classes_frac": round(sum(r["flops"] for r in recs) / (kern_ms * 1e-3) / 1e12
+                                          / FP64_PEAK_TFLOPS, 5)}
     cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline_grad(shells, D, cj, ck, tau, args.cpu_seconds)
-        # the whole geometry at the sampled rate (kernel work only)
+    if with_cpu:
+        cpu = cpu_baseline_grad(shells, D, cj, ck, tau, cpu_seconds)
         cpu["est_s_per_geometry"] = gstats[0] / cpu["value"]
+    return {"s_per_geometry": per, "plan_ms": parts[0] * 1e3, "forward_ms": parts[1] * 1e3,
+            "backward_ms": parts[2] * 1e3, "grad_quartets": gstats[0], "roofline": roofline,
+            "classes": [{"cls": "".join("spdfghi"[l] for l in r["cls"]), "quartets": r["quartets"],
+                         "ms": round(r["ms"], 4)} for r in recs[:12]], "cpu_baseline": cpu}
+
+
+def bench_grad(args):
+    """The gradient workload on its own JSON line (see measure_grad)."""
+    g = measure_grad(args.steps, args.warmup, not args.no_cpu_baseline, args.cpu_seconds)
+    per = g["s_per_geometry"]
     out = {
         "metric": METRIC, "value": per, "unit": "s/geometry", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": per * 1e3, "higher_is_better": False,
@@ -151,10 +225,12 @@ def bench_grad(args):
         "config": {"workload": "J/K energy forward + adjoint forces backward, 30 atoms in a 12 "
                                "bohr cube, spherical s,s,p,p,d (3,3,2,2,1 primitives), nao=390, "
                                "Coulomb 1.0 / exchange -0.5, tau=1e-12, FP64",
-                   "plan_ms": parts[0] * 1e3, "forward_ms": parts[1] * 1e3,
-                   "backward_ms": parts[2] * 1e3, "grad_quartets": gstats[0],
+                   "plan_ms": g["plan_ms"], "forward_ms": g["forward_ms"],
+                   "backward_ms": g["backward_ms"], "grad_quartets": g["grad_quartets"],
                    "parallelism": "1 GPU"},
-        "cpu_baseline": cpu,
+        "roofline": g["roofline"],
+        "cpu_baseline": g["cpu_baseline"],
+        "classes": g["classes"],
     }
     print(json.dumps(out))
 
@@ -426,11 +502,62 @@ def bench_eri(args):
                                  "quartets": quartets, "parallelism": "1 GPU"}}))
 
 
+def measure_eri(steps, warmup):
+    """Milliseconds per dense ERI tensor of the 10-atom Cartesian s,s,p
+    molecule (nao 50): plan, kernels and the 50 MB tensor to host memory."""
+    from integral_amd import synthetic
+    shells, _ = synthetic.dense_eri_config()
+    basis = synthetic.to_basis(shells)
+    for _ in range(warmup):
+        basis.eri()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        basis.eri()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    nsh = len(shells)
+    npair = nsh * (nsh + 1) // 2
+    log("dense eri: %.2f ms per tensor" % ms)
+    return {"ms_per_tensor": ms, "quartets": npair * (npair + 1) // 2, "nao": 50}
+
+
+def measure_highl(steps, warmup):
+    """Milliseconds per J/K build of the nao 565 molecule with g shells and an
+    l = 6 shell on every fourth atom (one density, tau 1e-12, FP64, built-in
+    schedule), and the per-class split of one build."""
+    import torch
+    from integral_amd import synthetic
+    shells, D = synthetic.high_l_config()
+    dc = synthetic.to_basis(shells).direct_contractor()
+    dev = torch.device("cuda:0")
+    Dd = torch.tensor(D[None], dtype=torch.float64, device=dev).contiguous()
+    J, K = torch.zeros_like(Dd), torch.zeros_like(Dd)
+    for _ in range(warmup):
+        dc.jk_device(Dd, J, K, threshold=1e-12, symmetric=True)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dc.jk_device(Dd, J, K, threshold=1e-12, symmetric=True)
+    torch.cuda.synchronize(dev)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    dc.set_profiling(True)
+    dc.jk_device(Dd, J, K, threshold=1e-12, symmetric=True)
+    torch.cuda.synchronize(dev)
+    recs = {}
+    for r in dc.profile():
+        key = "".join("spdfghi"[l] for l in r["cls"])
+        recs[key] = recs.get(key, 0.0) + r["ms"]
+    dc.set_profiling(False)
+    log("high-l: %.1f ms per J/K build" % ms)
+    top = sorted(recs.items(), key=lambda kv: -kv[1])[:8]
+    return {"ms_per_build": ms, "quartets": dc.last_stats()[0], "nao": 565,
+            "classes_ms": {k: round(v, 3) for k, v in top}}
+
+
 def cpu_baseline_grad(shells, D, cj, ck, tau, seconds):
     """Oracle adjoint gradient on the host cores over a contiguous bra-pair
     sample, reported as gradient quartets per second."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     ob = O.Basis(shells)
     npairs = ob.nsh * (ob.nsh + 1) // 2
     b0, m = npairs // 2, 8
@@ -469,6 +596,13 @@ def main():
                          "the J/K energy and its adjoint forces; highl: J/K with shells up to "
                          "l = 6; eri: the dense ERI tensor; batch: many molecules' J/K in "
                          "one HIP graph")
+    ap.add_argument("--ndens", type=int, default=2, choices=[1, 2],
+                    help="jk workload: densities per build (the workload has two)")
+    ap.add_argument("--coulomb-only", action="store_true",
+                    help="jk workload: build J only (no exchange)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="jk workload: skip the other workloads' figures (gradient per "
+                         "geometry, dense ERI tensor, high-l J/K) reported beside the headline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once and exit: the reference "
                          "kernel of known traffic that scales the profiler's FETCH_SIZE and "
@@ -508,15 +642,19 @@ def main():
     dc = ig.DirectContractor(basis)
     plan_s = time.perf_counter() - t0
     dev = torch.device("cuda:0")
+    dens = dens[:args.ndens]
     D = torch.tensor(np.stack(dens), dtype=torch.float64, device=dev).contiguous()
     J = torch.zeros_like(D)
     K = torch.zeros_like(D)
+    Kout = None if args.coulomb_only else K
     stream = torch.cuda.current_stream(dev)
 
     def step():
+        # D = C C^T is symmetric by construction: declared, so the build
+        # never synchronises the host
         J.zero_()
         K.zero_()
-        dc.jk_device(D, J, K, threshold=tau, stream=stream)
+        dc.jk_device(D, J, Kout, threshold=tau, stream=stream, symmetric=True)
 
     log("plan built in %.2f s (nao=%d)" % (plan_s, nao))
     tune_s = 0.0
@@ -574,12 +712,13 @@ def main():
     dc.set_profiling(False)
     recs = sorted(acc.values(), key=lambda r: -r["ms"])
     dom = recs[0]
-    dom_flops = class_flops(dom)
-    total_flops = sum(class_flops(r) for r in recs)
+    nd_flops = args.ndens * (0.5 if args.coulomb_only else 1.0)  # the digest's share
+    dom_flops = class_flops(dom, ndens=nd_flops)
+    total_flops = sum(class_flops(r, ndens=nd_flops) for r in recs)
     kern_ms = sum(r["ms"] for r in recs)
     achieved = dom_flops / (dom["ms"] * 1e-3) / 1e12
     roofline = {
-        "bound": "mfma", "pipe": "FP64 vector ALU (78.6 TFLOP/s, same as dense FP64 matrix)",
+        "bound": "valu", "pipe": "FP64 vector ALU (78.6 TFLOP/s; no MFMA in this kernel)",
         "kernel": kernel_name(dom["cls"], sched.get(tuple(dom["cls"]), 0)),
         "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
@@ -601,6 +740,10 @@ def main():
         pass
     cpu = None
     log("profiled classes; dominant %s %.3f ms" % (dom["cls"], dom["ms"]))
+    extras = None
+    if not args.no_extras:
+        extras = {"grad": measure_grad(3, 1), "eri": measure_eri(3, 1),
+                  "highl": measure_highl(2, 1)}
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(shells, dens, tau, args.cpu_seconds)
     out = {
@@ -616,10 +759,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded molecule and densities, no checkpoint)",
-        "config": {"workload": "integral-direct J/K, 40 atoms in a 14 bohr cube, spherical "
-                               "s,s,p,p,d (3,3,3,3,1 primitives), nao=%d, 2 densities, "
-                               "tau=1e-12, FP64" % nao,
-                   "nao": nao, "nshell": len(shells), "ndens": 2, "tau": tau,
+        "config": {"workload": "integral-direct J%s, 40 atoms in a 14 bohr cube, spherical "
+                               "s,s,p,p,d (3,3,3,3,1 primitives), nao=%d, %d densit%s, "
+                               "tau=1e-12, FP64" % ("" if args.coulomb_only else "/K", nao,
+                                                    args.ndens, "ies" if args.ndens > 1 else "y"),
+                   "nao": nao, "nshell": len(shells), "ndens": args.ndens, "tau": tau,
+                   "exchange": not args.coulomb_only,
                    "quartets": quartets, "prim_quartets": prim_quartets,
                    "jk_build_s": ms * 1e-3, "plan_build_s": plan_s,
                    "schedule": "autotuned" if not args.no_autotune else "built-in",
@@ -630,6 +775,7 @@ def main():
         "classes": [{"cls": "".join("spdfghi"[l] for l in r["cls"]), "quartets": r["quartets"],
                      "variant": VARIANTS[sched.get(tuple(r["cls"]), 0)],
                      "ms": round(r["ms"], 4)} for r in recs],
+        "other_workloads": extras,
     }
     print(json.dumps(out))
 

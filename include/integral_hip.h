@@ -224,13 +224,60 @@ int ig_direct_jk(const ig_direct *direct, const double *const *densities, size_t
                  double *const *exchange, int64_t bra_begin, int64_t bra_end, void *stream);
 /* Device-resident variant of ig_direct_jk for the full bra range: d_dens,
  * d_J, d_K are DEVICE pointers to ndens contiguous nao*nao matrices each
- * (d_K may be NULL when want_exchange == 0).  Asynchronous on `stream`. */
+ * (d_K may be NULL when want_exchange == 0).  The densities' exact symmetry
+ * is tested on the device and its 4-byte flag read back, which synchronises
+ * `stream` once per call; the kernels then run asynchronously on `stream`.
+ * ig_direct_jk_device_ws takes the symmetry from the caller and never
+ * synchronises. */
 int ig_direct_jk_device(const ig_direct *direct, const double *d_dens, size_t ndens,
                         double threshold, int want_exchange, double *d_J, double *d_K,
                         void *stream);
 /* Number of canonical quartets accepted by the screening of the last call on
  * this plan ([0]) and primitive quartets evaluated ([1]). */
 int ig_direct_last_stats(const ig_direct *direct, int64_t *stats2);
+
+/* DirectWorkspace (direct.rs:19-33, ::new at :30): per-worker scratch.  The
+ * reference lets many workers call accumulate_bra_pair(&self, &mut
+ * DirectWorkspace, ...) on one shared contractor; here a workspace holds its
+ * own device scratch over the contractor's read-only plan (shell data, pair
+ * tables, Schwarz bounds), so calls through different workspaces run
+ * concurrently, each on its own stream.  Calls through one workspace (or
+ * with workspace NULL, through the contractor's own scratch) are ordered on
+ * the device: a call waits for the previous call's device work even when it
+ * came on another stream.  The schedule and precision policy are the
+ * contractor's, read at each call.  A workspace is tied to the contractor it
+ * was created for (IG_ERR_INVALID_ARGUMENT otherwise) and must be destroyed
+ * before it. */
+typedef struct ig_direct_workspace ig_direct_workspace;
+int ig_direct_workspace_create(const ig_direct *direct, ig_direct_workspace **out);
+/* Waits for the workspace's last device work, then frees it. */
+void ig_direct_workspace_destroy(ig_direct_workspace *workspace);
+/* DirectContractor::accumulate_bra_pair (direct.rs:131-218) with an explicit
+ * workspace; as ig_direct_accumulate_bra_pair otherwise (NULL: the
+ * contractor's own scratch). */
+int ig_direct_accumulate_bra_pair_ws(const ig_direct *direct, ig_direct_workspace *workspace,
+                                     int32_t i, int32_t j, const double *const *densities,
+                                     const double *bounds, size_t ndens, double threshold,
+                                     int want_exchange, double *const *coulomb,
+                                     double *const *exchange, void *stream);
+/* ig_direct_jk with an explicit workspace. */
+int ig_direct_jk_ws(const ig_direct *direct, ig_direct_workspace *workspace,
+                    const double *const *densities, size_t ndens, double threshold,
+                    int want_exchange, double *const *coulomb, double *const *exchange,
+                    int64_t bra_begin, int64_t bra_end, void *stream);
+/* Fully asynchronous device-resident J/K (the performance path): as
+ * ig_direct_jk_device, with nothing read back to the host.  dens_symmetric
+ * != 0 declares every density symmetric (the caller's guarantee; the
+ * antisymmetric part is then not formed); 0 runs the general path, exact for
+ * any density (direct_contractor.rs:30-91).  workspace may be NULL (the
+ * contractor's own scratch). */
+int ig_direct_jk_device_ws(const ig_direct *direct, ig_direct_workspace *workspace,
+                           const double *d_dens, size_t ndens, double threshold,
+                           int want_exchange, double *d_J, double *d_K, int dens_symmetric,
+                           void *stream);
+/* ig_direct_last_stats of the last call through a workspace (synchronises
+ * that call's stream). */
+int ig_direct_workspace_last_stats(ig_direct_workspace *workspace, int64_t *stats2);
 
 /* Per-class attribution of the last J/K call (the analogue of the
  * reference's `eri_profile` classes mode, benches/examples/eri_profile.rs:
@@ -365,6 +412,26 @@ int ig_grad_accumulate_bra_pair(const ig_grad *grad, const ig_grad_density *dens
 int ig_grad_contract(const ig_grad *grad, const ig_grad_density *density, int kernel,
                      double omega, double threshold, double *out, size_t out_len, void *stream);
 int ig_grad_last_stats(const ig_grad *grad, int64_t *stats2);
+/* Per-class attribution of the last gradient contraction, as
+ * ig_direct_set_profiling / ig_direct_profile_read for J/K (quartets and
+ * primitive quartets per class launch; kernel time from HIP events when
+ * profiling is enabled). */
+int ig_grad_set_profiling(ig_grad *grad, int enable);
+int ig_grad_profile_read(const ig_grad *grad, ig_class_profile *out, size_t cap, size_t *n);
+/* DirectGradWorkspace (direct_grad.rs:51-67): per-worker scratch over a
+ * gradient contractor's read-only plan, as ig_direct_workspace. */
+typedef struct ig_grad_workspace ig_grad_workspace;
+int ig_grad_workspace_create(const ig_grad *grad, ig_grad_workspace **out);
+void ig_grad_workspace_destroy(ig_grad_workspace *workspace);
+/* ig_grad_accumulate_bra_pair / ig_grad_contract through a workspace (NULL:
+ * the contractor's own scratch). */
+int ig_grad_accumulate_bra_pair_ws(const ig_grad *grad, ig_grad_workspace *workspace,
+                                   const ig_grad_density *density, int32_t i, int32_t j,
+                                   int kernel, double omega, double threshold, double *out,
+                                   size_t out_len, void *stream);
+int ig_grad_contract_ws(const ig_grad *grad, ig_grad_workspace *workspace,
+                        const ig_grad_density *density, int kernel, double omega,
+                        double threshold, double *out, size_t out_len, void *stream);
 
 /* ---- dense two-particle-density gradient: Basis::eri_grad_contract ------ */
 /* Basis::eri_grad_contract / eri_grad_contract_with / eri_grad_contract_kernel

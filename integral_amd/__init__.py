@@ -186,6 +186,17 @@ def library():
     lib.ig_direct_jk_device.argtypes = [_vp, _vp, sz, ctypes.c_double, ctypes.c_int, _vp, _vp,
                                         _vp]
     lib.ig_direct_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
+    lib.ig_direct_workspace_create.argtypes = [_vp, ctypes.POINTER(_vp)]
+    lib.ig_direct_workspace_destroy.argtypes = [_vp]
+    lib.ig_direct_workspace_destroy.restype = None
+    lib.ig_direct_accumulate_bra_pair_ws.argtypes = [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
+                                                     _dpp, _dp, sz, ctypes.c_double, ctypes.c_int,
+                                                     _dpp, _dpp, _vp]
+    lib.ig_direct_jk_ws.argtypes = [_vp, _vp, _dpp, sz, ctypes.c_double, ctypes.c_int, _dpp, _dpp,
+                                    i64, i64, _vp]
+    lib.ig_direct_jk_device_ws.argtypes = [_vp, _vp, _vp, sz, ctypes.c_double, ctypes.c_int, _vp,
+                                           _vp, ctypes.c_int, _vp]
+    lib.ig_direct_workspace_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
     lib.ig_direct_set_profiling.argtypes = [_vp, ctypes.c_int]
     lib.ig_direct_profile_read.argtypes = [_vp, ctypes.POINTER(_ClassProfile), sz,
                                            ctypes.POINTER(sz)]
@@ -242,6 +253,17 @@ def library():
     lib.ig_grad_contract.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      _dp, sz, _vp]
     lib.ig_grad_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
+    lib.ig_grad_set_profiling.argtypes = [_vp, ctypes.c_int]
+    lib.ig_grad_profile_read.argtypes = [_vp, ctypes.POINTER(_ClassProfile), sz,
+                                         ctypes.POINTER(sz)]
+    lib.ig_grad_workspace_create.argtypes = [_vp, ctypes.POINTER(_vp)]
+    lib.ig_grad_workspace_destroy.argtypes = [_vp]
+    lib.ig_grad_workspace_destroy.restype = None
+    lib.ig_grad_accumulate_bra_pair_ws.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32,
+                                                   ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                   _dp, sz, _vp]
+    lib.ig_grad_contract_ws.argtypes = [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, _dp, sz, _vp]
     lib.ig_eri_grad_contract_device.argtypes = [_vp, ctypes.c_int, _vp, sz, ctypes.c_int,
                                                 ctypes.c_double, _dp, sz, _vp]
     _lib = lib
@@ -278,6 +300,29 @@ def _stream_ptr(stream):
     if stream is None:
         return None
     return ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
+
+
+def _check_device_jk(nao, dens, J=None, K=None):
+    """Shape checks of the device J/K entry points, done before any library
+    call (the kernels trust ndens x nao x nao): dens is a contiguous float64
+    CUDA tensor (ndens, nao, nao) with 1 <= ndens <= 64 (direct.rs:141-148),
+    J and K (when given) have its shape, dtype and device."""
+    def ok(t):
+        return (getattr(t, "is_cuda", False) and str(getattr(t, "dtype", "")) == "torch.float64"
+                and t.is_contiguous())
+    if not ok(dens):
+        raise UsageError("densities must be a contiguous float64 CUDA tensor")
+    shape = tuple(dens.shape)
+    if len(shape) != 3 or shape[1:] != (nao, nao):
+        raise UsageError("densities must have shape (ndens, %d, %d), got %s" % (nao, nao, shape))
+    if not 1 <= shape[0] <= 64:
+        raise UsageError("between 1 and 64 densities, got %d" % shape[0])
+    for name, t in (("J", J), ("K", K)):
+        if t is None:
+            continue
+        if not ok(t) or tuple(t.shape) != shape or t.device != dens.device:
+            raise UsageError("%s must be a contiguous float64 CUDA tensor of shape %s on %s"
+                             % (name, shape, dens.device))
 
 
 def boys(m_max, T):
@@ -715,8 +760,46 @@ class PackedEriBuilder:
         return out
 
 
-class DirectWorkspace:
-    """direct.rs:22-33.  The device plan owns all scratch; kept for API parity."""
+class _Workspace:
+    """Per-worker device scratch bound to one contractor on first use (the
+    reference's workspaces grow on demand and serve any contractor; here a
+    workspace used with another contractor rebinds to it)."""
+
+    _create = _destroy = None
+
+    def __init__(self):
+        self._h = None
+        self._owner = None
+
+    def _handle(self, contractor):
+        if self._owner is not contractor:
+            self._release()
+            h = _vp()
+            _raise(getattr(library(), self._create)(contractor._h, ctypes.byref(h)))
+            self._h, self._owner = h, contractor
+        return self._h
+
+    def _release(self):
+        if self._h is not None and _lib is not None:
+            getattr(_lib, self._destroy)(self._h)
+        self._h = self._owner = None
+
+    def __del__(self):
+        self._release()
+
+
+class DirectWorkspace(_Workspace):
+    """DirectWorkspace (direct.rs:19-33): a worker's own device scratch, so
+    workers sharing one DirectContractor run concurrently."""
+
+    _create, _destroy = "ig_direct_workspace_create", "ig_direct_workspace_destroy"
+
+    def last_stats(self):
+        if self._h is None:
+            return 0, 0
+        st = (ctypes.c_int64 * 2)()
+        _raise(library().ig_direct_workspace_last_stats(self._h, st))
+        return int(st[0]), int(st[1])
 
 
 class DirectBuffers:
@@ -778,13 +861,14 @@ class DirectContractor:
         ks = [o.exchange for o in outputs] if want_exchange else []
         self._check_out(js + ks, n2)
         bounds = _f64(np.concatenate([np.ravel(b) for b in density_bounds])) if ds else _f64([0.0])
-        _raise(library().ig_direct_accumulate_bra_pair(
-            self._h, int(bra[0]), int(bra[1]), _ptrs(ds), _ptr(bounds), len(ds), float(threshold),
-            1 if want_exchange else 0, _ptrs(js), _ptrs(ks) if want_exchange else None,
-            _stream_ptr(stream)))
+        ws = workspace._handle(self) if isinstance(workspace, DirectWorkspace) else None
+        _raise(library().ig_direct_accumulate_bra_pair_ws(
+            self._h, ws, int(bra[0]), int(bra[1]), _ptrs(ds), _ptr(bounds), len(ds),
+            float(threshold), 1 if want_exchange else 0, _ptrs(js),
+            _ptrs(ks) if want_exchange else None, _stream_ptr(stream)))
 
     def jk(self, densities, threshold=0.0, want_exchange=True, bra_range=(0, -1), J=None, K=None,
-           stream=None):
+           stream=None, workspace=None):
         """Bulk J/K over canonical bra pairs [bra_range) (all by default);
         accumulates into J/K when given, else returns fresh matrices."""
         ds = [_f64(d).ravel() for d in densities]
@@ -792,23 +876,33 @@ class DirectContractor:
         J = [np.zeros(n * n) for _ in ds] if J is None else J
         K = ([np.zeros(n * n) for _ in ds] if K is None else K) if want_exchange else None
         self._check_out(J + (K or []), n * n)
-        _raise(library().ig_direct_jk(self._h, _ptrs(ds), len(ds), float(threshold),
-                                      1 if want_exchange else 0, _ptrs(J),
-                                      _ptrs(K) if want_exchange else None, int(bra_range[0]),
-                                      int(bra_range[1]), _stream_ptr(stream)))
+        ws = workspace._handle(self) if workspace is not None else None
+        _raise(library().ig_direct_jk_ws(self._h, ws, _ptrs(ds), len(ds), float(threshold),
+                                         1 if want_exchange else 0, _ptrs(J),
+                                         _ptrs(K) if want_exchange else None, int(bra_range[0]),
+                                         int(bra_range[1]), _stream_ptr(stream)))
         return J, K
 
-    def jk_device(self, dens, J, K=None, threshold=0.0, stream=None):
+    def jk_device(self, dens, J, K=None, threshold=0.0, stream=None, workspace=None,
+                  symmetric=None):
         """Device-resident J/K: dens, J, K are torch.float64 CUDA tensors of
-        shape (ndens, nao, nao); J/K are accumulated.  Asynchronous."""
-        for t in [dens, J] + ([K] if K is not None else []):
-            if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8
-                    and t.is_contiguous()):
-                raise UsageError("device buffers must be contiguous float64 CUDA tensors")
-        _raise(library().ig_direct_jk_device(
-            self._h, _vp(dens.data_ptr()), dens.shape[0], float(threshold),
+        shape (ndens, nao, nao); J/K are accumulated.  With symmetric=None
+        the densities' symmetry is tested on the device (one 4-byte read
+        back); True/False declares it and the call never synchronises.
+        `workspace` (a DirectWorkspace) gives the call its own scratch."""
+        _check_device_jk(self.nao(), dens, J, K)
+        if symmetric is None and workspace is None:
+            _raise(library().ig_direct_jk_device(
+                self._h, _vp(dens.data_ptr()), dens.shape[0], float(threshold),
+                1 if K is not None else 0, _vp(J.data_ptr()),
+                _vp(K.data_ptr()) if K is not None else None, _stream_ptr(stream)))
+            return
+        ws = workspace._handle(self) if workspace is not None else None
+        _raise(library().ig_direct_jk_device_ws(
+            self._h, ws, _vp(dens.data_ptr()), dens.shape[0], float(threshold),
             1 if K is not None else 0, _vp(J.data_ptr()),
-            _vp(K.data_ptr()) if K is not None else None, _stream_ptr(stream)))
+            _vp(K.data_ptr()) if K is not None else None, 0 if symmetric is False else 1,
+            _stream_ptr(stream)))
 
     def last_stats(self):
         st = (ctypes.c_int64 * 2)()
@@ -825,8 +919,7 @@ class DirectContractor:
         """Time every J/K kernel variant of every class pair on the device
         densities `dens` (float64 CUDA tensor, ndens x nao x nao) and keep the
         faster per class pair; returns the resulting schedule."""
-        if not (dens.is_cuda and dens.element_size() == 8 and dens.is_contiguous()):
-            raise UsageError("densities must be a contiguous float64 CUDA tensor")
+        _check_device_jk(self.nao(), dens)
         _raise(library().ig_direct_autotune(self._h, _vp(dens.data_ptr()), dens.shape[0],
                                             float(threshold), 1 if want_exchange else 0,
                                             int(reps), _stream_ptr(stream)))
@@ -966,8 +1059,11 @@ class GradDensityTerm:
         self.right = right
 
 
-class DirectGradWorkspace:
-    """direct_grad.rs:51-67.  The device plan owns all scratch."""
+class DirectGradWorkspace(_Workspace):
+    """DirectGradWorkspace (direct_grad.rs:51-67): a worker's own device
+    scratch over a shared DirectGradContractor."""
+
+    _create, _destroy = "ig_grad_workspace_create", "ig_grad_workspace_destroy"
 
 
 class GradDensity:
@@ -1032,19 +1128,35 @@ class DirectGradContractor:
         if out.dtype != np.float64 or not out.flags.c_contiguous:
             raise UsageError("out must be a C-contiguous float64 array of natom x 3")
         kernel = kernel or EriKernel.coulomb()
-        _raise(library().ig_grad_accumulate_bra_pair(self._h, density._h, int(bra[0]),
-                                                     int(bra[1]), *kernel._args(),
-                                                     float(threshold), _ptr(out), out.size,
-                                                     _stream_ptr(stream)))
+        ws = workspace._handle(self) if isinstance(workspace, DirectGradWorkspace) else None
+        _raise(library().ig_grad_accumulate_bra_pair_ws(self._h, ws, density._h, int(bra[0]),
+                                                        int(bra[1]), *kernel._args(),
+                                                        float(threshold), _ptr(out), out.size,
+                                                        _stream_ptr(stream)))
 
-    def contract(self, density, kernel=None, threshold=0.0, stream=None):
+    def contract(self, density, kernel=None, threshold=0.0, stream=None, workspace=None):
         kernel = kernel or EriKernel.coulomb()
         out = np.zeros((self.natom(), 3))
-        _raise(library().ig_grad_contract(self._h, density._h, *kernel._args(), float(threshold),
-                                          _ptr(out), out.size, _stream_ptr(stream)))
+        ws = workspace._handle(self) if workspace is not None else None
+        _raise(library().ig_grad_contract_ws(self._h, ws, density._h, *kernel._args(),
+                                             float(threshold), _ptr(out), out.size,
+                                             _stream_ptr(stream)))
         return out
 
     def last_stats(self):
         st = (ctypes.c_int64 * 2)()
         _raise(library().ig_grad_last_stats(self._h, st))
         return int(st[0]), int(st[1])
+
+    def set_profiling(self, enable):
+        _raise(library().ig_grad_set_profiling(self._h, 1 if enable else 0))
+
+    def profile(self):
+        """Per-class records of the last contraction (as
+        DirectContractor.profile)."""
+        cap = 1024
+        buf = (_ClassProfile * cap)()
+        n = ctypes.c_size_t()
+        _raise(library().ig_grad_profile_read(self._h, buf, cap, ctypes.byref(n)))
+        return [{"cls": (r.la, r.lb, r.lc, r.ld), "quartets": r.quartets,
+                 "prim_quartets": r.prim_quartets, "ms": r.ms} for r in buf[:min(n.value, cap)]]

@@ -52,10 +52,21 @@ __host__ __device__ inline int table_size(int la, int lb, int lc, int ld) {
 // Same recurrences as rys_axis_table (engine/rys.rs:234-308), runtime extents:
 // vertical recurrence on (bra, ket) combined powers into g, then the bra and
 // ket horizontal transfers; out[((i*(lb+1)+j)*(lc+1)+k)*(ld+1)+l].
-__device__ void table_rt(int la, int lb, int lc, int ld, double c00, double cp0, double b10,
-                         double b01, double b00, double ab, double cd, double *g, double *t,
-                         double *out) {
+// With swap_b the bra recurrence builds on the second shell's centre (c00
+// from P - B, ab = B - A) and the transfer moves momentum to the first; with
+// swap_k likewise on the ket (cp0 from Q - D, cd = D - C).  The output keeps
+// the standard layout either way.  The caller builds on the centre of the
+// tighter primitive: the transfer onto a tight shell across a long AB
+// cancels catastrophically (an l = 4 shell of exponent 5.3 paired with a
+// diffuse l = 6 one 5.6 bohr away: building on the l = 6 shell loses 5e-7
+// of the block, building on the tight one 5e-14, against a 40-digit
+// McMurchie-Davidson value).
+__device__ void table_rt(int la, int lb, int lc, int ld, bool swap_b, bool swap_k, double c00,
+                         double cp0, double b10, double b01, double b00, double ab, double cd,
+                         double *g, double *t, double *out) {
   const int NB = la + lb, NK = lc + ld, W = NK + 1;
+  const int ea = swap_b ? lb : la, eb = swap_b ? la : lb;  // built / received bra momenta
+  const int ec = swap_k ? ld : lc, ed = swap_k ? lc : ld;
   g[0] = 1.0;
   if (NB >= 1) g[W] = c00;
   for (int n = 1; n < NB; ++n) g[(n + 1) * W] = c00 * g[n * W] + double(n) * b10 * g[(n - 1) * W];
@@ -66,16 +77,20 @@ __device__ void table_rt(int la, int lb, int lc, int ld, double c00, double cp0,
       if (n >= 1) v += double(n) * b00 * g[(n - 1) * W + m];
       g[n * W + m + 1] = v;
     }
-  for (int j = 0; j <= lb; ++j) {
+  for (int j = 0; j <= eb; ++j) {
     if (j > 0)  // bra transfer: I(i, j) = I(i+1, j-1) + AB I(i, j-1), in place over i
       for (int i = 0; i <= NB - j; ++i)
         for (int m = 0; m <= NK; ++m) g[i * W + m] = g[(i + 1) * W + m] + ab * g[i * W + m];
-    for (int i = 0; i <= la; ++i) {
+    for (int i = 0; i <= ea; ++i) {
       for (int k = 0; k <= NK; ++k) t[k] = g[i * W + k];
-      for (int l = 0; l <= ld; ++l) {
+      const int a = swap_b ? j : i, bb = swap_b ? i : j;
+      for (int l = 0; l <= ed; ++l) {
         if (l > 0)
           for (int k = 0; k <= NK - l; ++k) t[k] = t[k + 1] + cd * t[k];
-        for (int k = 0; k <= lc; ++k) out[((i * (lb + 1) + j) * (lc + 1) + k) * (ld + 1) + l] = t[k];
+        for (int k = 0; k <= ec; ++k) {
+          const int c = swap_k ? l : k, d = swap_k ? k : l;
+          out[((a * (lb + 1) + bb) * (lc + 1) + c) * (ld + 1) + d] = t[k];
+        }
       }
     }
   }
@@ -104,7 +119,7 @@ struct BigClass {
 
 // Primitive-quartet scalars (same expressions as quartet_block in ig_kernels.hip).
 struct PrimQ {
-  double pz, qz, ipq, i2p, i2q, P[3], Q[3], PQ[3], pref, T, s_att, al, be, ga;
+  double pz, qz, ipq, i2p, i2q, P[3], Q[3], PQ[3], pref, T, s_att, al, be, ga, de;
 };
 
 __device__ __forceinline__ void prim_quartet(const PairsDev &p, int ob, int ib, int ok, int ik,
@@ -120,6 +135,7 @@ __device__ __forceinline__ void prim_quartet(const PairsDev &p, int ob, int ib, 
   q.al = p.alpha[ob + ib];
   q.be = p.beta[ob + ib];
   q.ga = p.alpha[ok + ik];
+  q.de = p.beta[ok + ik];
   const double rs = rsqrt(q.pz + q.qz);
   q.ipq = rs * rs;
   q.i2p = p.i2z[ob + ib];
@@ -180,10 +196,16 @@ __device__ __forceinline__ void tables_rt(int la, int lb, int lc, int ld, int r0
     const double b00 = 0.5 * x * q.ipq;
     const double b10 = (1.0 - fq) * q.i2p;
     const double b01 = (1.0 - fp) * q.i2q;
-    const double c00 = (q.P[ax] - g.A[ax]) - fq * q.PQ[ax];
-    const double cp0 = (q.Q[ax] - g.C[ax]) + fp * q.PQ[ax];
+    // build each pair on the centre of its tighter primitive (uniform over
+    // the workgroup: one primitive quartet at a time)
+    const bool swap_b = q.be > q.al, swap_k = q.de > q.ga;
+    const double c00 =
+        (q.P[ax] - (swap_b ? g.A[ax] - g.AB[ax] : g.A[ax])) - fq * q.PQ[ax];
+    const double cp0 =
+        (q.Q[ax] - (swap_k ? g.C[ax] - g.CD[ax] : g.C[ax])) + fp * q.PQ[ax];
     double *w = work + task * GW;
-    table_rt(la, lb, lc, ld, c00, cp0, b10, b01, b00, g.AB[ax], g.CD[ax], w, w + NW,
+    table_rt(la, lb, lc, ld, swap_b, swap_k, c00, cp0, b10, b01, b00,
+             swap_b ? -g.AB[ax] : g.AB[ax], swap_k ? -g.CD[ax] : g.CD[ax], w, w + NW,
              tab + task * T);
   }
   __syncthreads();

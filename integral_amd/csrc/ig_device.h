@@ -246,10 +246,10 @@ __device__ __forceinline__ void rys_axis_table(F c00, F cp0, F b10, F b01, F b00
 // search at the first index, then rows advance by comparing against the cached
 // end of the current row (implicit sets), or a direct list read (explicit).
 struct RowCursor {
-  int P = 0;
-  long long row_end = -1;  // rowptr[P + 1]
-  long long row_begin = 0;
-  int qb = 0;              // first ket of row P (qs.qbase[P], 0 without it)
+  int seg = 0, P = 0, k = 0;  // segment, its bra row and ket bucket
+  long long seg_end = -1;     // rowptr[seg + 1]
+  long long seg_begin = 0;
+  int qb = 0;                 // first ket of the segment
   __device__ __forceinline__ void next(const QuartetSet &qs, long long t, int &Pout, int &Qout) {
     if (qs.list) {
       int2 v = qs.list[t];
@@ -257,8 +257,8 @@ struct RowCursor {
       Qout = v.y;
       return;
     }
-    if (row_end < 0) {
-      int lo = 0, hi = qs.x_n - 1;
+    if (seg_end < 0) {
+      int lo = 0, hi = qs.x_n * qs.nseg - 1;
       while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (qs.rowptr[mid] <= t)
@@ -266,21 +266,27 @@ struct RowCursor {
         else
           hi = mid - 1;
       }
-      P = lo;
-      row_begin = qs.rowptr[P];
-      row_end = qs.rowptr[P + 1];
-      qb = qs.qbase ? qs.qbase[P] : 0;
+      seg = lo;
+      P = seg / qs.nseg;
+      k = seg - P * qs.nseg;
+      seg_begin = qs.rowptr[seg];
+      seg_end = qs.rowptr[seg + 1];
+      qb = qs.qbase[seg];
     }
-    if (t >= row_end) {
-      while (t >= row_end) {
-        ++P;
-        row_begin = row_end;
-        row_end = qs.rowptr[P + 1];
+    if (t >= seg_end) {
+      while (t >= seg_end) {
+        ++seg;
+        if (++k == qs.nseg) {
+          k = 0;
+          ++P;
+        }
+        seg_begin = seg_end;
+        seg_end = qs.rowptr[seg + 1];
       }
-      qb = qs.qbase ? qs.qbase[P] : 0;
+      qb = qs.qbase[seg];
     }
     Pout = P;
-    Qout = qb + int(t - row_begin);
+    Qout = qb + int(t - seg_begin);
   }
 };
 
@@ -301,7 +307,7 @@ __device__ __forceinline__ void decode_quartet(const QuartetSet &qs, long long t
     Q = v.y;
     return;
   }
-  int lo = 0, hi = qs.x_n - 1;
+  int lo = 0, hi = qs.x_n * qs.nseg - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
     if (qs.rowptr[mid] <= t)
@@ -309,8 +315,8 @@ __device__ __forceinline__ void decode_quartet(const QuartetSet &qs, long long t
     else
       hi = mid - 1;
   }
-  P = lo;
-  Q = int(t - qs.rowptr[lo]) + (qs.qbase ? qs.qbase[lo] : 0);
+  P = lo / qs.nseg;
+  Q = int(t - qs.rowptr[lo]) + qs.qbase[lo];
 }
 
 }  // namespace ig

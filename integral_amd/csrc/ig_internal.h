@@ -84,15 +84,23 @@ struct PairsDev {
 };
 
 // One launch covers the quartets {P, Q} with P in bra class X and Q in ket
-// class Y (class(X) >= class(Y)); either implicit (per-row prefix counts,
-// Q a prefix of Y's Schwarz-sorted list) or an explicit (P, Q) list.
+// class Y (class(X) >= class(Y)); either implicit or an explicit (P, Q) list.
+// Implicit mode: inside a class the pairs are grouped into buckets of equal
+// primitive-pair count (descending), each sorted by Schwarz bound
+// (descending); a bra row's kets in one bucket are a contiguous run of it
+// (a Schwarz prefix, or for mixed precision the part of it above or below the
+// FP32 cut).  Row r, bucket k is segment r * nseg + k: rowptr holds the
+// x_n * nseg + 1 prefix sums of the segments' lengths and qbase the first
+// ket (relative to y_off) of each segment.  Consecutive quartets of a row
+// therefore share their ket primitive count, so the lanes of a wavefront
+// run equal primitive loops.
 struct QuartetSet {
   int x_off, x_n;  // bra pairs [x_off, x_off + x_n) of the sorted pair list
   int y_off, y_n;
   int same;        // X == Y
-  const long long *rowptr;  // x_n + 1 prefix sums (implicit mode) or null
-  const int *qbase;         // implicit mode: first ket of each row (null: 0), so a row
-                            // holds kets [qbase, qbase + count) of Y's sorted list
+  int nseg;        // implicit mode: segments (ket buckets) per bra row
+  const long long *rowptr;  // implicit mode: x_n * nseg + 1 prefix sums, or null
+  const int *qbase;         // implicit mode: first ket of each segment
   const int2 *list;         // explicit (P, Q) pairs relative to x_off / y_off, or null
   long long total;
 };

@@ -35,8 +35,12 @@ struct DeviceError {
 };
 
 void check(hipError_t e, const char *what) {
-  if (e != hipSuccess)
+  if (e != hipSuccess) {
+    // clear the runtime's last-error state, so the failure is reported once
+    // and not again by a later call's launch check
+    (void)hipGetLastError();
     throw DeviceError{IG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+  }
 }
 
 template <class F>
@@ -52,18 +56,27 @@ int guarded(F &&f) {
   }
 }
 
-// device buffer owning hipMalloc'd memory
+// device buffer owning hipMalloc'd memory, or (fixed) a view of a region
+// carved from memory it does not own, which never grows
 struct DBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
   ~DBuf() {
-    if (p) (void)hipFree(p);
+    if (p && !fixed) (void)hipFree(p);
+  }
+  void view(void *q, size_t n) {
+    if (p && !fixed) check(hipFree(p), "hipFree");
+    p = q;
+    bytes = n;
+    fixed = true;
   }
   void reserve(size_t n) {
     if (n <= bytes) return;
+    if (fixed) throw DeviceError{IG_ERR_DEVICE, "a fixed scratch region is too small"};
     if (p) check(hipFree(p), "hipFree");
     p = nullptr;
     bytes = 0;
@@ -272,6 +285,9 @@ int xcd_copies() {
 
 struct ClassRange {
   int off = 0, n = 0, l1 = 0, l2 = 0;
+  // buckets of equal primitive-pair count: bucket k holds the class's pairs
+  // [off + bk[k], off + bk[k + 1])
+  std::vector<int> bk;
 };
 
 struct Plan {
@@ -309,8 +325,10 @@ struct Plan {
     int x, y;
     size_t rp_off;
     long long total;
-    int fp32 = 0;         // 1: the FP32 part (rows start at qbase)
-    long long qb_off = -1;  // offset of its per-row first kets in s_qbase
+    int fp32 = 0;         // 1: the FP32 part (the segments' kets below the cut)
+    long long qb_off = -1;  // offset of its per-segment first kets in s_qbase
+    int nseg = 1;         // segments (ket buckets) per bra row
+    double prims = 0.0;   // primitive quartets before density screening
   };
   std::vector<CP> cached_cps;
   int eff_key = -1;
@@ -328,16 +346,71 @@ struct Plan {
   bool stats_pending = false;
   hipStream_t stats_stream = nullptr;
   int64_t last_stats[2] = {0, 0};
+  // device-side order of the calls that use this plan's scratch: each call
+  // waits on the stream for the previous call's last work (recorded here),
+  // so calls on different streams never overlap on the scratch buffers
+  hipEvent_t idle = nullptr;
+  bool idle_recorded = false;
+  // streams the class launches of one J/K call are spread over (the call's
+  // own stream first), with the events that fork them from it and join them
+  // back, so one launch's tail overlaps the others' work
+  std::vector<hipStream_t> aux;
+  hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> join_ev;
+  hipStream_t cur_stream = nullptr;  // stream of the call in progress
+  // guards the schedule and precision policy (variant, variant_ms,
+  // fp32_below) that workspaces copy at each call
+  std::mutex settings_mu;
+  // scratch carved from a fixed region (the no-allocation entry): one
+  // accumulator copy, no events, every upload an asynchronous copy
+  bool fixed_scratch = false;
   ~Plan() {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (idle) (void)hipEventDestroy(idle);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
+    for (hipStream_t a : aux) {
+      (void)hipStreamSynchronize(a);
+      (void)hipStreamDestroy(a);
+    }
   }
 };
+
+// Order a call on stream s after every earlier call on the plan's scratch.
+void plan_enter(Plan &P, hipStream_t s) {
+  P.cur_stream = s;
+  if (P.idle_recorded) check(hipStreamWaitEvent(s, P.idle, 0), "wait for the previous call");
+}
+
+// Host data into a plan scratch buffer: a synchronous upload, or with fixed
+// scratch an asynchronous copy on the call's stream (the host vector is the
+// plan's own and outlives the copy's staging).
+template <class T>
+void upload_to(Plan &P, DBuf &buf, const std::vector<T> &v) {
+  if (!P.fixed_scratch) {
+    // a synchronous upload: the previous call's kernels may still read buf
+    if (P.idle_recorded) check(hipEventSynchronize(P.idle), "wait for the previous call");
+    buf.upload(v);
+    return;
+  }
+  buf.reserve(v.size() * sizeof(T));
+  if (!v.empty())
+    check(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
+                         P.cur_stream),
+          "copy to scratch");
+}
+
+// Mark the end of a call's device work on stream s.
+void plan_leave(Plan &P, hipStream_t s) {
+  if (!P.idle) check(hipEventCreateWithFlags(&P.idle, hipEventDisableTiming), "hipEventCreate");
+  check(hipEventRecord(P.idle, s), "record call end");
+  P.idle_recorded = true;
+}
 
 void begin_launches(Plan &P, hipStream_t s) {
   P.launches.clear();
   P.s_counters.reserve(2 * Plan::kMaxLaunch * sizeof(unsigned long long));
-  check(hipMemsetAsync(P.s_counters.p, 0, 2 * Plan::kMaxLaunch * sizeof(unsigned long long), s),
-        "zero counters");
+  check(launch_fill(P.s_counters.as<double>(), 2 * Plan::kMaxLaunch, 0.0, s), "zero counters");
   if (P.profiling && P.events.empty()) {
     P.events.resize(2 * Plan::kMaxLaunch);
     for (hipEvent_t &e : P.events) check(hipEventCreate(&e), "hipEventCreate");
@@ -450,6 +523,31 @@ void upload_pairs(Plan &P) {
   P.pdev.pp_n = P.d_ppn.as<int>();
 }
 
+// Orientation of a pair of equal angular momentum: should shell x be the
+// first (the centre the vertical recurrences build on, from which the
+// horizontal transfers move momentum to the second)?  The transfers
+// (a, b+1| = (a+1, b| + AB (a, b| cancel badly when the second shell is the
+// tight one and the product centre P sits next to it, far from A: on l = 6
+// pairs with exponents 9.7 and 0.21, 6.4 bohr apart, the diffuse-first order
+// loses 12-28% of the block against a 40-digit McMurchie-Davidson value, the
+// tight-first order agrees to 4e-14.  So the first shell is the one whose
+// worst primitive pair keeps P closest to it: smaller max over pairs of
+// beta / (alpha + beta), beta the other shell's exponent.  Shells with a zero
+// exponent (the unit-s dummies of density fitting) keep the canonical order.
+bool tighter_first(const HostShell &x, const HostShell &y) {
+  double xmin = INFINITY, xmax = 0.0, ymin = INFINITY, ymax = 0.0;
+  for (double e : x.exps) {
+    xmin = std::min(xmin, e);
+    xmax = std::max(xmax, e);
+  }
+  for (double e : y.exps) {
+    ymin = std::min(ymin, e);
+    ymax = std::max(ymax, e);
+  }
+  if (!(xmin > 0.0 && ymin > 0.0)) return false;
+  return ymax / (xmin + ymax) < xmax / (ymin + xmax);
+}
+
 // Build the plan: pairs in kernel orientation, primitive-pair SoA tables,
 // Schwarz bounds on the device, then the final (class, Q desc) order.
 std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
@@ -470,7 +568,7 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
     pair_of(c, i, j);
     const HostShell *s1 = &b.shells[i], *s2 = &b.shells[j];
     int a1 = i, a2 = j;
-    if (s1->l < s2->l) {
+    if (s1->l < s2->l || (s1->l == s2->l && tighter_first(*s2, *s1))) {
       std::swap(s1, s2);
       std::swap(a1, a2);
     }
@@ -578,12 +676,14 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
   std::vector<double> q(np);
   check(hipMemcpyAsync(q.data(), dq.p, np * sizeof(double), hipMemcpyDeviceToHost, s), "copy Q");
   check(hipStreamSynchronize(s), "schwarz sync");
-  // final order: class, then Q descending (stable)
+  // final order: class, then primitive-pair count descending (the ket
+  // buckets of the quartet sets), then Q descending (stable)
   std::vector<int> order2(np);
   std::iota(order2.begin(), order2.end(), 0);
   std::stable_sort(order2.begin(), order2.end(), [&](int x, int y) {
-    int cx = tmp[order[x]].cls, cy = tmp[order[y]].cls;
-    if (cx != cy) return cx < cy;
+    const Tmp &tx = tmp[order[x]], &ty = tmp[order[y]];
+    if (tx.cls != ty.cls) return tx.cls < ty.cls;
+    if (tx.n != ty.n) return tx.n > ty.n;
     return q[x] > q[y];
   });
   std::vector<int> final_order(np);
@@ -594,6 +694,13 @@ std::unique_ptr<Plan> build_plan(const ig_basis &b, hipStream_t s) {
   }
   fill(final_order);
   P->p_q = qs;
+  for (int c = 0; c < kNumPairClass; ++c) {
+    ClassRange &r = P->cls[c];
+    r.bk.assign(1, 0);
+    for (int k = 1; k < r.n; ++k)
+      if (P->p_ppn[r.off + k] != P->p_ppn[r.off + k - 1]) r.bk.push_back(k);
+    r.bk.push_back(r.n);
+  }
   P->canon_to_pos.assign(np, 0);
   for (int64_t k = 0; k < np; ++k) P->canon_to_pos[P->p_canon[k]] = int(k);
   upload_pairs(*P);
@@ -622,16 +729,12 @@ int count_kets(const double *qy, int ny, double qb, double tau) {
   return lo;
 }
 
-// implicit quartet sets for every (X >= Y) class pair at threshold tau;
-// pairs with Q == 0 contribute nothing (their integrals vanish) and are skipped.
-// implicit quartet sets for every (X >= Y) class pair at threshold tau;
-// pairs with Q == 0 contribute nothing (their integrals vanish) and are
-// skipped.  With cut > tau each class pair has two sets: kets with
-// Q_P Q_Q >= cut (a prefix of the row) in FP64 and the rest of the row, down
-// to tau, as the FP32 set.
-void build_rowptrs(Plan &P, double tau, double cut) {
-  if (!(cut > tau)) cut = tau;
-  if (P.cached_tau == tau && P.cached_cut == cut && !P.cached_cps.empty()) return;
+// Implicit quartet sets for every (X >= Y) class pair at threshold tau, on
+// the host (build_rowptrs uploads them): pairs with Q == 0 contribute nothing
+// (their integrals vanish) and are skipped.  With cut > tau each class pair
+// has two sets: kets with Q_P Q_Q >= cut (a prefix of the row) in FP64 and
+// the rest of the row, down to tau, as the FP32 set.
+void rowptrs_host(Plan &P, double tau, double cut) {
   P.cached_cps.clear();
   P.cached_rowptr_host.clear();
   P.cached_qbase_host.clear();
@@ -642,38 +745,55 @@ void build_rowptrs(Plan &P, double tau, double cut) {
     for (int y = 0; y <= x; ++y) {
       const ClassRange &Y = P.cls[y];
       if (!Y.n) continue;
-      const double *qy = P.p_q.data() + Y.off;
-      lo.assign(X.n, 0);
-      hi.assign(X.n, 0);
+      const int nseg = int(Y.bk.size()) - 1;
+      lo.assign(size_t(X.n) * nseg, 0);
+      hi.assign(size_t(X.n) * nseg, 0);
       for (int r = 0; r < X.n; ++r) {
         const double qb = P.p_q[X.off + r];
         if (qb == 0.0) continue;
-        int cnt = count_kets(qy, Y.n, qb, tau);
-        if (x == y) cnt = std::min(cnt, r + 1);
-        while (cnt > 0 && qy[cnt - 1] == 0.0) --cnt;  // Q == 0 kets
-        hi[r] = cnt;
-        lo[r] = cut > tau ? std::min(cnt, count_kets(qy, Y.n, qb, cut)) : cnt;
+        for (int k = 0; k < nseg; ++k) {
+          const int b0 = Y.bk[k], bn = Y.bk[k + 1] - b0;
+          const double *qy = P.p_q.data() + Y.off + b0;
+          int cnt = count_kets(qy, bn, qb, tau);
+          // one class: each unordered pair once, the ket at or before the
+          // bra in the sorted order
+          if (x == y) cnt = std::min(cnt, std::max(0, std::min(bn, r - b0 + 1)));
+          while (cnt > 0 && qy[cnt - 1] == 0.0) --cnt;  // Q == 0 kets
+          hi[size_t(r) * nseg + k] = cnt;
+          lo[size_t(r) * nseg + k] =
+              cut > tau ? std::min(cnt, count_kets(qy, bn, qb, cut)) : cnt;
+        }
       }
       for (int part = 0; part < (cut > tau ? 2 : 1); ++part) {
         Plan::CP cp{x, y, P.cached_rowptr_host.size(), 0};
+        cp.nseg = nseg;
+        cp.qb_off = (long long)P.cached_qbase_host.size();
         long long acc = 0;
         P.cached_rowptr_host.push_back(0);
-        for (int r = 0; r < X.n; ++r) {
-          acc += part == 0 ? lo[r] : hi[r] - lo[r];
-          P.cached_rowptr_host.push_back(acc);
-        }
+        for (int r = 0; r < X.n; ++r)
+          for (int k = 0; k < nseg; ++k) {
+            const size_t sg = size_t(r) * nseg + k;
+            const int len = part == 0 ? lo[sg] : hi[sg] - lo[sg];
+            acc += len;
+            cp.prims += double(len) * P.p_ppn[X.off + r] * P.p_ppn[Y.off + Y.bk[k]];
+            P.cached_rowptr_host.push_back(acc);
+            P.cached_qbase_host.push_back(Y.bk[k] + (part == 0 ? 0 : lo[sg]));
+          }
         cp.total = acc;
-        if (part == 1) {
-          cp.fp32 = 1;
-          cp.qb_off = (long long)P.cached_qbase_host.size();
-          P.cached_qbase_host.insert(P.cached_qbase_host.end(), lo.begin(), lo.end());
-        }
+        cp.fp32 = part;
         P.cached_cps.push_back(cp);
       }
     }
   }
-  P.s_rowptr.upload(P.cached_rowptr_host);
-  if (!P.cached_qbase_host.empty()) P.s_qbase.upload(P.cached_qbase_host);
+}
+
+void build_rowptrs(Plan &P, double tau, double cut) {
+  if (!(cut > tau)) cut = tau;
+  if (P.cached_tau == tau && P.cached_cut == cut && !P.cached_cps.empty()) return;
+  P.cached_tau = P.cached_cut = NAN;
+  rowptrs_host(P, tau, cut);
+  upload_to(P, P.s_rowptr, P.cached_rowptr_host);
+  upload_to(P, P.s_qbase, P.cached_qbase_host);
   P.cached_tau = tau;
   P.cached_cut = cut;
 }
@@ -687,8 +807,9 @@ QuartetSet implicit_set(Plan &P, const Plan::CP &cp) {
   qs.y_off = P.cls[cp.y].off;
   qs.y_n = P.cls[cp.y].n;
   qs.same = cp.x == cp.y;
+  qs.nseg = cp.nseg;
   qs.rowptr = P.s_rowptr.as<long long>() + cp.rp_off;
-  qs.qbase = cp.qb_off >= 0 ? P.s_qbase.as<int>() + cp.qb_off : nullptr;
+  qs.qbase = P.s_qbase.as<int>() + cp.qb_off;
   qs.list = nullptr;
   qs.total = cp.total;
   return qs;
@@ -745,6 +866,14 @@ ExplicitSets explicit_sets(const Plan &P, int64_t b0, int64_t b1, double tau) {
 struct ig_direct {
   std::unique_ptr<Plan> plan;
   int engine;
+};
+
+// DirectWorkspace (direct.rs:19-33): a caller's own scratch over a
+// contractor's read-only plan data, so calls through different workspaces
+// run concurrently (each on its own stream).
+struct ig_direct_workspace {
+  const ig_direct *owner;
+  std::unique_ptr<Plan> plan;
 };
 
 struct ig_grad {
@@ -833,7 +962,9 @@ DensDev reserve_densities(Plan &P, int ndens, bool has_anti, double tau, bool wa
   P.s_dc.reserve(nc2 * neff * sizeof(double));
   P.s_bound.reserve(size_t(P.nsh) * P.nsh * ndens * sizeof(double));
   // per-XCD copies while each stays under 1 GiB (large bases: one copy)
-  const int ncopy = nc2 * neff * xcd_copies() * sizeof(double) <= (size_t(1) << 30) ? xcd_copies() : 1;
+  const int ncopy = P.fixed_scratch ? 1
+                    : nc2 * neff * xcd_copies() * sizeof(double) <= (size_t(1) << 30) ? xcd_copies()
+                                                                                        : 1;
   P.s_jc.reserve(nc2 * neff * ncopy * sizeof(double));
   if (want_k) P.s_kc.reserve(nc2 * neff * ncopy * sizeof(double));
   if (P.eff_key != neff * 128 + ndens) {  // (source, antisymmetric) per effective density
@@ -842,7 +973,7 @@ DensDev reserve_densities(Plan &P, int ndens, bool has_anti, double tau, bool wa
       eff[e] = e % ndens;
       eff[neff + e] = e >= ndens;
     }
-    P.s_eff.upload(eff);
+    upload_to(P, P.s_eff, eff);
     P.eff_key = neff * 128 + ndens;
   }
   DensDev dd{};
@@ -890,14 +1021,50 @@ void run_jk(Plan &P, const double *d_dens, int ndens, bool has_anti, double tau,
     size_t big = 0;
     for (auto &cp : P.cached_cps) big = std::max(big, big_need_jk(P.cls[cp.x], P.cls[cp.y], cp.total));
     P.s_big.reserve(big * sizeof(double));
-    // heavy classes first so the long launches start early
+    // Class launches spread over kJkStreams streams, each launch to the
+    // stream with the least estimated work so far, heavy classes first (the
+    // long launches start early; the last ones fill the others' tails).
+    // The runtime-l launches share one set of block slots (s_big) and all go
+    // to the call's stream; per-class timing (profiling) and the fixed
+    // scratch of the no-allocation entry keep everything on it.
+    std::vector<hipStream_t> lanes{s};
+    if (!P.fixed_scratch && !P.profiling) {
+      constexpr int kJkStreams = 4;
+      while (int(P.aux.size()) < kJkStreams - 1) {
+        hipStream_t a = nullptr;
+        check(hipStreamCreateWithFlags(&a, hipStreamNonBlocking), "stream");
+        P.aux.push_back(a);
+        hipEvent_t e = nullptr;
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        P.join_ev.push_back(e);
+      }
+      if (!P.fork_ev) check(hipEventCreateWithFlags(&P.fork_ev, hipEventDisableTiming), "hipEventCreate");
+      check(hipEventRecord(P.fork_ev, s), "fork");
+      for (hipStream_t a : P.aux) {
+        check(hipStreamWaitEvent(a, P.fork_ev, 0), "fork wait");
+        lanes.push_back(a);
+      }
+    }
+    std::vector<double> load(lanes.size(), 0.0);
     for (auto it = P.cached_cps.rbegin(); it != P.cached_cps.rend(); ++it) {
       if (!it->total) continue;
       QuartetSet qs = implicit_set(P, *it);
       const ClassRange &X = P.cls[it->x], &Y = P.cls[it->y];
-      unsigned long long *ctr = next_launch(P, X, Y, s, it->fp32);
-      check(jk_launch(P, X, Y, qs, dd, ctr, s, -1, it->fp32 != 0), "jk launch");
-      end_launch(P, s);
+      const double n4 = double(ncart(X.l1) * ncart(X.l2) * ncart(Y.l1) * ncart(Y.l2));
+      const double cost = n4 * (it->prims * ((X.l1 + X.l2 + Y.l1 + Y.l2) / 2 + 1) +
+                                8.0 * double(it->total) * dd.neff);
+      size_t k = 0;
+      if (jk_compiled(X.l1, X.l2, Y.l1, Y.l2))
+        for (size_t q = 1; q < lanes.size(); ++q)
+          if (load[q] < load[k]) k = q;
+      load[k] += cost;
+      unsigned long long *ctr = next_launch(P, X, Y, lanes[k], it->fp32);
+      check(jk_launch(P, X, Y, qs, dd, ctr, lanes[k], -1, it->fp32 != 0), "jk launch");
+      end_launch(P, lanes[k]);
+    }
+    for (size_t q = 1; q < lanes.size(); ++q) {
+      check(hipEventRecord(P.join_ev[q - 1], lanes[q]), "join");
+      check(hipStreamWaitEvent(s, P.join_ev[q - 1], 0), "join wait");
     }
   } else {
     size_t total = 0;
@@ -996,13 +1163,12 @@ extern "C" int ig_direct_density_shell_bounds(const ig_direct *d, const double *
   return IG_OK;
 }
 
-static int jk_host(const ig_direct *d, const double *const *dens, size_t ndens, double tau,
-                   int want_k, double *const *J, double *const *K, const ExplicitSets *ex,
-                   void *stream) {
-  Plan &P = *d->plan;
+static int jk_host(Plan &P, const double *const *dens, size_t ndens, double tau, int want_k,
+                   double *const *J, double *const *K, const ExplicitSets *ex, void *stream) {
   std::lock_guard<std::mutex> g(P.mu);
   const size_t n2 = size_t(P.nao) * P.nao;
   hipStream_t s = as_stream(stream);
+  plan_enter(P, s);
   std::vector<double> host(n2 * ndens);
   bool sym = true;
   for (size_t k = 0; k < ndens; ++k) {
@@ -1022,6 +1188,7 @@ static int jk_host(const ig_direct *d, const double *const *dens, size_t ndens, 
   if (want_k)
     check(hipMemcpyAsync(hk.data(), P.s_K.p, hk.size() * sizeof(double), hipMemcpyDeviceToHost, s),
           "copy K");
+  plan_leave(P, s);
   finalize_stats(P);  // synchronises the stream
   for (size_t k = 0; k < ndens; ++k) {
     for (size_t x = 0; x < n2; ++x) J[k][x] += hj[k * n2 + x];
@@ -1056,7 +1223,7 @@ extern "C" int ig_direct_accumulate_bra_pair(const ig_direct *d, int32_t i, int3
   return guarded([&] {
     int64_t bc = int64_t(i) * (i + 1) / 2 + j;
     ExplicitSets ex = explicit_sets(P, bc, bc + 1, threshold);
-    return jk_host(d, dens, ndens, threshold, want_k, J, K, &ex, stream);
+    return jk_host(*d->plan, dens, ndens, threshold, want_k, J, K, &ex, stream);
   });
 }
 
@@ -1069,9 +1236,10 @@ extern "C" int ig_direct_jk(const ig_direct *d, const double *const *dens, size_
   if (b1 < 0 || b1 > np) b1 = np;
   if (b0 < 0 || b0 > b1) return fail(IG_ERR_INVALID_ARGUMENT, "bad bra range");
   return guarded([&] {
-    if (b0 == 0 && b1 == np) return jk_host(d, dens, ndens, threshold, want_k, J, K, nullptr, stream);
+    if (b0 == 0 && b1 == np)
+      return jk_host(*d->plan, dens, ndens, threshold, want_k, J, K, nullptr, stream);
     ExplicitSets ex = explicit_sets(P, b0, b1, threshold);
-    return jk_host(d, dens, ndens, threshold, want_k, J, K, &ex, stream);
+    return jk_host(*d->plan, dens, ndens, threshold, want_k, J, K, &ex, stream);
   });
 }
 
@@ -1086,6 +1254,7 @@ extern "C" int ig_direct_jk_device(const ig_direct *d, const double *d_dens, siz
     Plan &P = *d->plan;
     std::lock_guard<std::mutex> g(P.mu);
     hipStream_t s = as_stream(stream);
+    plan_enter(P, s);
     // exact symmetry test on the device; only a 4-byte flag comes back
     P.s_flag.reserve(sizeof(int));
     check(launch_asym_flag(d_dens, P.nao, int(ndens), P.s_flag.as<int>(), s), "symmetry check");
@@ -1094,9 +1263,161 @@ extern "C" int ig_direct_jk_device(const ig_direct *d, const double *d_dens, siz
     check(hipStreamSynchronize(s), "sync");
     run_jk(P, d_dens, int(ndens), asym != 0, threshold, want_k != 0, d_J, want_k ? d_K : nullptr,
            nullptr, s);
+    plan_leave(P, s);
     return int(IG_OK);
   });
 }
+
+namespace {
+
+// A caller's workspace plan: the source plan's read-only data (shells,
+// pairs, Schwarz bounds and views of its device buffers, which it does not
+// own) and scratch of its own.
+std::unique_ptr<Plan> workspace_plan(const Plan &src) {
+  auto W = std::make_unique<Plan>();
+  W->basis = src.basis;
+  W->nsh = src.nsh;
+  W->nao = src.nao;
+  W->nao_cart = src.nao_cart;
+  W->natom = src.natom;
+  W->sh_l = src.sh_l;
+  W->sh_kind = src.sh_kind;
+  W->sh_atom = src.sh_atom;
+  W->sh_off_func = src.sh_off_func;
+  W->sh_off_cart = src.sh_off_cart;
+  W->sh_nprim = src.sh_nprim;
+  W->bdev = src.bdev;
+  W->npairs = src.npairs;
+  W->p_sh1 = src.p_sh1;
+  W->p_sh2 = src.p_sh2;
+  W->p_canon = src.p_canon;
+  W->p_ppoff = src.p_ppoff;
+  W->p_ppn = src.p_ppn;
+  W->p_q = src.p_q;
+  W->canon_to_pos = src.canon_to_pos;
+  std::copy(std::begin(src.cls), std::end(src.cls), std::begin(W->cls));
+  W->pdev = src.pdev;
+  W->schwarz = src.schwarz;
+  return W;
+}
+
+// The schedule and precision policy of the source plan, copied into a
+// workspace plan at the start of each of its calls.
+void follow_settings(Plan &W, Plan &src) {
+  std::lock_guard<std::mutex> g(src.settings_mu);
+  W.variant = src.variant;
+  W.variant_ms = src.variant_ms;
+  W.fp32_below = src.fp32_below;
+}
+
+}  // namespace
+
+extern "C" int ig_direct_workspace_create(const ig_direct *d, ig_direct_workspace **out) {
+  if (!d || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    auto w = std::make_unique<ig_direct_workspace>();
+    w->owner = d;
+    w->plan = workspace_plan(*d->plan);
+    *out = w.release();
+    return int(IG_OK);
+  });
+}
+
+extern "C" void ig_direct_workspace_destroy(ig_direct_workspace *w) {
+  if (!w) return;
+  if (w->plan->idle_recorded) (void)hipEventSynchronize(w->plan->idle);
+  delete w;
+}
+
+static int check_workspace(const ig_direct *d, const ig_direct_workspace *w) {
+  if (w && w->owner != d)
+    return fail(IG_ERR_INVALID_ARGUMENT, "workspace was created for another contractor");
+  return IG_OK;
+}
+
+// the plan a call runs on: the workspace's (settings copied from the
+// contractor) or the contractor's own
+static Plan &call_plan(const ig_direct *d, ig_direct_workspace *w) {
+  if (!w) return *d->plan;
+  follow_settings(*w->plan, *d->plan);
+  return *w->plan;
+}
+
+extern "C" int ig_direct_accumulate_bra_pair_ws(const ig_direct *d, ig_direct_workspace *w,
+                                                int32_t i, int32_t j, const double *const *dens,
+                                                const double *bounds, size_t ndens,
+                                                double threshold, int want_k, double *const *J,
+                                                double *const *K, void *stream) {
+  if (int rc = check_dens_args(d, dens, ndens, want_k, J, K)) return rc;
+  if (int rc = check_workspace(d, w)) return rc;
+  const Plan &P = *d->plan;
+  if (!(j >= 0 && j <= i && i < P.nsh))
+    return fail(IG_ERR_NON_CANONICAL_BRA, "bra pair is not canonical (i >= j)");
+  (void)bounds;  // the device recomputes the same bounds from the densities
+  return guarded([&] {
+    int64_t bc = int64_t(i) * (i + 1) / 2 + j;
+    ExplicitSets ex = explicit_sets(P, bc, bc + 1, threshold);
+    return jk_host(call_plan(d, w), dens, ndens, threshold, want_k, J, K, &ex, stream);
+  });
+}
+
+extern "C" int ig_direct_jk_ws(const ig_direct *d, ig_direct_workspace *w,
+                               const double *const *dens, size_t ndens, double threshold,
+                               int want_k, double *const *J, double *const *K, int64_t b0,
+                               int64_t b1, void *stream) {
+  if (int rc = check_dens_args(d, dens, ndens, want_k, J, K)) return rc;
+  if (int rc = check_workspace(d, w)) return rc;
+  const Plan &P = *d->plan;
+  const int64_t np = P.npairs;
+  if (b1 < 0 || b1 > np) b1 = np;
+  if (b0 < 0 || b0 > b1) return fail(IG_ERR_INVALID_ARGUMENT, "bad bra range");
+  return guarded([&] {
+    Plan &W = call_plan(d, w);
+    if (b0 == 0 && b1 == np) return jk_host(W, dens, ndens, threshold, want_k, J, K, nullptr, stream);
+    ExplicitSets ex = explicit_sets(P, b0, b1, threshold);
+    return jk_host(W, dens, ndens, threshold, want_k, J, K, &ex, stream);
+  });
+}
+
+static int check_device_args(const ig_direct *d, const double *d_dens, size_t ndens, int want_k,
+                             const double *d_J, const double *d_K) {
+  if (!d || !d_dens || !d_J || (want_k && !d_K)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (ndens == 0) return fail(IG_ERR_NO_DENSITY, "direct contraction needs at least one density");
+  if (ndens > size_t(kMaxDens))
+    return fail(IG_ERR_TOO_MANY_DENSITIES, "at most 64 simultaneous densities are supported");
+  return IG_OK;
+}
+
+extern "C" int ig_direct_jk_device_ws(const ig_direct *d, ig_direct_workspace *w,
+                                      const double *d_dens, size_t ndens, double threshold,
+                                      int want_k, double *d_J, double *d_K, int dens_symmetric,
+                                      void *stream) {
+  if (int rc = check_device_args(d, d_dens, ndens, want_k, d_J, d_K)) return rc;
+  if (int rc = check_workspace(d, w)) return rc;
+  return guarded([&] {
+    Plan &P = call_plan(d, w);
+    std::lock_guard<std::mutex> g(P.mu);
+    hipStream_t s = as_stream(stream);
+    plan_enter(P, s);
+    run_jk(P, d_dens, int(ndens), dens_symmetric == 0, threshold, want_k != 0, d_J,
+           want_k ? d_K : nullptr, nullptr, s);
+    plan_leave(P, s);
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_direct_workspace_last_stats(ig_direct_workspace *w, int64_t *st) {
+  if (!w || !st) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  return guarded([&] {
+    std::lock_guard<std::mutex> g(w->plan->mu);
+    finalize_stats(*w->plan);
+    st[0] = w->plan->last_stats[0];
+    st[1] = w->plan->last_stats[1];
+    return int(IG_OK);
+  });
+}
+
 
 namespace {
 
@@ -1144,6 +1465,7 @@ void autotune_plan(Plan &P, const double *d_dens, size_t ndens, double threshold
         }
       }
       const int key = cp.x * kNumPairClass + cp.y;
+      std::lock_guard<std::mutex> sg(P.settings_mu);
       P.variant[key] = int8_t(best);
       P.variant_ms[key] = best_ms;
     }
@@ -1171,7 +1493,9 @@ extern "C" int ig_direct_autotune(ig_direct *d, const double *d_dens, size_t nde
     Plan &P = *d->plan;
     std::lock_guard<std::mutex> g(P.mu);
     hipStream_t s = as_stream(stream);
+    plan_enter(P, s);
     autotune_plan(P, d_dens, ndens, threshold, want_k, reps, s);
+    plan_leave(P, s);
     return int(IG_OK);
   });
 }
@@ -1223,6 +1547,7 @@ int check_schedule(const ig_schedule_entry *in, size_t n) {
 }
 
 void apply_schedule(Plan &P, const ig_schedule_entry *in, size_t n) {
+  std::lock_guard<std::mutex> sg(P.settings_mu);
   for (size_t k = 0; k < n; ++k) {
     const int key = pair_class(in[k].la, in[k].lb) * kNumPairClass + pair_class(in[k].lc, in[k].ld);
     P.variant[key] = int8_t(in[k].variant);
@@ -1248,6 +1573,7 @@ extern "C" int ig_direct_set_mixed_precision(ig_direct *d, double fp32_below) {
   if (!(std::isfinite(fp32_below) && fp32_below >= 0.0))
     return fail(IG_ERR_INVALID_ARGUMENT, "fp32_below must be finite and >= 0");
   std::lock_guard<std::mutex> g(d->plan->mu);
+  std::lock_guard<std::mutex> sg(d->plan->settings_mu);
   d->plan->fp32_below = fp32_below;
   return IG_OK;
 }
@@ -1978,10 +2304,20 @@ extern "C" int ig_grad_density_create(const ig_grad *g, size_t nc, const double 
 
 extern "C" void ig_grad_density_destroy(ig_grad_density *d) { delete d; }
 
-static int grad_run(const ig_grad *g, const ig_grad_density *dens, int kernel, double omega,
-                    double tau, double *out, size_t len, int64_t b0, int64_t b1, void *stream) {
+// DirectGradWorkspace (direct_grad.rs:51-67): a caller's own scratch over a
+// gradient contractor's read-only plan data
+struct ig_grad_workspace {
+  const ig_grad *owner;
+  std::unique_ptr<Plan> plan;
+};
+
+static int grad_run(const ig_grad *g, ig_grad_workspace *w, const ig_grad_density *dens,
+                    int kernel, double omega, double tau, double *out, size_t len, int64_t b0,
+                    int64_t b1, void *stream) {
   if (!g || !dens || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
-  Plan &P = *g->plan;
+  if (w && w->owner != g)
+    return fail(IG_ERR_INVALID_ARGUMENT, "workspace was created for another contractor");
+  Plan &P = w ? *w->plan : *g->plan;
   if (dens->owner != g)
     return fail(IG_ERR_FOREIGN_DENSITY,
                 "density was prepared by a plan with a different shell layout");
@@ -1993,21 +2329,19 @@ static int grad_run(const ig_grad *g, const ig_grad_density *dens, int kernel, d
   return guarded([&] {
     std::lock_guard<std::mutex> lk(P.mu);
     hipStream_t s = as_stream(stream);
-    DBuf dout;
-    dout.reserve(len * sizeof(double));
-    check(launch_fill(dout.as<double>(), len, 0.0, s), "zero grad");
-    P.s_counters.reserve(2 * sizeof(unsigned long long));
-    check(hipMemsetAsync(P.s_counters.p, 0, 2 * sizeof(unsigned long long), s), "zero counters");
+    plan_enter(P, s);
+    P.s_J.reserve(len * sizeof(double));  // the device gradient accumulator
+    check(launch_fill(P.s_J.as<double>(), len, 0.0, s), "zero grad");
+    begin_launches(P, s);  // per-class counters (and events when profiling)
     GradDev gd{};
     gd.ncoul = dens->ncoul;
     gd.nexch = dens->nexch;
     gd.fac = dens->d_fac.as<double>();
     gd.mats = dens->d_mats.as<double>();
     gd.bmax = dens->d_bmax.as<double>();
-    gd.out = dout.as<double>();
+    gd.out = P.s_J.as<double>();
     gd.tau = tau;
     gd.omega = kernel == IG_KERNEL_ERF ? omega : 0.0;
-    auto *ctr = P.s_counters.as<unsigned long long>();
     const int64_t np = P.npairs;
     if (gd.ncoul + gd.nexch > 0) {
       if (b0 == 0 && b1 == np) {
@@ -2020,7 +2354,9 @@ static int grad_run(const ig_grad *g, const ig_grad_density *dens, int kernel, d
           if (!it->total) continue;
           QuartetSet qs = implicit_set(P, *it);
           const ClassRange &X = P.cls[it->x], &Y = P.cls[it->y];
+          unsigned long long *ctr = next_launch(P, X, Y, s);
           check(grad_launch(P, X, Y, qs, gd, ctr, s), "grad launch");
+          end_launch(P, s);
         }
       } else {
         ExplicitSets ex = explicit_sets(P, b0, b1, tau);
@@ -2044,18 +2380,19 @@ static int grad_run(const ig_grad *g, const ig_grad_density *dens, int kernel, d
           qs.list = P.s_list.as<int2>() + off;
           qs.total = (long long)ex.lists[c].size();
           off += ex.lists[c].size();
+          unsigned long long *ctr = next_launch(P, X, Y, s);
           check(grad_launch(P, X, Y, qs, gd, ctr, s), "grad launch");
+          end_launch(P, s);
         }
       }
     }
     std::vector<double> h(len);
-    unsigned long long c2[2];
-    check(hipMemcpyAsync(h.data(), dout.p, len * sizeof(double), hipMemcpyDeviceToHost, s),
+    check(hipMemcpyAsync(h.data(), P.s_J.p, len * sizeof(double), hipMemcpyDeviceToHost, s),
           "copy grad");
-    check(hipMemcpyAsync(c2, P.s_counters.p, sizeof(c2), hipMemcpyDeviceToHost, s), "copy ctr");
-    check(hipStreamSynchronize(s), "grad sync");
-    P.last_stats[0] = int64_t(c2[0]);
-    P.last_stats[1] = int64_t(c2[1]);
+    plan_leave(P, s);
+    P.stats_stream = s;
+    P.stats_pending = true;
+    finalize_stats(P);  // per-class counters and times; synchronises the stream
     for (size_t x = 0; x < len; ++x) out[x] += h[x];
     return int(IG_OK);
   });
@@ -2069,21 +2406,88 @@ extern "C" int ig_grad_accumulate_bra_pair(const ig_grad *g, const ig_grad_densi
   if (!(j >= 0 && j <= i && i < g->plan->nsh))
     return fail(IG_ERR_NON_CANONICAL_BRA, "bra pair is not canonical (i >= j)");
   int64_t bc = int64_t(i) * (i + 1) / 2 + j;
-  return grad_run(g, dens, kernel, omega, threshold, out, len, bc, bc + 1, stream);
+  return grad_run(g, nullptr, dens, kernel, omega, threshold, out, len, bc, bc + 1, stream);
 }
 
 extern "C" int ig_grad_contract(const ig_grad *g, const ig_grad_density *dens, int kernel,
                                 double omega, double threshold, double *out, size_t len,
                                 void *stream) {
   if (!g) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
-  return grad_run(g, dens, kernel, omega, threshold, out, len, 0, g->plan->npairs, stream);
+  return grad_run(g, nullptr, dens, kernel, omega, threshold, out, len, 0, g->plan->npairs,
+                  stream);
 }
 
 extern "C" int ig_grad_last_stats(const ig_grad *g, int64_t *st) {
   if (!g || !st) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(g->plan->mu);
   st[0] = g->plan->last_stats[0];
   st[1] = g->plan->last_stats[1];
   return IG_OK;
+}
+
+extern "C" int ig_grad_set_profiling(ig_grad *g, int enable) {
+  if (!g) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(g->plan->mu);
+  g->plan->profiling = enable != 0;
+  return IG_OK;
+}
+
+extern "C" int ig_grad_profile_read(const ig_grad *g, ig_class_profile *out, size_t cap,
+                                    size_t *n) {
+  if (!g || !n || (cap && !out)) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  return guarded([&] {
+    Plan &P = *g->plan;
+    std::lock_guard<std::mutex> lk(P.mu);
+    finalize_stats(P);
+    *n = P.launches.size();
+    for (size_t k = 0; k < P.launches.size() && k < cap; ++k) {
+      out[k].fp32 = 0;
+      out[k].la = P.launches[k].la;
+      out[k].lb = P.launches[k].lb;
+      out[k].lc = P.launches[k].lc;
+      out[k].ld = P.launches[k].ld;
+      out[k].quartets = int64_t(P.launch_ctr[2 * k]);
+      out[k].prim_quartets = int64_t(P.launch_ctr[2 * k + 1]);
+      out[k].ms = P.profiling && k < P.launch_ms.size() ? double(P.launch_ms[k]) : 0.0;
+    }
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_grad_workspace_create(const ig_grad *g, ig_grad_workspace **out) {
+  if (!g || !out) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  return guarded([&] {
+    auto w = std::make_unique<ig_grad_workspace>();
+    w->owner = g;
+    w->plan = workspace_plan(*g->plan);
+    *out = w.release();
+    return int(IG_OK);
+  });
+}
+
+extern "C" void ig_grad_workspace_destroy(ig_grad_workspace *w) {
+  if (!w) return;
+  if (w->plan->idle_recorded) (void)hipEventSynchronize(w->plan->idle);
+  delete w;
+}
+
+extern "C" int ig_grad_accumulate_bra_pair_ws(const ig_grad *g, ig_grad_workspace *w,
+                                              const ig_grad_density *dens, int32_t i, int32_t j,
+                                              int kernel, double omega, double threshold,
+                                              double *out, size_t len, void *stream) {
+  if (!g) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (!(j >= 0 && j <= i && i < g->plan->nsh))
+    return fail(IG_ERR_NON_CANONICAL_BRA, "bra pair is not canonical (i >= j)");
+  int64_t bc = int64_t(i) * (i + 1) / 2 + j;
+  return grad_run(g, w, dens, kernel, omega, threshold, out, len, bc, bc + 1, stream);
+}
+
+extern "C" int ig_grad_contract_ws(const ig_grad *g, ig_grad_workspace *w,
+                                   const ig_grad_density *dens, int kernel, double omega,
+                                   double threshold, double *out, size_t len, void *stream) {
+  if (!g) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  return grad_run(g, w, dens, kernel, omega, threshold, out, len, 0, g->plan->npairs, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -2388,13 +2792,21 @@ extern "C" int ig_batch_jk_device(ig_batch *b, const double *const *d_dens, doub
     if (!b->exec || b->tau != threshold || b->cut != cut || b->want_k != (want_k != 0) ||
         b->dens != dens || b->J != J || b->K != K) {
       check(hipStreamSynchronize(b->stream), "sync");
+      // the capture rewrites the device views and work lists an older graph
+      // reads: drop that graph first, so a capture that fails leaves nothing
+      // a later call could replay
+      if (b->exec) (void)hipGraphExecDestroy(b->exec);
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->exec = nullptr;
+      b->graph = nullptr;
+      b->tau = NAN;  // set again only once the capture has succeeded
       b->dens = dens;
       b->J = J;
       b->K = K;
-      b->tau = threshold;
       b->cut = cut;
       b->want_k = want_k != 0;
       batch_capture(*b, threshold, cut, want_k != 0);
+      b->tau = threshold;
     }
     hipStream_t s = as_stream(stream);
     check(hipEventRecord(b->ev_in, s), "event");

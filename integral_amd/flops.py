@@ -99,3 +99,28 @@ def quartet_flops(ls, K, spherical, ndens=1, c2s_in_loop=False):
         nfunc *= (2 * l + 1) if s else ncart(l)
     f_c2s = c2s_flops(ls, spherical) if c2s_in_loop else 0
     return min(f_rys, f_hgp) + f_c2s + 32 * nfunc * ndens
+
+
+def grad_quartet_flops(ls, K, nterms=2):
+    """Algorithmic flops of one canonical quartet's contribution to the
+    density-contracted ERI gradient (direct_grad.rs:394-620): the shifted
+    blocks of the first three centres -- raised (l+1) and, for l > 0, lowered
+    (l-1) -- each at min(F_Rys, F_HGP) with the same K primitive quartets
+    (the fourth centre follows from translational invariance); 2 flops per
+    weight x shifted-element product on each of the 3 axes; the weight block,
+    about 6 flops per element and density term (the 4-term exchange
+    symmetrisation, direct_grad.rs:490-547)."""
+    la, lb, lc, ld = ls
+    total = 0.0
+    for pos in range(3):
+        for dl in (1, -1):
+            sh = list(ls)
+            sh[pos] += dl
+            if sh[pos] < 0:
+                continue
+            f_rys = K * rys_prim(*sh)
+            f_hgp = K * hgp_prim(*sh) + hgp_hrr(*sh)
+            n4 = ncart(sh[0]) * ncart(sh[1]) * ncart(sh[2]) * ncart(sh[3])
+            total += min(f_rys, f_hgp) + 2 * 3 * n4
+    n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld)
+    return total + 6 * n4 * nterms

@@ -144,3 +144,61 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+class _FakeTensor:
+    """Just the metadata the device-argument checks read (no GPU needed)."""
+
+    def __init__(self, shape, dtype="torch.float64", cuda=True, contiguous=True, device="cuda:0"):
+        self.shape, self.dtype, self.is_cuda, self.device = tuple(shape), dtype, cuda, device
+        self._contiguous = contiguous
+
+    def is_contiguous(self):
+        return self._contiguous
+
+
+@pytest.mark.parametrize("dens,J,K,ok", [
+    ((2, 5, 5), (2, 5, 5), (2, 5, 5), True),
+    ((2, 5, 5), (2, 5, 5), None, True),
+    ((5, 5), (5, 5), (5, 5), False),             # a 2-D density would read as 5 densities
+    ((2, 25), (2, 25), (2, 25), False),
+    ((2, 5, 5), (1, 5, 5), (2, 5, 5), False),    # J smaller than the densities
+    ((2, 5, 5), (2, 5, 5), (2, 4, 5), False),
+    ((0, 5, 5), (0, 5, 5), None, False),         # direct.rs:141-144: at least one density
+    ((65, 5, 5), (65, 5, 5), None, False),       # direct.rs:145-148: at most 64
+])
+def test_device_jk_shapes_are_checked_before_any_library_call(dens, J, K, ok):
+    args = [_FakeTensor(dens), _FakeTensor(J), _FakeTensor(K) if K else None]
+    if ok:
+        ig._check_device_jk(5, *args)
+    else:
+        with pytest.raises(ig.UsageError):
+            ig._check_device_jk(5, *args)
+
+
+def test_device_jk_dtype_device_and_layout_are_checked():
+    good = (2, 5, 5)
+    for bad in (_FakeTensor(good, dtype="torch.int64"), _FakeTensor(good, cuda=False),
+                _FakeTensor(good, contiguous=False), _FakeTensor(good, device="cuda:1")):
+        with pytest.raises(ig.UsageError):
+            ig._check_device_jk(5, _FakeTensor(good), bad, None)
+    with pytest.raises(ig.UsageError):
+        ig._check_device_jk(5, _FakeTensor(good, dtype="torch.float32"), _FakeTensor(good))
+
+
+def test_density_count_limits_through_the_c_abi_before_any_device_work():
+    """direct.rs:141-148 through the C entry points: the density count is
+    checked before the contractor is touched (a dummy handle is never
+    dereferenced), with the reference's two panic categories as codes."""
+    lib = ig.library()
+    fake = ctypes.c_void_p(256)
+    for fn in (lib.ig_direct_jk_device, ):
+        assert fn(fake, fake, 0, 0.0, 1, fake, fake, None) == ig.E_NODENS
+        assert fn(fake, fake, 65, 0.0, 1, fake, fake, None) == ig.E_TOOMANY
+    assert lib.ig_direct_jk_device_ws(fake, None, fake, 0, 0.0, 1, fake, fake, 1, None) == ig.E_NODENS
+    assert lib.ig_direct_jk_device_ws(fake, None, fake, 65, 0.0, 1, fake, fake, 1, None) == ig.E_TOOMANY
+    # want_exchange without a K buffer, and a null contractor
+    assert lib.ig_direct_jk_device_ws(fake, None, fake, 1, 0.0, 1, fake, None, 1, None) == ig.E_ARG
+    assert lib.ig_direct_jk_device_ws(None, None, fake, 1, 0.0, 1, fake, fake, 1, None) == ig.E_ARG
+    assert lib.ig_direct_workspace_create(None, ctypes.byref(ctypes.c_void_p())) == ig.E_ARG
+    assert lib.ig_grad_workspace_create(None, ctypes.byref(ctypes.c_void_p())) == ig.E_ARG
