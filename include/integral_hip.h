@@ -279,6 +279,24 @@ int ig_direct_jk_device_ws(const ig_direct *direct, ig_direct_workspace *workspa
  * that call's stream). */
 int ig_direct_workspace_last_stats(ig_direct_workspace *workspace, int64_t *stats2);
 
+/* J/K that allocates no device memory.  No counterpart in the reference: for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  As
+ * ig_direct_jk_device_ws (the same kernels and results), with all call
+ * scratch -- the split and Cartesian densities, block bounds, J'/K'
+ * accumulators (one copy), quartet-set rows, counters and runtime-l block
+ * slots -- carved from one fixed device array of 64 MiB per GPU (a
+ * file-scope __device__ array, held by each GPU the library runs on until
+ * the process ends).  A call needing more fails with IG_ERR_UNSUPPORTED and
+ * the size in the message (a nao 520 spherical s,s,p,p,d basis with two
+ * symmetric densities needs about 25 MiB).  Every call on one GPU shares the
+ * array and its host uploads go on `stream`: calls that overlap on one GPU
+ * (other streams, other threads, other contractors) race.  The contractor's
+ * plan must exist (ig_direct_create allocates); no statistics are recorded. */
+int ig_direct_jk_device_noalloc(const ig_direct *direct, const double *d_dens, size_t ndens,
+                                double threshold, int want_exchange, double *d_J, double *d_K,
+                                int dens_symmetric, void *stream);
+
 /* Per-class attribution of the last J/K call (the analogue of the
  * reference's `eri_profile` classes mode, benches/examples/eri_profile.rs:
  * 396-531).  With profiling enabled each quartet-class launch is bracketed by

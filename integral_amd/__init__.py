@@ -197,6 +197,8 @@ def library():
     lib.ig_direct_jk_device_ws.argtypes = [_vp, _vp, _vp, sz, ctypes.c_double, ctypes.c_int, _vp,
                                            _vp, ctypes.c_int, _vp]
     lib.ig_direct_workspace_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
+    lib.ig_direct_jk_device_noalloc.argtypes = [_vp, _vp, sz, ctypes.c_double, ctypes.c_int, _vp,
+                                                _vp, ctypes.c_int, _vp]
     lib.ig_direct_set_profiling.argtypes = [_vp, ctypes.c_int]
     lib.ig_direct_profile_read.argtypes = [_vp, ctypes.POINTER(_ClassProfile), sz,
                                            ctypes.POINTER(sz)]
@@ -902,6 +904,17 @@ class DirectContractor:
             self._h, ws, _vp(dens.data_ptr()), dens.shape[0], float(threshold),
             1 if K is not None else 0, _vp(J.data_ptr()),
             _vp(K.data_ptr()) if K is not None else None, 0 if symmetric is False else 1,
+            _stream_ptr(stream)))
+
+    def jk_device_noalloc(self, dens, J, K=None, threshold=0.0, symmetric=True, stream=None):
+        """jk_device with no device allocation: the call's scratch comes from
+        the library's fixed 64 MiB device array (shared by every call on the
+        GPU: overlapping calls race).  No statistics are recorded."""
+        _check_device_jk(self.nao(), dens, J, K)
+        _raise(library().ig_direct_jk_device_noalloc(
+            self._h, _vp(dens.data_ptr()), dens.shape[0], float(threshold),
+            1 if K is not None else 0, _vp(J.data_ptr()),
+            _vp(K.data_ptr()) if K is not None else None, 1 if symmetric else 0,
             _stream_ptr(stream)))
 
     def last_stats(self):

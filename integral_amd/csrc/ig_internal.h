@@ -302,6 +302,10 @@ hipError_t launch_boys_eval(const RysDev &r, int mmax, const double *T, size_t n
                             hipStream_t s);
 hipError_t launch_rys_eval(const RysDev &r, int nr, const double *T, size_t n, double *x,
                            double *w, hipStream_t s);
+// the fixed device array of the no-allocation entry points on the current
+// device (its address from hipGetSymbolAddress, taken once per device) and
+// its size in bytes
+hipError_t noalloc_scratch(void **ptr, size_t *bytes);
 // histogram (16 bins) of the XCC_ID seen by `groups` workgroups
 hipError_t launch_xcc_probe(unsigned *hist, int groups, hipStream_t s);
 

@@ -866,6 +866,10 @@ ExplicitSets explicit_sets(const Plan &P, int64_t b0, int64_t b1, double tau) {
 struct ig_direct {
   std::unique_ptr<Plan> plan;
   int engine;
+  // the no-allocation entry's plan: scratch carved from the fixed device
+  // array (built on first use; host memory only)
+  std::unique_ptr<Plan> fixed;
+  std::mutex fixed_mu;
 };
 
 // DirectWorkspace (direct.rs:19-33): a caller's own scratch over a
@@ -1418,6 +1422,76 @@ extern "C" int ig_direct_workspace_last_stats(ig_direct_workspace *w, int64_t *s
   });
 }
 
+// ---------------------------------------------------------------------------
+// J/K with no device allocation: the scratch of the call (densities, bounds,
+// J'/K' accumulators, quartet-set rows, counters, runtime-l block slots) is
+// carved from the fixed device array of ig_noalloc.hip.
+extern "C" int ig_direct_jk_device_noalloc(const ig_direct *d, const double *d_dens,
+                                           size_t ndens, double threshold, int want_k,
+                                           double *d_J, double *d_K, int dens_symmetric,
+                                           void *stream) {
+  if (int rc = check_device_args(d, d_dens, ndens, want_k, d_J, d_K)) return rc;
+  return guarded([&] {
+    void *base = nullptr;
+    size_t cap = 0;
+    check(noalloc_scratch(&base, &cap), "scratch address");
+    ig_direct *dm = const_cast<ig_direct *>(d);
+    std::lock_guard<std::mutex> fg(dm->fixed_mu);
+    if (!dm->fixed) {
+      dm->fixed = workspace_plan(*d->plan);
+      dm->fixed->fixed_scratch = true;
+    }
+    Plan &P = *dm->fixed;
+    follow_settings(P, *d->plan);
+    hipStream_t s = as_stream(stream);
+    P.cur_stream = s;
+    const bool has_anti = dens_symmetric == 0;
+    const int nd = int(ndens), neff = nd * (has_anti ? 2 : 1);
+    const size_t n2 = size_t(P.nao) * P.nao, nc2 = size_t(P.nao_cart) * P.nao_cart;
+    // other plans' calls reuse the array: every upload is redone
+    P.eff_key = -1;
+    const double cut = P.fp32_below > threshold ? P.fp32_below : threshold;
+    rowptrs_host(P, threshold, cut);
+    size_t big = 0;
+    for (auto &cp : P.cached_cps) big = std::max(big, big_need_jk(P.cls[cp.x], P.cls[cp.y], cp.total));
+    struct Part {
+      DBuf *buf;
+      size_t bytes;
+    };
+    const Part parts[] = {
+        {&P.s_ds, n2 * nd * sizeof(double)},
+        {&P.s_da, (has_anti ? n2 * nd * sizeof(double) : 0) + 8},
+        {&P.s_dc, nc2 * neff * sizeof(double)},
+        {&P.s_bound, size_t(P.nsh) * P.nsh * nd * sizeof(double)},
+        {&P.s_jc, nc2 * neff * sizeof(double)},
+        {&P.s_kc, want_k ? nc2 * neff * sizeof(double) : 8},
+        {&P.s_eff, 2 * size_t(neff) * sizeof(int)},
+        {&P.s_rowptr, P.cached_rowptr_host.size() * sizeof(long long)},
+        {&P.s_qbase, std::max<size_t>(1, P.cached_qbase_host.size()) * sizeof(int)},
+        {&P.s_counters, 2 * Plan::kMaxLaunch * sizeof(unsigned long long)},
+        {&P.s_big, big * sizeof(double)},
+    };
+    size_t need = 0;
+    for (const Part &q : parts) need += (q.bytes + 255) / 256 * 256;
+    if (need > cap)
+      return fail(IG_ERR_UNSUPPORTED,
+                  "this call needs " + std::to_string(need) +
+                      " bytes of scratch; the no-allocation entry holds " + std::to_string(cap));
+    size_t off = 0;
+    for (const Part &q : parts) {
+      q.buf->view(static_cast<char *>(base) + off, (q.bytes + 255) / 256 * 256);
+      off += (q.bytes + 255) / 256 * 256;
+    }
+    upload_to(P, P.s_rowptr, P.cached_rowptr_host);
+    upload_to(P, P.s_qbase, P.cached_qbase_host);
+    P.cached_tau = threshold;
+    P.cached_cut = cut;
+    run_jk(P, d_dens, nd, has_anti, threshold, want_k != 0, d_J, want_k ? d_K : nullptr,
+           nullptr, s);
+    P.stats_pending = false;  // no counters are read back on this path
+    return int(IG_OK);
+  });
+}
 
 namespace {
 

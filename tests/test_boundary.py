@@ -197,6 +197,9 @@ def test_density_count_limits_through_the_c_abi_before_any_device_work():
         assert fn(fake, fake, 65, 0.0, 1, fake, fake, None) == ig.E_TOOMANY
     assert lib.ig_direct_jk_device_ws(fake, None, fake, 0, 0.0, 1, fake, fake, 1, None) == ig.E_NODENS
     assert lib.ig_direct_jk_device_ws(fake, None, fake, 65, 0.0, 1, fake, fake, 1, None) == ig.E_TOOMANY
+    assert lib.ig_direct_jk_device_noalloc(fake, fake, 0, 0.0, 1, fake, fake, 1, None) == ig.E_NODENS
+    assert lib.ig_direct_jk_device_noalloc(fake, fake, 65, 0.0, 1, fake, fake, 1,
+                                           None) == ig.E_TOOMANY
     # want_exchange without a K buffer, and a null contractor
     assert lib.ig_direct_jk_device_ws(fake, None, fake, 1, 0.0, 1, fake, None, 1, None) == ig.E_ARG
     assert lib.ig_direct_jk_device_ws(None, None, fake, 1, 0.0, 1, fake, fake, 1, None) == ig.E_ARG
