@@ -268,8 +268,12 @@ __device__ __forceinline__ void next_quartet(const QuartetSet &qs, long long t, 
 }
 
 // ---------------------------------------------------------------------------
-// Direct J/K: same screening and digestion as jk_reg_kernel (direct.rs:131-218
-// contract, Cartesian digestion of the canonical quartet with weight f).
+// Direct J/K for the runtime-l classes with blocks under kRtMinElems elements:
+// one workgroup per quartet, the block in its global slot (big_block), then
+// the digest read back from the slot; the screening and digestion of
+// jk_reg_kernel (direct.rs:131-218 contract, Cartesian digestion of the
+// canonical quartet with weight f).  Few elements per quartet leave the
+// sliced kernel below with little work between its barriers.
 __global__ void __launch_bounds__(kBigThreads)
     jk_big_kernel(int la, int lb, int lc, int ld, int rb, BasisDev b, PairsDev p, RysDev R,
                   QuartetSet qs, DensDev dd, double omega, const unsigned long long *eoff,
@@ -381,6 +385,379 @@ __global__ void __launch_bounds__(kBigThreads)
       }
     }
     __syncthreads();  // the slot is overwritten by the next quartet
+  }
+  if (counters && threadIdx.x == 0) {
+    atomicAdd(counters, my_q);
+    atomicAdd(counters + 1, my_p);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Direct J/K for the runtime-l classes: the screening and digestion of
+// jk_reg_kernel (direct.rs:131-218 contract, Cartesian digestion of the
+// canonical quartet with weight f).
+//
+// Work item = (quartet, slice of kRtThreads * EPL block elements in (a, b,
+// c, d) order).  Lane l owns elements s0 + l + kRtThreads * k (k < EPL) and
+// accumulates them in registers over primitives and roots, so the block never
+// goes to memory.  Per primitive quartet and batch of roots the workgroup
+// builds the 2D tables in LDS in three parallel steps:
+//   vertical recurrence g(n, m): one lane per (root, axis), columns in VGPRs;
+//   bra transfer h(u, v; m): one lane per (root, axis, u, m), every v;
+//   ket transfer I(u, v; uk, vk): one lane per (root, axis, u, v, uk), every vk;
+// each pair built on the centre of its tighter primitive and transferred to
+// the other (as table_rt).  The Rys weight and the prefactor ride on the z
+// table.  The digest copies the slice to LDS and sums every J'/K' entry the
+// slice touches against the quartet's six density sub-blocks, staged in LDS,
+// with one device atomic per entry and slice.
+constexpr int kRtThreads = 256;
+constexpr int kRtN = 2 * kMaxL + 1;  // vertical-recurrence extent (n, m <= 12)
+constexpr int kRtX = kMaxL + 1;      // transfer extent (received momentum <= 6)
+constexpr size_t kRtLdsBudget = 64 * 1024;  // dynamic LDS of a root batch
+constexpr int kRtMaxWG = 4096;
+// Blocks of at least this many elements take jk_rt_kernel, smaller ones
+// jk_big_kernel (measured on a 20-atom s..g + l = 6 basis: the sliced kernel
+// is 1.2-1.8x faster from 16384 elements up, 1.1-2.3x slower below).
+constexpr int kRtMinElems = 16384;
+
+// Doubles per (root, axis) of the table steps: vertical-recurrence table,
+// bra-transferred table, final table.
+struct RtLayout {
+  int G, H, T;
+  __host__ __device__ RtLayout(int la, int lb, int lc, int ld)
+      : G((la + lb + 1) * (lc + ld + 1)), H((la + 1) * (lb + 1) * (lc + ld + 1)),
+        T((la + 1) * (lb + 1) * (lc + 1) * (ld + 1)) {}
+  __host__ __device__ int per_root() const { return 3 * (G + H + T); }
+};
+
+// Vertical recurrence of one (root, axis) into gout[n * (NK + 1) + m],
+// n <= NB, m <= NK (the recurrences of table_rt, column by column).
+__device__ __forceinline__ void vrr_rt(int NB, int NK, double c00, double cp0, double b10,
+                                       double b01, double b00, double *gout) {
+  const int W = NK + 1;
+  double prev[kRtN], cur[kRtN];
+#pragma unroll
+  for (int n = 0; n < kRtN; ++n) {
+    prev[n] = 0.0;
+    cur[n] = 0.0;
+  }
+  cur[0] = 1.0;
+  if (NB >= 1) cur[1] = c00;
+#pragma unroll
+  for (int n = 1; n + 1 < kRtN; ++n)
+    if (n < NB) cur[n + 1] = c00 * cur[n] + double(n) * b10 * cur[n - 1];
+#pragma unroll
+  for (int n = 0; n < kRtN; ++n)
+    if (n <= NB) gout[n * W] = cur[n];
+  for (int m = 0; m < NK; ++m) {
+    double nxt[kRtN];
+#pragma unroll
+    for (int n = 0; n < kRtN; ++n) {
+      double v = cp0 * cur[n] + double(m) * b01 * prev[n];
+      if (n >= 1) v += double(n) * b00 * cur[n - 1];
+      nxt[n] = v;
+    }
+#pragma unroll
+    for (int n = 0; n < kRtN; ++n) {
+      if (n <= NB) gout[n * W + m + 1] = nxt[n];
+      prev[n] = cur[n];
+      cur[n] = nxt[n];
+    }
+  }
+}
+
+template <int EPL>
+__global__ void __launch_bounds__(kRtThreads)
+    jk_rt_kernel(int la, int lb, int lc, int ld, int rb, int nsl, BasisDev b,
+                 PairsDev p, RysDev R, QuartetSet qs, DensDev dd, double omega,
+                 const unsigned long long *__restrict__ eoff, unsigned long long *counters) {
+  __shared__ BigStatic S;
+  extern __shared__ double dyn[];
+  const BigClass K(la, lb, lc, ld, 0);
+  const RtLayout L(la, lb, lc, ld);
+  const int NB = la + lb, NK = lc + ld, W = NK + 1;
+  const int nsh = b.nsh, ncrt = b.nao_cart;
+  const size_t n2 = (size_t)ncrt * ncrt;
+  // a contiguous range of work items per workgroup: the quartet index only
+  // grows, so the row cursor decodes it without a search per item
+  const long long work = qs.total * nsl;
+  const long long per_wg = (work + gridDim.x - 1) / gridDim.x;
+  const long long w_begin = (long long)blockIdx.x * per_wg;
+  const long long w_end = w_begin + per_wg < work ? w_begin + per_wg : work;
+  constexpr unsigned long long m21 = (1ull << 21) - 1;
+  unsigned long long my_q = 0, my_p = 0;
+  RowCursor cur;
+  for (long long w = w_begin; w < w_end; ++w) {
+    const long long t = w / nsl;
+    const int sl = int(w - t * nsl);
+    int pb, pk;
+    {
+      int P, Q;
+      cur.next(qs, t, P, Q);
+      pb = qs.x_off + P;
+      pk = qs.y_off + Q;
+    }
+    BigGeom g;
+    g.load(b, p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk]);
+    const double pair_bound = p.q[pb] * p.q[pk];
+    unsigned long long mask = 0;
+    for (int d = 0; d < dd.ndens; ++d) {
+      const double *bd = dd.bound + (size_t)d * nsh * nsh;
+      double m = bd[g.sA * nsh + g.sB];
+      m = fmax(m, bd[g.sC * nsh + g.sD]);
+      m = fmax(m, bd[g.sA * nsh + g.sC]);
+      m = fmax(m, bd[g.sA * nsh + g.sD]);
+      m = fmax(m, bd[g.sB * nsh + g.sC]);
+      m = fmax(m, bd[g.sB * nsh + g.sD]);
+      if (pair_bound * m >= dd.tau) mask |= 1ull << d;
+    }
+    if (!mask) continue;  // uniform over the workgroup
+    // this lane's elements of the slice and their table offsets
+    const int s0 = sl * kRtThreads * EPL;
+    int ox[EPL], oy[EPL], oz[EPL];
+    double acc[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      const int e = s0 + int(threadIdx.x) + kRtThreads * k;
+      ox[k] = oy[k] = oz[k] = 0;
+      acc[k] = 0.0;
+      if (e < K.N4) {
+        const unsigned long long o = eoff[e];
+        ox[k] = int(o & m21);
+        oy[k] = int((o >> 21) & m21);
+        oz[k] = int(o >> 42);
+      }
+    }
+    const int ob = p.pp_off[pb], nbp = p.pp_n[pb], ok = p.pp_off[pk], nkp = p.pp_n[pk];
+    for (int ib = 0; ib < nbp; ++ib)
+      for (int ik = 0; ik < nkp; ++ik) {
+        PrimQ q;
+        prim_quartet(p, ob, ib, ok, ik, omega, q);
+        roots_rt(R, K.NR, q, S);
+        // build each pair on the centre of its tighter primitive (uniform
+        // over the workgroup: one primitive quartet at a time)
+        const bool swap_b = q.be > q.al, swap_k = q.de > q.ga;
+        const int ea = swap_b ? lb : la, eb = swap_b ? la : lb;  // built / received
+        const int ec = swap_k ? ld : lc, ed = swap_k ? lc : ld;
+        for (int r0 = 0; r0 < K.NR; r0 += rb) {
+          const int nr = K.NR - r0 < rb ? K.NR - r0 : rb;
+          double *out = dyn, *gt = dyn + 3 * rb * L.T, *ht = gt + 3 * rb * L.G;
+          for (int task = threadIdx.x; task < 3 * nr; task += kRtThreads) {
+            const int r = r0 + task / 3, ax = task % 3;
+            const double x = S.rw[r];
+            const double fq = x * q.qz * q.ipq, fp = x * q.pz * q.ipq;
+            const double b00 = 0.5 * x * q.ipq;
+            const double b10 = (1.0 - fq) * q.i2p;
+            const double b01 = (1.0 - fp) * q.i2q;
+            const double c00 =
+                (q.P[ax] - (swap_b ? g.A[ax] - g.AB[ax] : g.A[ax])) - fq * q.PQ[ax];
+            const double cp0 =
+                (q.Q[ax] - (swap_k ? g.C[ax] - g.CD[ax] : g.C[ax])) + fp * q.PQ[ax];
+            vrr_rt(NB, NK, c00, cp0, b10, b01, b00, gt + task * L.G);
+          }
+          __syncthreads();
+          // bra transfer: h(u, v; m) = h(u+1, v-1; m) + sab h(u, v-1; m)
+          const int nbra = 3 * nr * (ea + 1) * W;
+          for (int idx = threadIdx.x; idx < nbra; idx += kRtThreads) {
+            const int m = idx % W, r1 = idx / W, u = r1 % (ea + 1), task = r1 / (ea + 1);
+            const double sab = swap_b ? -g.AB[task % 3] : g.AB[task % 3];
+            const double *gg = gt + task * L.G + u * W + m;
+            double vec[kRtX];
+#pragma unroll
+            for (int i = 0; i < kRtX; ++i) vec[i] = i <= eb ? gg[i * W] : 0.0;
+            double *hh = ht + task * L.H + u * (eb + 1) * W + m;
+#pragma unroll
+            for (int v = 0; v < kRtX; ++v) {
+              if (v > eb) break;
+              hh[v * W] = vec[0];
+#pragma unroll
+              for (int i = 0; i + 1 < kRtX; ++i) vec[i] = vec[i + 1] + sab * vec[i];
+            }
+          }
+          __syncthreads();
+          // ket transfer: I(.., uk, vk) = I(.., uk+1, vk-1) + scd I(.., uk, vk-1);
+          // the z table takes the weight and prefactor
+          const int nab = (la + 1) * (lb + 1);
+          const int nket = 3 * nr * nab * (ec + 1);
+          for (int idx = threadIdx.x; idx < nket; idx += kRtThreads) {
+            const int uk = idx % (ec + 1), r1 = idx / (ec + 1), uv = r1 % nab, task = r1 / nab;
+            const int u = uv / (eb + 1), v = uv % (eb + 1);
+            const int i = swap_b ? v : u, j = swap_b ? u : v;
+            const int ax = task % 3;
+            const double scd = swap_k ? -g.CD[ax] : g.CD[ax];
+            const double scale = ax == 2 ? q.pref * S.rw[K.NR + r0 + task / 3] : 1.0;
+            const double *hh = ht + task * L.H + uv * W + uk;
+            double vec[kRtX];
+#pragma unroll
+            for (int c = 0; c < kRtX; ++c) vec[c] = c <= ed ? hh[c] : 0.0;
+            double *oo = out + task * L.T + (i * (lb + 1) + j) * (lc + 1) * (ld + 1);
+#pragma unroll
+            for (int vk = 0; vk < kRtX; ++vk) {
+              if (vk > ed) break;
+              const int kk = swap_k ? vk : uk, ll = swap_k ? uk : vk;
+              oo[kk * (ld + 1) + ll] = scale * vec[0];
+#pragma unroll
+              for (int c = 0; c + 1 < kRtX; ++c) vec[c] = vec[c + 1] + scd * vec[c];
+            }
+          }
+          __syncthreads();
+          for (int rr = 0; rr < nr; ++rr) {
+            const double *X = out + 3 * rr * L.T, *Y = X + L.T, *Z = Y + L.T;
+#pragma unroll
+            for (int k = 0; k < EPL; ++k) acc[k] += X[ox[k]] * Y[oy[k]] * Z[oz[k]];
+          }
+          __syncthreads();  // the next batch rebuilds the tables
+        }
+      }
+    if (sl == 0 && threadIdx.x == 0) {
+      ++my_q;
+      my_p += (unsigned long long)nbp * nkp;
+    }
+    // digest (the tables' LDS is free: every lane passed the last barrier):
+    // the slice to LDS once, then per density the six density sub-blocks of
+    // the quartet and the J'/K' entries the slice touches.  Entries with many
+    // terms (J[a,b], K[a,c], K[a,d]) take a wavefront each (lanes stride over
+    // the terms, then a wavefront sum); entries with few (J[c,d], K[b,c],
+    // K[b,d]) a lane each.  Every sum runs over the rows (a, b) the slice
+    // holds.
+    const double f =
+        (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (pb == pk ? 0.5 : 1.0);
+    const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
+              d0 = b.off_cart[g.sD];
+    const int na = K.na, nb = K.nb, nc = K.nc, nd = K.nd, ncd = nc * nd;
+    const int nds = na * nb + ncd + na * nc + na * nd + nb * nc + nb * nd;
+    double *Dab = dyn, *Dcd = Dab + na * nb, *Dac = Dcd + ncd, *Dad = Dac + na * nc,
+           *Dbc = Dad + na * nd, *Dbd = Dbc + nb * nc, *blk = Dab + nds;
+    const int s1 = s0 + kRtThreads * EPL < K.N4 ? s0 + kRtThreads * EPL : K.N4;
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+      const int el = s0 + int(threadIdx.x) + kRtThreads * k;
+      if (el < s1) blk[el - s0] = acc[k];
+    }
+    const int r0 = s0 / ncd, r1 = (s1 - 1) / ncd + 1;  // rows (a, b) of the slice
+    const int A0 = r0 / nb, A1 = (r1 - 1) / nb + 1, nA = A1 - A0;
+    const bool wk = dd.kc && dd.want_k;
+    const size_t xo = jk_copy_offset(dd);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // block element (a, bb, c, d) of the slice, or 0 outside it
+#define IG_RT_G(a, bb, c, d)                                           \
+  ([&]() -> double {                                                   \
+    const int el_ = (((a) * nb + (bb)) * nc + (c)) * nd + (d);         \
+    return (el_ >= s0 && el_ < s1) ? blk[el_ - s0] : 0.0;              \
+  }())
+    for (int e = 0; e < dd.neff; ++e) {
+      if (!((mask >> dd.eff_src[e]) & 1ull)) continue;  // uniform
+      const bool anti = dd.eff_anti[e];                 // antisymmetric parts feed K only
+      if (anti && !wk) continue;
+      const double *D = dd.dc + (size_t)e * n2;
+      double *J = dd.jc + xo + (size_t)e * n2;
+      double *Kp = wk ? dd.kc + xo + (size_t)e * n2 : nullptr;
+      __syncthreads();  // the slice is in LDS; the previous density's sums are done
+      for (int i = threadIdx.x; i < nds; i += kRtThreads) {
+        int q = i, row, col;
+        if (q < na * nb) {
+          row = a0 + q / nb;
+          col = b0 + q % nb;
+        } else if ((q -= na * nb) < ncd) {
+          row = c0 + q / nd;
+          col = d0 + q % nd;
+        } else if ((q -= ncd) < na * nc) {
+          row = a0 + q / nc;
+          col = c0 + q % nc;
+        } else if ((q -= na * nc) < na * nd) {
+          row = a0 + q / nd;
+          col = d0 + q % nd;
+        } else if ((q -= na * nd) < nb * nc) {
+          row = b0 + q / nc;
+          col = c0 + q % nc;
+        } else {
+          q -= nb * nc;
+          row = b0 + q / nd;
+          col = d0 + q % nd;
+        }
+        Dab[i] = D[(size_t)row * ncrt + col];
+      }
+      __syncthreads();
+      // wavefront entries: J[a,b] rows, then K[a,c], then K[a,d]
+      const int nw0 = anti ? 0 : r1 - r0, nw1 = wk ? nA * nc : 0, nw2 = wk ? nA * nd : 0;
+      for (int o = wave; o < nw0 + nw1 + nw2; o += kRtThreads / 64) {
+        double v = 0.0;
+        double *dst;
+        int row, col;
+        if (o < nw0) {  // J[a,b] += 2f sum_cd g D[c,d]
+          const int rr = r0 + o;
+          const int lo = s0 > rr * ncd ? s0 - rr * ncd : 0;
+          const int hi = s1 < (rr + 1) * ncd ? s1 - rr * ncd : ncd;
+          for (int cd = lo + lane; cd < hi; cd += 64) v += blk[rr * ncd + cd - s0] * Dcd[cd];
+          v *= 2.0 * f;
+          dst = J;
+          row = a0 + rr / nb;
+          col = b0 + rr % nb;
+        } else {
+          int q = o - nw0;
+          const bool ac = q < nw1;
+          if (!ac) q -= nw1;
+          const int a = A0 + q / (ac ? nc : nd), x = q % (ac ? nc : nd);
+          const int bl = r0 - a * nb > 0 ? r0 - a * nb : 0;
+          const int bh = r1 - a * nb < nb ? r1 - a * nb : nb;
+          const int ny = ac ? nd : nc;  // summed index of the other ket shell
+          for (int t2 = bl * ny + lane; t2 < bh * ny; t2 += 64) {
+            const int bb = t2 / ny, y = t2 % ny;
+            if (ac)  // Kp[a,c] += f sum_bd g D[b,d]
+              v += IG_RT_G(a, bb, x, y) * Dbd[bb * nd + y];
+            else  // Kp[a,d] += f sum_bc g D[b,c]
+              v += IG_RT_G(a, bb, y, x) * Dbc[bb * nc + y];
+          }
+          v *= f;
+          dst = Kp;
+          row = a0 + a;
+          col = (ac ? c0 : d0) + x;
+        }
+        v = wave_sum_d(v);
+        if (lane == 0 && v != 0.0) jk_add(dd, dst + (size_t)row * ncrt + col, v);
+      }
+      // lane entries: J[c,d], then K[b,c], then K[b,d]
+      const int nl0 = anti ? 0 : ncd, nl1 = wk ? nb * nc : 0, nl2 = wk ? nb * nd : 0;
+      for (int o = threadIdx.x; o < nl0 + nl1 + nl2; o += kRtThreads) {
+        double v = 0.0;
+        double *dst;
+        int row, col;
+        if (o < nl0) {  // J[c,d] += 2f sum_ab g D[a,b]
+          for (int rr = r0; rr < r1; ++rr) {
+            const int el = rr * ncd + o;
+            if (el >= s0 && el < s1) v += blk[el - s0] * Dab[rr];
+          }
+          v *= 2.0 * f;
+          dst = J;
+          row = c0 + o / nd;
+          col = d0 + o % nd;
+        } else {
+          int q = o - nl0;
+          const bool bc = q < nl1;
+          if (!bc) q -= nl1;
+          const int bb = q / (bc ? nc : nd), x = q % (bc ? nc : nd);
+          const int ny = bc ? nd : nc;
+          // a with row (a, bb) in the slice
+          int al = (r0 - bb + nb - 1) / nb;
+          if (r0 - bb <= 0) al = 0;
+          const int ah = (r1 - 1 - bb) >= 0 ? (r1 - 1 - bb) / nb + 1 : 0;
+          for (int a = al; a < ah; ++a)
+            for (int y = 0; y < ny; ++y) {
+              if (bc)  // Kp[b,c] += f sum_ad g D[a,d]
+                v += IG_RT_G(a, bb, x, y) * Dad[a * nd + y];
+              else  // Kp[b,d] += f sum_ac g D[a,c]
+                v += IG_RT_G(a, bb, y, x) * Dac[a * nc + y];
+            }
+          v *= f;
+          dst = Kp;
+          row = b0 + bb;
+          col = (bc ? c0 : d0) + x;
+        }
+        if (v != 0.0) jk_add(dd, dst + (size_t)row * ncrt + col, v);
+      }
+    }
+#undef IG_RT_G
+    __syncthreads();  // the next work item rebuilds the tables in this LDS
   }
   if (counters && threadIdx.x == 0) {
     atomicAdd(counters, my_q);
@@ -722,20 +1099,66 @@ double *big_prepare(int la, int lb, int lc, int ld, double *scratch, hipStream_t
                      reinterpret_cast<unsigned long long *>(scratch));
   return scratch + (n4 + 31) / 32 * 32;
 }
+
+// Dynamic LDS and launch shape of jk_rt_kernel for class (la..ld): roots per
+// table batch (as many as the budget holds, at least one), the digest's
+// staged sub-blocks and slice, elements per lane and slices per quartet.
+struct RtLaunch {
+  int rb, epl, nsl;
+  size_t lds;
+  RtLaunch(int la, int lb, int lc, int ld) {
+    const RtLayout L(la, lb, lc, ld);
+    const int nr = (la + lb + lc + ld) / 2 + 1;
+    const size_t per = size_t(L.per_root()) * sizeof(double);
+    const size_t budget = per > kRtLdsBudget ? per : kRtLdsBudget;
+    rb = int(budget / per);
+    rb = rb < 1 ? 1 : (rb > nr ? nr : rb);
+    const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    const int nds = na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
+    const int n4 = na * nb * nc * nd;
+    epl = n4 <= 4 * kRtThreads ? 4 : 16;
+    const int slice = n4 < kRtThreads * epl ? n4 : kRtThreads * epl;
+    const size_t dig = size_t(nds + slice) * sizeof(double);
+    lds = per * rb > dig ? per * rb : dig;
+    nsl = (n4 + kRtThreads * epl - 1) / (kRtThreads * epl);
+  }
+};
 }  // namespace
+
+// J/K scratch of the runtime-l classes: the element offset table, and for
+// the blocks under kRtMinElems one block slot per workgroup.
+size_t jk_big_scratch_elems(int la, int lb, int lc, int ld, long long total) {
+  const size_t n4 = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
+  if (n4 < size_t(kRtMinElems)) return big_scratch_elems(la, lb, lc, ld, total, 0);
+  return (n4 + 31) / 32 * 32;
+}
 
 hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, const PairsDev &p,
                          const RysDev &r, const QuartetSet &qs, const DensDev &d, double omega,
                          double *scratch, unsigned long long *ctr, hipStream_t s) {
   if (qs.total <= 0) return hipSuccess;
-  const size_t slot = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
-  int rb;
-  size_t lds;
-  root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
-  double *slots = big_prepare(la, lb, lc, ld, scratch, s);
-  hipLaunchKernelGGL(jk_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la,
-                     lb, lc, ld, rb, b, p, r, qs, d, omega,
-                     reinterpret_cast<const unsigned long long *>(scratch), slots, slot, ctr);
+  const size_t n4 = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
+  if (n4 < size_t(kRtMinElems)) {  // one workgroup per quartet, block slots in scratch
+    int rb;
+    size_t lds;
+    root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
+    double *slots = big_prepare(la, lb, lc, ld, scratch, s);
+    hipLaunchKernelGGL(jk_big_kernel, dim3(big_grid(qs.total, n4)), dim3(kBigThreads), lds, s,
+                       la, lb, lc, ld, rb, b, p, r, qs, d, omega,
+                       reinterpret_cast<const unsigned long long *>(scratch), slots, n4, ctr);
+    return hipGetLastError();
+  }
+  const RtLaunch RL(la, lb, lc, ld);
+  const long long work = qs.total * RL.nsl;
+  const int grid = int(work < kRtMaxWG ? work : kRtMaxWG);
+  big_prepare(la, lb, lc, ld, scratch, s);  // the offset table at the front of scratch
+  const auto *eoff = reinterpret_cast<const unsigned long long *>(scratch);
+  if (RL.epl == 4)
+    hipLaunchKernelGGL(jk_rt_kernel<4>, dim3(grid), dim3(kRtThreads), RL.lds, s, la, lb, lc, ld,
+                       RL.rb, RL.nsl, b, p, r, qs, d, omega, eoff, ctr);
+  else
+    hipLaunchKernelGGL(jk_rt_kernel<16>, dim3(grid), dim3(kRtThreads), RL.lds, s, la, lb, lc, ld,
+                       RL.rb, RL.nsl, b, p, r, qs, d, omega, eoff, ctr);
   return hipGetLastError();
 }
 

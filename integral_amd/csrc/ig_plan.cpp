@@ -913,7 +913,7 @@ size_t big_need(const ClassRange &X, const ClassRange &Y, long long total, int t
 // the same for J/K, whose compiled kernels also cover most f classes
 size_t big_need_jk(const ClassRange &X, const ClassRange &Y, long long total) {
   if (total <= 0 || jk_compiled(X.l1, X.l2, Y.l1, Y.l2)) return 0;
-  return big_scratch_elems(X.l1, X.l2, Y.l1, Y.l2, total, 0);
+  return jk_big_scratch_elems(X.l1, X.l2, Y.l1, Y.l2, total);
 }
 
 int jk_variant(const Plan &P, const ClassRange &X, const ClassRange &Y) {
