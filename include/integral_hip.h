@@ -397,6 +397,16 @@ int ig_batch_schedule_write(ig_batch *batch, const ig_schedule_entry *in, size_t
 /* Quartets accepted ([0]) and primitive quartets ([1]) of the last call,
  * summed over the molecules.  Synchronises the batch's stream. */
 int ig_batch_last_stats(ig_batch *batch, int64_t *stats2);
+/* Diagnostic with no counterpart in the reference: prepares the device views
+ * and work lists that ig_batch_jk_device would capture into its graph for
+ * (threshold, want_exchange, the batch's precision policy), runs no kernel,
+ * and walks every quartet of every batched row-sum launch on the host with
+ * that kernel's indexing (chunk -> molecule decode, row cursor, pair, shell
+ * and matrix indices, LDS row-sum columns), checking each baked pointer
+ * against its plan buffer.  IG_OK when every access is in range; otherwise
+ * IG_ERR_DEVICE with the first offending access, also written to msg (cap
+ * bytes, NUL-terminated; msg may be null).  For tests of the batched path. */
+int ig_batch_check(ig_batch *batch, double threshold, int want_exchange, char *msg, size_t cap);
 
 /* ---- ERI-derivative contraction: DirectGradContractor (direct_grad.rs) --- */
 typedef struct ig_grad ig_grad;

@@ -237,6 +237,7 @@ def library():
     lib.ig_batch_jk_device.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]
     lib.ig_batch_last_stats.argtypes = [_vp, ctypes.POINTER(i64)]
+    lib.ig_batch_check.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_char_p, sz]
     lib.ig_batch_autotune.argtypes = [_vp, sz, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       _vp]
     lib.ig_batch_schedule_write.argtypes = [_vp, ctypes.POINTER(_ScheduleEntry), sz]
@@ -1034,6 +1035,21 @@ class BatchContractor:
         st = (ctypes.c_int64 * 2)()
         _raise(library().ig_batch_last_stats(self._h, st))
         return int(st[0]), int(st[1])
+
+    def check(self, threshold=0.0, want_exchange=True):
+        """Host walk of the batched launches' indexing for this threshold
+        (ig_batch_check): runs no kernel; returns "" when every access is in
+        range, else the first offending access."""
+        buf = ctypes.create_string_buffer(512)
+        rc = library().ig_batch_check(self._h, float(threshold), 1 if want_exchange else 0,
+                                      buf, len(buf))
+        if rc == 0:
+            return ""
+        msg = buf.value.decode()
+        if msg:
+            return msg
+        _raise(rc)
+        return "check failed"
 
     def set_schedule(self, entries):
         """Load entries as returned by DirectContractor.schedule() into every

@@ -11,6 +11,7 @@
 #include <mutex>
 #include <tuple>
 #include <numeric>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -2667,16 +2668,42 @@ struct ig_batch {
 
 namespace {
 
-// Capture the whole batched J/K into b.graph: per basis the density kernels,
-// then per (class pair, precision) one launch of the row-sum kernel over every
-// basis (work queue) -- or per-basis launches for classes that kernel cannot
-// run -- then per basis the fold and finishing transform.
-void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
+// Views and work lists of a batched J/K: what a capture bakes into the graph.
+struct BatchKey {
+  int x, y, fp32;
+  bool operator<(const BatchKey &o) const {
+    return std::tie(x, y, fp32) < std::tie(o.x, o.y, o.fp32);
+  }
+};
+struct BatchLaunch {
+  BatchKey k;
+  bool batched;  // one row-sum launch over every basis (else per-basis launches)
+  int variant;
+  size_t acc;    // LDS accumulator doubles of the widest basis with the key
+  long long total;
+  size_t set_off, seg_off;
+};
+struct BatchPrep {
+  std::vector<BasisDev> bs;
+  std::vector<PairsDev> ps;
+  std::vector<DensDev> dds;
+  std::map<BatchKey, std::vector<std::pair<int, size_t>>> work;  // key -> (basis, cp index)
+  std::vector<BatchLaunch> launches;
+  std::vector<QuartetSet> sets;  // nb per batched launch (empty sets for bases without the key)
+  std::vector<long long> seg;    // nb + 1 prefix sums per batched launch
+};
+
+// Host side of a batched J/K for (tau, cut, want_k): quartet sets, density
+// buffers, device views and work lists uploaded, scratch reserved.  No
+// kernel runs.
+void batch_prepare(ig_batch &B, double tau, double cut, bool want_k, BatchPrep &bp) {
   const int nb = int(B.plans.size());
-  RysDevice &R = rys_device();
-  std::vector<BasisDev> bs(nb);
-  std::vector<PairsDev> ps(nb);
-  std::vector<DensDev> dds(nb);
+  std::vector<BasisDev> &bs = bp.bs;
+  std::vector<PairsDev> &ps = bp.ps;
+  std::vector<DensDev> &dds = bp.dds;
+  bs.assign(nb, BasisDev{});
+  ps.assign(nb, PairsDev{});
+  dds.assign(nb, DensDev{});
   for (int m = 0; m < nb; ++m) {
     Plan &P = *B.plans[m];
     build_rowptrs(P, tau, cut);
@@ -2685,13 +2712,8 @@ void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
     ps[m] = P.pdev;
   }
   // work lists: every (x, y, fp32) key present in some basis
-  struct Key {
-    int x, y, fp32;
-    bool operator<(const Key &o) const {
-      return std::tie(x, y, fp32) < std::tie(o.x, o.y, o.fp32);
-    }
-  };
-  std::map<Key, std::vector<std::pair<int, size_t>>> work;  // key -> (basis, cp index)
+  auto &work = bp.work;
+  work.clear();
   for (int m = 0; m < nb; ++m) {
     Plan &P = *B.plans[m];
     for (size_t c = 0; c < P.cached_cps.size(); ++c) {
@@ -2699,22 +2721,17 @@ void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
       if (cp.total) work[{cp.x, cp.y, cp.fp32}].push_back({m, c});
     }
   }
-  struct Launch {
-    Key k;
-    bool batched;
-    int variant;
-    size_t acc;
-    long long total;
-    size_t set_off, seg_off;
-  };
-  std::vector<Launch> launches;
-  std::vector<QuartetSet> sets;
-  std::vector<long long> seg;
+  auto &launches = bp.launches;
+  auto &sets = bp.sets;
+  auto &seg = bp.seg;
+  launches.clear();
+  sets.clear();
+  seg.clear();
   for (auto &kv : work) {
-    const Key &k = kv.first;
+    const BatchKey &k = kv.first;
     const Plan &P0 = *B.plans[kv.second[0].first];
     const ClassRange &X0 = P0.cls[k.x], &Y0 = P0.cls[k.y];
-    Launch L{k, false, -1, 0, 0, sets.size(), seg.size()};
+    BatchLaunch L{k, false, -1, 0, 0, sets.size(), seg.size()};
     if (jk_lane_compiled(X0.l1, X0.l2, Y0.l1, Y0.l2)) {
       int v = jk_variant(P0, X0, Y0);
       if ((v == kJkTeam || v == kJkTeamHrr) && k.fp32) v = kJkRowAcc;
@@ -2761,6 +2778,20 @@ void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
       if (need > P.s_big.bytes / sizeof(double)) P.s_big.reserve(need * sizeof(double));
     }
   }
+}
+
+// Capture the whole batched J/K into b.graph: per basis the density kernels,
+// then per (class pair, precision) one launch of the row-sum kernel over every
+// basis (work queue) -- or per-basis launches for classes that kernel cannot
+// run -- then per basis the fold and finishing transform.
+void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
+  const int nb = int(B.plans.size());
+  RysDevice &R = rys_device();
+  BatchPrep bp;
+  batch_prepare(B, tau, cut, want_k, bp);
+  auto &work = bp.work;
+  const auto &launches = bp.launches;
+  const auto &dds = bp.dds;
   hipStream_t s = B.stream;
   check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
   try {
@@ -2769,7 +2800,7 @@ void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
     check(launch_fill(B.d_queue.as<double>(), B.d_queue.bytes / sizeof(double), 0.0, s), "zero queues");
     check(launch_fill(B.d_ctr.as<double>(), B.d_ctr.bytes / sizeof(double), 0.0, s), "zero counters");
     for (size_t li = 0; li < launches.size(); ++li) {
-      const Launch &L = launches[li];
+      const BatchLaunch &L = launches[li];
       unsigned long long *ctr = B.d_ctr.as<unsigned long long>() + 2 * li;
       const Plan &P0 = *B.plans[work[L.k][0].first];
       const ClassRange &X0 = P0.cls[L.k.x], &Y0 = P0.cls[L.k.y];
@@ -2807,6 +2838,153 @@ void batch_capture(ig_batch &B, double tau, double cut, bool want_k) {
   B.graph = nullptr;
   check(hipStreamEndCapture(s, &B.graph), "end capture");
   check(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0), "instantiate graph");
+}
+
+
+// Device array [p, p + n) back to the host (small tables; a synchronous copy).
+template <class T>
+std::vector<T> batch_copy_back(const T *p, size_t n) {
+  std::vector<T> h(n);
+  if (n) check(hipMemcpy(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost), "copy back");
+  return h;
+}
+
+// Is [p, p + bytes) inside buffer b?
+bool batch_inside(const DBuf &b, const void *p, size_t bytes) {
+  const char *lo = static_cast<const char *>(b.p), *q = static_cast<const char *>(p);
+  return b.p && q >= lo && q + bytes <= lo + b.bytes;
+}
+
+// Host walk of every quartet of every batched launch with the indexing of
+// jk_rowacc_batch_kernel (chunk -> basis decode, row cursor, pair, shell and
+// matrix indices, LDS row-sum columns): the first access outside its buffer,
+// or "" when there is none.  Reads back from the device what the kernels
+// would read (row pointers, segment kets, pair and shell tables) and checks
+// every pointer the launch views hold against the plan buffers.
+std::string batch_replay(ig_batch &B, const BatchPrep &bp) {
+  const int nb = int(B.plans.size());
+  std::ostringstream err;
+  auto fail_at = [&](const BatchLaunch &L, int m, long long t, const std::string &what) {
+    err << "class pair (" << L.k.x << "," << L.k.y << ") fp32 " << L.k.fp32 << ", basis " << m
+        << ", quartet " << t << ": " << what;
+    return err.str();
+  };
+  // per basis: density views against the plan buffers
+  for (int m = 0; m < nb; ++m) {
+    Plan &P = *B.plans[m];
+    const DensDev &dd = bp.dds[m];
+    const size_t n2 = size_t(P.nao_cart) * P.nao_cart;
+    BatchLaunch none{};
+    if (!batch_inside(P.s_dc, dd.dc, n2 * dd.neff * sizeof(double)))
+      return fail_at(none, m, -1, "Cartesian densities outside s_dc");
+    if (!batch_inside(P.s_jc, dd.jc, dd.copy_stride * dd.ncopy * sizeof(double)))
+      return fail_at(none, m, -1, "J accumulators outside s_jc");
+    if (dd.kc && !batch_inside(P.s_kc, dd.kc, dd.copy_stride * dd.ncopy * sizeof(double)))
+      return fail_at(none, m, -1, "K accumulators outside s_kc");
+    if (dd.copy_stride < n2 * size_t(dd.neff)) return fail_at(none, m, -1, "copy stride too small");
+    if (!batch_inside(P.s_bound, dd.bound, size_t(dd.ndens) * P.nsh * P.nsh * sizeof(double)))
+      return fail_at(none, m, -1, "density bounds outside s_bound");
+    if (!batch_inside(P.s_eff, dd.eff_src, 2 * size_t(dd.neff) * sizeof(int)))
+      return fail_at(none, m, -1, "effective-density table outside s_eff");
+  }
+  for (const BatchLaunch &L : bp.launches) {
+    if (!L.batched) continue;
+    const Plan &P0 = *B.plans[bp.work.at(L.k)[0].first];
+    const ClassRange &X0 = P0.cls[L.k.x], &Y0 = P0.cls[L.k.y];
+    const int la = X0.l1, lb = X0.l2, lc = Y0.l1, ld = Y0.l2;
+    const int na = ncart(la), nb_ = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    // chunk -> basis decode of the kernel over the launch's prefix sums
+    const long long *seg = bp.seg.data() + L.seg_off;
+    if (seg[0] != 0 || seg[nb] != L.total) return fail_at(L, -1, -1, "prefix sums do not span the launch");
+    for (int m = 0; m < nb; ++m)
+      if (seg[m + 1] < seg[m]) return fail_at(L, m, -1, "prefix sums decrease");
+    // the kernel's work items: chunks of 1024 quartets (4 per lane of its 256
+    // lanes), each cut at the basis boundaries it crosses
+    const long long chunk = 1024;
+    for (long long c0 = 0; c0 < L.total; c0 += chunk) {
+      long long g0 = c0;
+      const long long g1 = c0 + chunk < L.total ? c0 + chunk : L.total;
+      while (g0 < g1) {
+        int lo = 0, hi = nb - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (seg[mid] <= g0) lo = mid; else hi = mid - 1;
+        }
+        if (!(seg[lo] <= g0 && g0 < seg[lo + 1])) return fail_at(L, lo, g0, "basis decode misses");
+        const long long e = g1 < seg[lo + 1] ? g1 : seg[lo + 1];
+        g0 = e;
+      }
+    }
+    for (int m = 0; m < nb; ++m) {
+      const QuartetSet &qs = bp.sets[L.set_off + m];
+      if (qs.total != seg[m + 1] - seg[m]) return fail_at(L, m, -1, "set total differs from the prefix sums");
+      if (!qs.total) continue;
+      Plan &P = *B.plans[m];
+      if (qs.list) return fail_at(L, m, -1, "explicit list in a batched launch");
+      const ClassRange &X = P.cls[L.k.x], &Y = P.cls[L.k.y];
+      if (X.l1 != la || X.l2 != lb || Y.l1 != lc || Y.l2 != ld) return fail_at(L, m, -1, "class momenta differ between bases");
+      if (qs.x_off != X.off || qs.x_n != X.n || qs.y_off != Y.off || qs.y_n != Y.n)
+        return fail_at(L, m, -1, "set ranges differ from the plan's class ranges");
+      const size_t nrow = size_t(qs.x_n) * qs.nseg;
+      if (qs.nseg < 1) return fail_at(L, m, -1, "no segments");
+      if (!batch_inside(P.s_rowptr, qs.rowptr, (nrow + 1) * sizeof(long long)))
+        return fail_at(L, m, -1, "row pointers outside s_rowptr");
+      if (!batch_inside(P.s_qbase, qs.qbase, nrow * sizeof(int)))
+        return fail_at(L, m, -1, "segment kets outside s_qbase");
+      const std::vector<long long> rp = batch_copy_back(qs.rowptr, nrow + 1);
+      const std::vector<int> qb = batch_copy_back(qs.qbase, nrow);
+      if (rp[0] != 0 || rp[nrow] != qs.total) return fail_at(L, m, -1, "row pointers do not span the set");
+      const std::vector<int> sh1 = batch_copy_back(P.pdev.sh1, size_t(P.npairs));
+      const std::vector<int> sh2 = batch_copy_back(P.pdev.sh2, size_t(P.npairs));
+      const std::vector<int> ppo = batch_copy_back(P.pdev.pp_off, size_t(P.npairs));
+      const std::vector<int> ppn = batch_copy_back(P.pdev.pp_n, size_t(P.npairs));
+      const std::vector<int> shl = batch_copy_back(P.bdev.l, size_t(P.nsh));
+      const std::vector<int> offc = batch_copy_back(P.bdev.off_cart, size_t(P.nsh));
+      const std::vector<int> rank = batch_copy_back(P.bdev.rank, size_t(P.nsh));
+      const std::vector<int> lsh = batch_copy_back(P.bdev.lshells, size_t(P.nsh));
+      const size_t nprim = P.d_zeta.bytes / sizeof(double);
+      // row-sum columns of the ket class and the accumulators they index
+      const int nC = P.bdev.lshell_off[lc + 1] - P.bdev.lshell_off[lc];
+      const int nD = P.bdev.lshell_off[ld + 1] - P.bdev.lshell_off[ld];
+      const int W = nC * nc + (lc == ld ? 0 : nD * nd);
+      const int dbase = lc == ld ? 0 : nC * nc;
+      const size_t ntot = rowacc_acc_doubles(P.bdev, la, lb, lc, ld, bp.dds[m].neff);
+      if (ntot > L.acc) return fail_at(L, m, -1, "row sums need more LDS than the launch holds");
+      for (int r = 0; r < nC; ++r) {  // the flush's column -> function map
+        const int sh = lsh[P.bdev.lshell_off[lc] + r];
+        if (sh < 0 || sh >= P.nsh || shl[sh] != lc) return fail_at(L, m, -1, "column shell of the wrong momentum");
+      }
+      for (int r = 0; r < nD; ++r) {
+        const int sh = lsh[P.bdev.lshell_off[ld] + r];
+        if (sh < 0 || sh >= P.nsh || shl[sh] != ld) return fail_at(L, m, -1, "column shell of the wrong momentum");
+      }
+      size_t sg = 0;
+      for (long long t = 0; t < qs.total; ++t) {
+        while (sg + 1 <= nrow && rp[sg + 1] <= t) ++sg;  // the row cursor's segment
+        if (sg >= nrow) return fail_at(L, m, t, "segment past the set");
+        const int Prow = int(sg / qs.nseg);
+        const long long Q = qb[sg] + (t - rp[sg]);
+        if (Prow >= qs.x_n || Q < 0 || Q >= qs.y_n) return fail_at(L, m, t, "pair index outside the class");
+        const int pb = qs.x_off + Prow, pk = qs.y_off + int(Q);
+        if (pb >= P.npairs || pk >= P.npairs) return fail_at(L, m, t, "pair index outside the pair table");
+        if (ppo[pb] < 0 || size_t(ppo[pb]) + ppn[pb] > nprim || ppo[pk] < 0 || size_t(ppo[pk]) + ppn[pk] > nprim)
+          return fail_at(L, m, t, "primitive pairs outside the primitive table");
+        const int sA = sh1[pb], sB = sh2[pb], sC = sh1[pk], sD = sh2[pk];
+        for (int sh : {sA, sB, sC, sD})
+          if (sh < 0 || sh >= P.nsh) return fail_at(L, m, t, "shell index outside the basis");
+        if (shl[sA] != la || shl[sB] != lb || shl[sC] != lc || shl[sD] != ld)
+          return fail_at(L, m, t, "shell momentum differs from the class");
+        const int ns[4] = {na, nb_, nc, nd};
+        const int ss[4] = {sA, sB, sC, sD};
+        for (int i = 0; i < 4; ++i)
+          if (offc[ss[i]] < 0 || offc[ss[i]] + ns[i] > P.nao_cart)
+            return fail_at(L, m, t, "Cartesian functions outside the matrices");
+        if (rank[sC] * nc + nc > (lc == ld ? W : dbase) || dbase + rank[sD] * nd + nd > W)
+          return fail_at(L, m, t, "row-sum column outside the LDS rows");
+      }
+    }
+  }
+  return std::string();
 }
 
 }  // namespace
@@ -2889,6 +3067,27 @@ extern "C" int ig_batch_jk_device(ig_batch *b, const double *const *d_dens, doub
     check(hipEventRecord(b->ev_out, b->stream), "event");
     check(hipStreamWaitEvent(s, b->ev_out, 0), "wait");
     b->stats_pending = true;
+    return int(IG_OK);
+  });
+}
+
+extern "C" int ig_batch_check(ig_batch *b, double threshold, int want_k, char *msg, size_t cap) {
+  if (!b) return fail(IG_ERR_INVALID_ARGUMENT, "null argument");
+  if (!(std::isfinite(threshold) && threshold >= 0.0))
+    return fail(IG_ERR_INVALID_ARGUMENT, "threshold must be finite and >= 0");
+  return guarded([&] {
+    std::lock_guard<std::mutex> g(b->mu);
+    check(hipStreamSynchronize(b->stream), "sync");
+    const double cut = b->fp32_below > threshold ? b->fp32_below : threshold;
+    BatchPrep bp;
+    batch_prepare(*b, threshold, cut, want_k != 0, bp);
+    const std::string r = batch_replay(*b, bp);
+    if (msg && cap) {
+      const size_t n = std::min(cap - 1, r.size());
+      std::memcpy(msg, r.data(), n);
+      msg[n] = '\0';
+    }
+    if (!r.empty()) throw DeviceError{IG_ERR_DEVICE, "batched launch data out of range: " + r};
     return int(IG_OK);
   });
 }
