@@ -1116,7 +1116,7 @@ struct RtLaunch {
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
     const int nds = na * nb + nc * nd + na * nc + na * nd + nb * nc + nb * nd;
     const int n4 = na * nb * nc * nd;
-    epl = n4 <= 4 * kRtThreads ? 4 : 16;
+    epl = 16;
     const int slice = n4 < kRtThreads * epl ? n4 : kRtThreads * epl;
     const size_t dig = size_t(nds + slice) * sizeof(double);
     lds = per * rb > dig ? per * rb : dig;
@@ -1153,12 +1153,8 @@ hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, cons
   const int grid = int(work < kRtMaxWG ? work : kRtMaxWG);
   big_prepare(la, lb, lc, ld, scratch, s);  // the offset table at the front of scratch
   const auto *eoff = reinterpret_cast<const unsigned long long *>(scratch);
-  if (RL.epl == 4)
-    hipLaunchKernelGGL(jk_rt_kernel<4>, dim3(grid), dim3(kRtThreads), RL.lds, s, la, lb, lc, ld,
-                       RL.rb, RL.nsl, b, p, r, qs, d, omega, eoff, ctr);
-  else
-    hipLaunchKernelGGL(jk_rt_kernel<16>, dim3(grid), dim3(kRtThreads), RL.lds, s, la, lb, lc, ld,
-                       RL.rb, RL.nsl, b, p, r, qs, d, omega, eoff, ctr);
+  hipLaunchKernelGGL(jk_rt_kernel<16>, dim3(grid), dim3(kRtThreads), RL.lds, s, la, lb, lc, ld,
+                     RL.rb, RL.nsl, b, p, r, qs, d, omega, eoff, ctr);
   return hipGetLastError();
 }
 
