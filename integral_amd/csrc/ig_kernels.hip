@@ -335,10 +335,12 @@ int jk_default_variant(int la, int lb, int lc, int ld) {
   // (measured on s,p,d,f molecules: the contracted-space transfers win on
   // the largest blocks with a d or f second ket shell, (fd|fd), (ff|dd),
   // (fd|dd); the autotuner finds the others)
-  // total angular momentum 2 ((ps|ps), (pp|ss), (ds|ss)): the OS/HGP block
-  // with row sums, 0.5-0.9 ms faster per class than the Rys one on the
-  // nao 520 s,p,d workload (three Boys orders, no roots)
-  if (la + lb + lc + ld == 2) return kJkHgp;
+  // total angular momentum 2 ((ps|ps), (pp|ss), (ds|ss)) and (pp|ps): the
+  // OS/HGP block with row sums, 0.5-0.9 ms faster per class than the Rys one
+  // on the nao 520 s,p,d workload (no roots; (pp|ps) contracts its 9x9
+  // primitive quartets in [e|f] space and transfers once); (ds|ps) and
+  // (dp|ss) time the same or slower there and stay on Rys
+  if (la + lb + lc + ld == 2 || (la == 1 && lb == 1 && lc == 1 && ld == 0)) return kJkHgp;
   const int n4 = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
   if (class_supported(la, lb, lc, ld) && n4 <= 8 * kRegBlockMax) return kJkRowAcc;
   if (n4 <= 2 * kRegBlockMax && jk_lane_compiled(la, lb, lc, ld)) return kJkRowAcc;

@@ -57,7 +57,7 @@ bool jk_variant_fits(int variant, int la, int lb, int lc, int ld, const BasisDev
   if (!jk_lane_compiled(la, lb, lc, ld)) return false;
   if (variant == kJkReg) return true;
   if (listed) return false;
-  if (variant == kJkHgp && la + lb + lc + ld > 2) return false;
+  if (variant == kJkHgp && !hgp_covered(la, lb, lc, ld)) return false;
   const int nr = (la + lb + lc + ld) / 2 + 1;
   const size_t rys =
       variant == kJkHgp ? 0 : size_t(2) * rys_host_tables().nsub[nr] * nr * kRysTerms * sizeof(double);
