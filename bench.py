@@ -271,9 +271,11 @@ def bench_highl(args):
     dc.set_profiling(True)
     run(1)
     recs = {}
-    for r in dc.profile():
+    prof = dc.profile()
+    for r in prof:
         key = "".join("spdfghi"[l] for l in r["cls"])
         recs[key] = recs.get(key, 0.0) + r["ms"]
+    roof = highl_roofline(prof)
     dc.set_profiling(False)
     dc.set_mixed_precision(cut)
     run(1)
@@ -296,7 +298,7 @@ def bench_highl(args):
                       "autotune_s": tune_s, "parallelism": "1 GPU"},
            "mixed": {"fp32_below": cut, "ms_per_step": mms, "speedup_vs_fp64": ms / mms,
                      "max_abs_dJ": dJ, "max_abs_dK": dK},
-           "classes_ms": dict(top)}
+           "classes_ms": dict(top), "roofline": roof}
     print(json.dumps(out))
 
 
@@ -522,6 +524,38 @@ def measure_eri(steps, warmup):
     return {"ms_per_tensor": ms, "quartets": npair * (npair + 1) // 2, "nao": 50}
 
 
+def highl_roofline(recs, ndens=1):
+    """FP64 roofline of the dominant class of a J/K build (records of
+    DirectContractor.profile() summed over launches) and of all classes."""
+    acc = {}
+    for r in recs:
+        a = acc.setdefault(tuple(r["cls"]), {"cls": tuple(r["cls"]), "quartets": 0,
+                                             "prim_quartets": 0, "ms": 0.0})
+        a["quartets"] += r["quartets"]
+        a["prim_quartets"] += r["prim_quartets"]
+        a["ms"] += r["ms"]
+    dom = max(acc.values(), key=lambda r: r["ms"])
+    n4 = 1
+    for l in dom["cls"]:
+        n4 *= (l + 1) * (l + 2) // 2
+    if max(dom["cls"]) <= 2:
+        kern = "compiled lane/team kernel"
+    elif n4 >= 16384:
+        kern = "jk_rt_kernel<16> (runtime l, slices of 4096 elements)"
+    else:
+        kern = "jk_big_kernel (runtime l, one quartet per workgroup)"
+    f_dom = class_flops(dom, ndens=ndens)
+    f_all = sum(class_flops(r, ndens=ndens) for r in acc.values())
+    ms_all = sum(r["ms"] for r in acc.values())
+    achieved = f_dom / (dom["ms"] * 1e-3) / 1e12
+    return {"bound": "valu", "pipe": "FP64 vector ALU (78.6 TFLOP/s)",
+            "class": "".join("spdfghi"[l] for l in dom["cls"]), "kernel": kern,
+            "achieved": round(achieved, 4), "peak": 78.6, "unit": "TFLOP/s",
+            "frac": round(achieved / 78.6, 5), "launch_ms": round(dom["ms"], 4),
+            "all_classes_achieved": round(f_all / (ms_all * 1e-3) / 1e12, 4),
+            "all_classes_frac": round(f_all / (ms_all * 1e-3) / 1e12 / 78.6, 5)}
+
+
 def measure_highl(steps, warmup):
     """Milliseconds per J/K build of the nao 565 molecule with g shells and an
     l = 6 shell on every fourth atom (one density, tau 1e-12, FP64, built-in
@@ -545,14 +579,15 @@ def measure_highl(steps, warmup):
     dc.jk_device(Dd, J, K, threshold=1e-12, symmetric=True)
     torch.cuda.synchronize(dev)
     recs = {}
-    for r in dc.profile():
+    prof = dc.profile()
+    for r in prof:
         key = "".join("spdfghi"[l] for l in r["cls"])
         recs[key] = recs.get(key, 0.0) + r["ms"]
     dc.set_profiling(False)
     log("high-l: %.1f ms per J/K build" % ms)
     top = sorted(recs.items(), key=lambda kv: -kv[1])[:8]
     return {"ms_per_build": ms, "quartets": dc.last_stats()[0], "nao": 565,
-            "classes_ms": {k: round(v, 3) for k, v in top}}
+            "classes_ms": {k: round(v, 3) for k, v in top}, "roofline": highl_roofline(prof)}
 
 
 def cpu_baseline_grad(shells, D, cj, ck, tau, seconds):
