@@ -234,8 +234,9 @@ __device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const Ry
                           const unsigned long long *__restrict__ eoff, double *__restrict__ blk) {
   const int T = table_size(K.la, K.lb, K.lc, K.ld);
   double *tab = dyn, *work = dyn + 3 * rb * T;
-  for (int e = threadIdx.x; e < K.N4; e += blockDim.x) blk[e] = 0.0;
   const int ob = p.pp_off[pb], nbp = p.pp_n[pb], ok = p.pp_off[pk], nkp = p.pp_n[pk];
+  if (nbp * nkp == 0)  // no primitive quartet: an empty block
+    for (int e = threadIdx.x; e < K.N4; e += blockDim.x) blk[e] = 0.0;
   constexpr unsigned long long m21 = (1ull << 21) - 1;
   for (int ib = 0; ib < nbp; ++ib)
     for (int ik = 0; ik < nkp; ++ik) {
@@ -245,6 +246,10 @@ __device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const Ry
       for (int r0 = 0; r0 < K.NR; r0 += rb) {
         const int nr = K.NR - r0 < rb ? K.NR - r0 : rb;
         tables_rt(K.la, K.lb, K.lc, K.ld, r0, nr, q, g, S, tab, work);
+        // the first batch of the first primitive quartet writes every element
+        // of the slot (no read of it, no separate clearing pass)
+        const bool first = ib == 0 && ik == 0 && r0 == 0;
+#pragma unroll 4
         for (int e = threadIdx.x; e < K.N4; e += blockDim.x) {
           const unsigned long long o = eoff[e];
           const int ox = int(o & m21), oy = int((o >> 21) & m21), oz = int(o >> 42);
@@ -253,7 +258,10 @@ __device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const Ry
             const double *tx = tab + 3 * k * T;
             v += S.rw[K.NR + r0 + k] * tx[ox] * tx[T + oy] * tx[2 * T + oz];
           }
-          blk[e] += q.pref * v;
+          if (first)
+            blk[e] = q.pref * v;
+          else
+            blk[e] += q.pref * v;
         }
         __syncthreads();  // tables are rebuilt for the next batch
       }
