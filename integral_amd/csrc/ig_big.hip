@@ -321,74 +321,105 @@ __global__ void __launch_bounds__(kBigThreads)
         (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (pb == pk ? 0.5 : 1.0);
     const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
               d0 = b.off_cart[g.sD];
-    const int na = K.na, nb = K.nb, nc = K.nc, nd = K.nd;
-    const int NJ = na * nb + nc * nd;
-    const int NO = NJ + ((dd.kc && dd.want_k) ? na * nc + nb * nc + na * nd + nb * nd : 0);
+    const int na = K.na, nb = K.nb, nc = K.nc, nd = K.nd, ncd = nc * nd, nab = na * nb;
+    const int nds = nab + ncd + na * nc + na * nd + nb * nc + nb * nd;
+    const int NJ = nab + ncd;
+    const bool wk = dd.kc && dd.want_k;
+    const int NO = NJ + (wk ? na * nc + nb * nc + na * nd + nb * nd : 0);
+    // the quartet's six density sub-blocks in LDS (the tables' LDS is free:
+    // every lane passed big_block's last barrier)
+    double *Dab = dyn, *Dcd = Dab + nab, *Dac = Dcd + ncd, *Dad = Dac + na * nc,
+           *Dbc = Dad + na * nd, *Dbd = Dbc + nb * nc;
+    const size_t xo = jk_copy_offset(dd);
     for (int e = 0; e < dd.neff; ++e) {
-      if (!((mask >> dd.eff_src[e]) & 1ull)) continue;
-      const bool anti = dd.eff_anti[e];
+      if (!((mask >> dd.eff_src[e]) & 1ull)) continue;  // uniform
+      const bool anti = dd.eff_anti[e];                 // antisymmetric parts feed K only
+      if (anti && !wk) continue;
       const double *D = dd.dc + (size_t)e * n2;
-      const size_t xo = jk_copy_offset(dd);
       double *J = dd.jc + xo + (size_t)e * n2;
-      double *Kp = dd.kc ? dd.kc + xo + (size_t)e * n2 : nullptr;
+      double *Kp = wk ? dd.kc + xo + (size_t)e * n2 : nullptr;
+      __syncthreads();  // the previous density's sums are done with the sub-blocks
+      for (int i = threadIdx.x; i < nds; i += blockDim.x) {
+        int q = i, row, col;
+        if (q < nab) {
+          row = a0 + q / nb;
+          col = b0 + q % nb;
+        } else if ((q -= nab) < ncd) {
+          row = c0 + q / nd;
+          col = d0 + q % nd;
+        } else if ((q -= ncd) < na * nc) {
+          row = a0 + q / nc;
+          col = c0 + q % nc;
+        } else if ((q -= na * nc) < na * nd) {
+          row = a0 + q / nd;
+          col = d0 + q % nd;
+        } else if ((q -= na * nd) < nb * nc) {
+          row = b0 + q / nc;
+          col = c0 + q % nc;
+        } else {
+          q -= nb * nc;
+          row = b0 + q / nd;
+          col = d0 + q % nd;
+        }
+        Dab[i] = D[(size_t)row * ncrt + col];
+      }
+      __syncthreads();
+      // one lane per J'/K' entry: the entry sums n1 x m2 terms
+      // blk[base + i*S1 + j*S2] * Ds[i*m2 + j] (Ds the matching sub-block),
+      // eight terms per step with independent loads and partial sums
       for (int o = threadIdx.x; o < NO; o += blockDim.x) {
-        if (anti && o < NJ) continue;  // antisymmetric parts feed K only
-        double v = 0.0;
-        int row, col;
+        if (anti && o < NJ) continue;
+        int n1, m2, S1, S2, base, row, col;
+        const double *Ds;
+        double w;
         double *dst;
-        if (o < na * nb) {  // J[a,b] += 2f sum_cd g D[c,d]
-          const int a = o / nb, bb = o % nb;
-          for (int c = 0; c < nc; ++c)
-            for (int d = 0; d < nd; ++d)
-              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(c0 + c) * ncrt + d0 + d];
-          v *= 2.0 * f;
-          row = a0 + a;
-          col = b0 + bb;
-          dst = J;
+        if (o < nab) {  // J[a,b] += 2f sum_cd g D[c,d]
+          n1 = nc, m2 = nd, S1 = nd, S2 = 1, base = o * ncd, Ds = Dcd;
+          row = a0 + o / nb, col = b0 + o % nb, w = 2.0 * f, dst = J;
         } else if (o < NJ) {  // J[c,d] += 2f sum_ab g D[a,b]
-          const int q = o - na * nb, c = q / nd, d = q % nd;
-          for (int a = 0; a < na; ++a)
-            for (int bb = 0; bb < nb; ++bb)
-              v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(a0 + a) * ncrt + b0 + bb];
-          v *= 2.0 * f;
-          row = c0 + c;
-          col = d0 + d;
-          dst = J;
+          const int q = o - nab;
+          n1 = na, m2 = nb, S1 = nb * ncd, S2 = ncd, base = q, Ds = Dab;
+          row = c0 + q / nd, col = d0 + q % nd, w = 2.0 * f, dst = J;
         } else {
           int q = o - NJ;
-          if (q < na * nc) {  // Kp[a,c] += f g D[b,d]
+          if (q < na * nc) {  // Kp[a,c] += f sum_bd g D[b,d]
             const int a = q / nc, c = q % nc;
-            for (int bb = 0; bb < nb; ++bb)
-              for (int d = 0; d < nd; ++d)
-                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(b0 + bb) * ncrt + d0 + d];
-            row = a0 + a;
-            col = c0 + c;
-          } else if ((q -= na * nc) < nb * nc) {  // Kp[b,c] += f g D[a,d]
+            n1 = nb, m2 = nd, S1 = ncd, S2 = 1, base = a * nb * ncd + c * nd, Ds = Dbd;
+            row = a0 + a, col = c0 + c;
+          } else if ((q -= na * nc) < nb * nc) {  // Kp[b,c] += f sum_ad g D[a,d]
             const int bb = q / nc, c = q % nc;
-            for (int a = 0; a < na; ++a)
-              for (int d = 0; d < nd; ++d)
-                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(a0 + a) * ncrt + d0 + d];
-            row = b0 + bb;
-            col = c0 + c;
-          } else if ((q -= nb * nc) < na * nd) {  // Kp[a,d] += f g D[b,c]
+            n1 = na, m2 = nd, S1 = nb * ncd, S2 = 1, base = bb * ncd + c * nd, Ds = Dad;
+            row = b0 + bb, col = c0 + c;
+          } else if ((q -= nb * nc) < na * nd) {  // Kp[a,d] += f sum_bc g D[b,c]
             const int a = q / nd, d = q % nd;
-            for (int bb = 0; bb < nb; ++bb)
-              for (int c = 0; c < nc; ++c)
-                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(b0 + bb) * ncrt + c0 + c];
-            row = a0 + a;
-            col = d0 + d;
-          } else {  // Kp[b,d] += f g D[a,c]
+            n1 = nb, m2 = nc, S1 = ncd, S2 = nd, base = a * nb * ncd + d, Ds = Dbc;
+            row = a0 + a, col = d0 + d;
+          } else {  // Kp[b,d] += f sum_ac g D[a,c]
             q -= na * nd;
             const int bb = q / nd, d = q % nd;
-            for (int a = 0; a < na; ++a)
-              for (int c = 0; c < nc; ++c)
-                v += blk[((a * nb + bb) * nc + c) * nd + d] * D[(size_t)(a0 + a) * ncrt + c0 + c];
-            row = b0 + bb;
-            col = d0 + d;
+            n1 = na, m2 = nc, S1 = nb * ncd, S2 = nd, base = bb * ncd + d, Ds = Dac;
+            row = b0 + bb, col = d0 + d;
           }
-          v *= f;
-          dst = Kp;
+          w = f, dst = Kp;
         }
+        const int nt = n1 * m2;
+        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int i = 0, j = 0, t = 0;  // term t = i*m2 + j
+        for (; t + 8 <= nt; t += 8) {
+          double gv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            gv[u] = blk[base + i * S1 + j * S2];
+            if (++j == m2) j = 0, ++i;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc[u] += gv[u] * Ds[t + u];
+        }
+        for (; t < nt; ++t) {
+          acc[0] += blk[base + i * S1 + j * S2] * Ds[t];
+          if (++j == m2) j = 0, ++i;
+        }
+        const double v = w * (((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7])));
         if (v != 0.0) jk_add(dd, dst + (size_t)row * ncrt + col, v);
       }
     }
@@ -398,6 +429,32 @@ __global__ void __launch_bounds__(kBigThreads)
     atomicAdd(counters, my_q);
     atomicAdd(counters + 1, my_p);
   }
+}
+
+// Sum of blk[el(t) - s0] * Ds[t] over terms t = t0, t0 + step, ... < t1, with
+// el(t) = base + (t / m2) * S1 + (t % m2) * S2; terms whose element lies
+// outside the slice [s0, s0 + ns) count zero.  Eight terms per step, the
+// last step predicated: their LDS loads are independent and their products
+// go to separate sums.
+__device__ __forceinline__ double slice_terms(const double *blk, int s0, int ns,
+                                              const double *Ds, int base, int m2, int S1,
+                                              int S2, int t0, int t1, int step) {
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = t0; t < t1; t += 8 * step) {
+    double gv[8], dv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int tt = t + u * step, i = tt / m2;
+      const int el = base + i * S1 + (tt - i * m2) * S2 - s0;
+      const bool ok = tt < t1 && el >= 0 && el < ns;
+      gv[u] = blk[ok ? el : 0];
+      dv[u] = Ds[ok ? tt : 0];
+      gv[u] = ok ? gv[u] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += gv[u] * dv[u];
+  }
+  return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
 // ---------------------------------------------------------------------------
@@ -624,10 +681,10 @@ __global__ void __launch_bounds__(kRtThreads)
     // digest (the tables' LDS is free: every lane passed the last barrier):
     // the slice to LDS once, then per density the six density sub-blocks of
     // the quartet and the J'/K' entries the slice touches.  Entries with many
-    // terms (J[a,b], K[a,c], K[a,d]) take a wavefront each (lanes stride over
-    // the terms, then a wavefront sum); entries with few (J[c,d], K[b,c],
-    // K[b,d]) a lane each.  Every sum runs over the rows (a, b) the slice
-    // holds.
+    // terms (J[a,b], K[a,c], K[a,d]) take a group of lanes each (the lanes
+    // stride over the terms, then an in-group sum); entries with few (J[c,d],
+    // K[b,c], K[b,d]) a lane each.  Every sum runs over the rows (a, b) the
+    // slice holds.
     const double f =
         (g.sA == g.sB ? 0.5 : 1.0) * (g.sC == g.sD ? 0.5 : 1.0) * (pb == pk ? 0.5 : 1.0);
     const int a0 = b.off_cart[g.sA], b0 = b.off_cart[g.sB], c0 = b.off_cart[g.sC],
@@ -643,16 +700,10 @@ __global__ void __launch_bounds__(kRtThreads)
       if (el < s1) blk[el - s0] = acc[k];
     }
     const int r0 = s0 / ncd, r1 = (s1 - 1) / ncd + 1;  // rows (a, b) of the slice
+    const int ns = s1 - s0;
     const int A0 = r0 / nb, A1 = (r1 - 1) / nb + 1, nA = A1 - A0;
     const bool wk = dd.kc && dd.want_k;
     const size_t xo = jk_copy_offset(dd);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // block element (a, bb, c, d) of the slice, or 0 outside it
-#define IG_RT_G(a, bb, c, d)                                           \
-  ([&]() -> double {                                                   \
-    const int el_ = (((a) * nb + (bb)) * nc + (c)) * nd + (d);         \
-    return (el_ >= s0 && el_ < s1) ? blk[el_ - s0] : 0.0;              \
-  }())
     for (int e = 0; e < dd.neff; ++e) {
       if (!((mask >> dd.eff_src[e]) & 1ull)) continue;  // uniform
       const bool anti = dd.eff_anti[e];                 // antisymmetric parts feed K only
@@ -686,56 +737,58 @@ __global__ void __launch_bounds__(kRtThreads)
         Dab[i] = D[(size_t)row * ncrt + col];
       }
       __syncthreads();
-      // wavefront entries: J[a,b] rows, then K[a,c], then K[a,d]
-      const int nw0 = anti ? 0 : r1 - r0, nw1 = wk ? nA * nc : 0, nw2 = wk ? nA * nd : 0;
-      for (int o = wave; o < nw0 + nw1 + nw2; o += kRtThreads / 64) {
-        double v = 0.0;
-        double *dst;
-        int row, col;
-        if (o < nw0) {  // J[a,b] += 2f sum_cd g D[c,d]
-          const int rr = r0 + o;
-          const int lo = s0 > rr * ncd ? s0 - rr * ncd : 0;
-          const int hi = s1 < (rr + 1) * ncd ? s1 - rr * ncd : ncd;
-          for (int cd = lo + lane; cd < hi; cd += 64) v += blk[rr * ncd + cd - s0] * Dcd[cd];
-          v *= 2.0 * f;
-          dst = J;
-          row = a0 + rr / nb;
-          col = b0 + rr % nb;
-        } else {
-          int q = o - nw0;
-          const bool ac = q < nw1;
-          if (!ac) q -= nw1;
-          const int a = A0 + q / (ac ? nc : nd), x = q % (ac ? nc : nd);
-          const int bl = r0 - a * nb > 0 ? r0 - a * nb : 0;
-          const int bh = r1 - a * nb < nb ? r1 - a * nb : nb;
-          const int ny = ac ? nd : nc;  // summed index of the other ket shell
-          for (int t2 = bl * ny + lane; t2 < bh * ny; t2 += 64) {
-            const int bb = t2 / ny, y = t2 % ny;
+      // group entries: J[a,b] rows, then K[a,c], then K[a,d]; each entry
+      // takes a group of G lanes (a power of two, about eight terms per
+      // lane), the group's lanes stride over its terms and sum in-group
+      for (int kind = 0; kind < 3; ++kind) {
+        const int nent = kind == 0 ? (anti ? 0 : r1 - r0) : (wk ? nA * (kind == 1 ? nc : nd) : 0);
+        const int maxt = kind == 0 ? ncd : nb * (kind == 1 ? nd : nc);
+        int G = 1;
+        while (G < 64 && 8 * G < maxt) G <<= 1;
+        const int gl = threadIdx.x & (G - 1);
+        for (int o = threadIdx.x / G; o < nent; o += kRtThreads / G) {
+          double v;
+          double *dst;
+          int row, col;
+          if (kind == 0) {  // J[a,b] += 2f sum_cd g D[c,d]
+            const int rr = r0 + o;
+            const int lo = s0 > rr * ncd ? s0 - rr * ncd : 0;
+            const int hi = s1 < (rr + 1) * ncd ? s1 - rr * ncd : ncd;
+            v = 2.0 * f * slice_terms(blk, s0, ns, Dcd, rr * ncd, 1, 1, 0, lo + gl, hi, G);
+            dst = J;
+            row = a0 + rr / nb;
+            col = b0 + rr % nb;
+          } else {
+            const bool ac = kind == 1;
+            const int a = A0 + o / (ac ? nc : nd), x = o % (ac ? nc : nd);
+            const int bl = r0 - a * nb > 0 ? r0 - a * nb : 0;
+            const int bh = r1 - a * nb < nb ? r1 - a * nb : nb;
+            const int ny = ac ? nd : nc;  // summed index of the other ket shell
             if (ac)  // Kp[a,c] += f sum_bd g D[b,d]
-              v += IG_RT_G(a, bb, x, y) * Dbd[bb * nd + y];
+              v = slice_terms(blk, s0, ns, Dbd, a * nb * ncd + x * nd, ny, ncd, 1, bl * ny + gl,
+                              bh * ny, G);
             else  // Kp[a,d] += f sum_bc g D[b,c]
-              v += IG_RT_G(a, bb, y, x) * Dbc[bb * nc + y];
+              v = slice_terms(blk, s0, ns, Dbc, a * nb * ncd + x, ny, ncd, nd, bl * ny + gl,
+                              bh * ny, G);
+            v *= f;
+            dst = Kp;
+            row = a0 + a;
+            col = (ac ? c0 : d0) + x;
           }
-          v *= f;
-          dst = Kp;
-          row = a0 + a;
-          col = (ac ? c0 : d0) + x;
+          // in-group sum: the xor partners of a lane stay inside its group,
+          // whose lanes share the entry and so the loop trip
+          for (int m = G >> 1; m > 0; m >>= 1) v += __shfl_xor(v, m);
+          if (gl == 0 && v != 0.0) jk_add(dd, dst + (size_t)row * ncrt + col, v);
         }
-        v = wave_sum_d(v);
-        if (lane == 0 && v != 0.0) jk_add(dd, dst + (size_t)row * ncrt + col, v);
       }
       // lane entries: J[c,d], then K[b,c], then K[b,d]
       const int nl0 = anti ? 0 : ncd, nl1 = wk ? nb * nc : 0, nl2 = wk ? nb * nd : 0;
       for (int o = threadIdx.x; o < nl0 + nl1 + nl2; o += kRtThreads) {
-        double v = 0.0;
+        double v;
         double *dst;
         int row, col;
-        if (o < nl0) {  // J[c,d] += 2f sum_ab g D[a,b]
-          for (int rr = r0; rr < r1; ++rr) {
-            const int el = rr * ncd + o;
-            if (el >= s0 && el < s1) v += blk[el - s0] * Dab[rr];
-          }
-          v *= 2.0 * f;
+        if (o < nl0) {  // J[c,d] += 2f sum_ab g D[a,b] over the slice's rows
+          v = 2.0 * f * slice_terms(blk, s0, ns, Dab, o, 1, ncd, 0, r0, r1, 1);
           dst = J;
           row = c0 + o / nd;
           col = d0 + o % nd;
@@ -749,13 +802,12 @@ __global__ void __launch_bounds__(kRtThreads)
           int al = (r0 - bb + nb - 1) / nb;
           if (r0 - bb <= 0) al = 0;
           const int ah = (r1 - 1 - bb) >= 0 ? (r1 - 1 - bb) / nb + 1 : 0;
-          for (int a = al; a < ah; ++a)
-            for (int y = 0; y < ny; ++y) {
-              if (bc)  // Kp[b,c] += f sum_ad g D[a,d]
-                v += IG_RT_G(a, bb, x, y) * Dad[a * nd + y];
-              else  // Kp[b,d] += f sum_ac g D[a,c]
-                v += IG_RT_G(a, bb, y, x) * Dac[a * nc + y];
-            }
+          if (bc)  // Kp[b,c] += f sum_ad g D[a,d]
+            v = slice_terms(blk, s0, ns, Dad, bb * ncd + x * nd, ny, nb * ncd, 1, al * ny,
+                            ah * ny, 1);
+          else  // Kp[b,d] += f sum_ac g D[a,c]
+            v = slice_terms(blk, s0, ns, Dac, bb * ncd + x, ny, nb * ncd, nd, al * ny, ah * ny,
+                            1);
           v *= f;
           dst = Kp;
           row = b0 + bb;
@@ -764,7 +816,6 @@ __global__ void __launch_bounds__(kRtThreads)
         if (v != 0.0) jk_add(dd, dst + (size_t)row * ncrt + col, v);
       }
     }
-#undef IG_RT_G
     __syncthreads();  // the next work item rebuilds the tables in this LDS
   }
   if (counters && threadIdx.x == 0) {
@@ -1150,6 +1201,10 @@ hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, cons
     int rb;
     size_t lds;
     root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
+    // the epilogue stages the quartet's density sub-blocks in the same LDS
+    const size_t nds = size_t(ncart(la)) * ncart(lb) + size_t(ncart(lc)) * ncart(ld) +
+                       size_t(ncart(la) + ncart(lb)) * (ncart(lc) + ncart(ld));
+    if (nds * sizeof(double) > lds) lds = nds * sizeof(double);
     double *slots = big_prepare(la, lb, lc, ld, scratch, s);
     hipLaunchKernelGGL(jk_big_kernel, dim3(big_grid(qs.total, n4)), dim3(kBigThreads), lds, s,
                        la, lb, lc, ld, rb, b, p, r, qs, d, omega,
