@@ -24,6 +24,11 @@ constexpr int kBigThreads = 256;
 constexpr int kBigWaves = kBigThreads / 64;
 constexpr int kBigMaxWG = 1024;
 constexpr size_t kBigLdsBudget = 64 * 1024;         // dynamic LDS per workgroup
+// jk_big_kernel's root batches: up to kBigLdsTwo bytes keep two workgroups on
+// a CU; blocks under kBigOneElems elements take all their roots in one batch
+// of up to kBigLdsOne bytes (one workgroup per CU) when that saves a batch.
+constexpr size_t kBigLdsTwo = 78 * 1024, kBigLdsOne = 150 * 1024;
+constexpr int kBigOneElems = 20480;
 constexpr size_t kBigScratchBytes = size_t(1) << 30;  // all slots of one launch
 
 struct BigStatic {
@@ -475,15 +480,20 @@ __device__ __forceinline__ double slice_terms(const double *blk, int s0, int ns,
 // table.  The digest copies the slice to LDS and sums every J'/K' entry the
 // slice touches against the quartet's six density sub-blocks, staged in LDS,
 // with one device atomic per entry and slice.
-constexpr int kRtThreads = 256;
+// 512 lanes at about 250 VGPRs each fill a CU's register file: one workgroup
+// per CU, which may then take most of the LDS for its root batches (fewer
+// batches, so fewer table builds and barriers per work item).
+constexpr int kRtThreads = 512;
 constexpr int kRtN = 2 * kMaxL + 1;  // vertical-recurrence extent (n, m <= 12)
 constexpr int kRtX = kMaxL + 1;      // transfer extent (received momentum <= 6)
-constexpr size_t kRtLdsBudget = 64 * 1024;  // dynamic LDS of a root batch
+constexpr size_t kRtLdsBudget = 150 * 1024;  // dynamic LDS of a root batch
 constexpr int kRtMaxWG = 4096;
 // Blocks of at least this many elements take jk_rt_kernel, smaller ones
 // jk_big_kernel (measured on a 20-atom s..g + l = 6 basis: the sliced kernel
-// is 1.2-1.8x faster from 16384 elements up, 1.1-2.3x slower below).
-constexpr int kRtMinElems = 16384;
+// is 1.3-3.4x faster from 70000 elements up, the per-quartet one up to 1.3x
+// faster at 63000 and below, where building the tables once per quartet
+// outweighs the global slot).
+constexpr int kRtMinElems = 65536;
 
 // Doubles per (root, axis) of the table steps: vertical-recurrence table,
 // bra-transferred table, final table.
@@ -1140,6 +1150,24 @@ void root_batch(int la, int lb, int lc, int ld, int nr, int &rb, size_t &bytes) 
   bytes = per * rb;
 }
 
+// Roots per table batch and dynamic LDS bytes of jk_big_kernel for class
+// (la..ld) (measured on a 20-atom s..g + l = 6 basis: one batch at one
+// workgroup per CU is up to 1.5x faster for blocks under kBigOneElems
+// elements, two workgroups per CU with more batches up to 1.25x faster above).
+void jk_big_batch(int la, int lb, int lc, int ld, int &rb, size_t &bytes) {
+  const int nr = (la + lb + lc + ld) / 2 + 1;
+  const size_t n4 = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
+  const size_t per =
+      size_t(3) * (table_size(la, lb, lc, ld) + table_work(la, lb, lc, ld)) * sizeof(double);
+  if (per * nr <= kBigLdsTwo || per * nr > kBigLdsOne || n4 >= size_t(kBigOneElems)) {
+    rb = int(kBigLdsTwo / per);
+    rb = rb < 1 ? 1 : (rb > nr ? nr : rb);
+  } else {
+    rb = nr;
+  }
+  bytes = per * rb;
+}
+
 }  // namespace
 
 // Scratch layout of one launch: the element offset table (N4 64-bit words,
@@ -1200,7 +1228,7 @@ hipError_t launch_jk_big(int la, int lb, int lc, int ld, const BasisDev &b, cons
   if (n4 < size_t(kRtMinElems)) {  // one workgroup per quartet, block slots in scratch
     int rb;
     size_t lds;
-    root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
+    jk_big_batch(la, lb, lc, ld, rb, lds);
     // the epilogue stages the quartet's density sub-blocks in the same LDS
     const size_t nds = size_t(ncart(la)) * ncart(lb) + size_t(ncart(lc)) * ncart(ld) +
                        size_t(ncart(la) + ncart(lb)) * (ncart(lc) + ncart(ld));
