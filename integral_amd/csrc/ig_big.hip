@@ -28,7 +28,7 @@ constexpr size_t kBigLdsBudget = 64 * 1024;         // dynamic LDS per workgroup
 // a CU; blocks under kBigOneElems elements take all their roots in one batch
 // of up to kBigLdsOne bytes (one workgroup per CU) when that saves a batch.
 constexpr size_t kBigLdsTwo = 78 * 1024, kBigLdsOne = 150 * 1024;
-constexpr int kBigOneElems = 20480;
+constexpr int kBigOneElems = 20000;
 constexpr size_t kBigScratchBytes = size_t(1) << 30;  // all slots of one launch
 
 struct BigStatic {
@@ -216,6 +216,134 @@ __device__ __forceinline__ void tables_rt(int la, int lb, int lc, int ld, int r0
   __syncthreads();
 }
 
+// 512 lanes at about 250 VGPRs each fill a CU's register file: one workgroup
+// per CU, which may then take most of the LDS for its root batches (fewer
+// batches, so fewer table builds and barriers per work item).
+constexpr int kRtThreads = 512;
+constexpr int kRtN = 2 * kMaxL + 1;  // vertical-recurrence extent (n, m <= 12)
+constexpr int kRtX = kMaxL + 1;      // transfer extent (received momentum <= 6)
+constexpr size_t kRtLdsBudget = 150 * 1024;  // dynamic LDS of a root batch
+constexpr int kRtMaxWG = 4096;
+// Blocks of at least this many elements take jk_rt_kernel, smaller ones
+// jk_big_kernel (measured on a 20-atom s..g + l = 6 basis: the sliced kernel
+// is 1.3-3.4x faster from 70000 elements up, the per-quartet one up to 1.3x
+// faster at 63000 and below, where building the tables once per quartet
+// outweighs the global slot).
+constexpr int kRtMinElems = 65536;
+
+// Doubles per (root, axis) of the table steps: vertical-recurrence table,
+// bra-transferred table, final table.
+struct RtLayout {
+  int G, H, T;
+  __host__ __device__ RtLayout(int la, int lb, int lc, int ld)
+      : G((la + lb + 1) * (lc + ld + 1)), H((la + 1) * (lb + 1) * (lc + ld + 1)),
+        T((la + 1) * (lb + 1) * (lc + 1) * (ld + 1)) {}
+  __host__ __device__ int per_root() const { return 3 * (G + H + T); }
+};
+
+// Vertical recurrence of one (root, axis) into gout[n * (NK + 1) + m],
+// n <= NB, m <= NK (the recurrences of table_rt, column by column).
+__device__ __forceinline__ void vrr_rt(int NB, int NK, double c00, double cp0, double b10,
+                                       double b01, double b00, double *gout) {
+  const int W = NK + 1;
+  double prev[kRtN], cur[kRtN];
+#pragma unroll
+  for (int n = 0; n < kRtN; ++n) {
+    prev[n] = 0.0;
+    cur[n] = 0.0;
+  }
+  cur[0] = 1.0;
+  if (NB >= 1) cur[1] = c00;
+#pragma unroll
+  for (int n = 1; n + 1 < kRtN; ++n)
+    if (n < NB) cur[n + 1] = c00 * cur[n] + double(n) * b10 * cur[n - 1];
+#pragma unroll
+  for (int n = 0; n < kRtN; ++n)
+    if (n <= NB) gout[n * W] = cur[n];
+  for (int m = 0; m < NK; ++m) {
+    double nxt[kRtN];
+#pragma unroll
+    for (int n = 0; n < kRtN; ++n) {
+      double v = cp0 * cur[n] + double(m) * b01 * prev[n];
+      if (n >= 1) v += double(n) * b00 * cur[n - 1];
+      nxt[n] = v;
+    }
+#pragma unroll
+    for (int n = 0; n < kRtN; ++n) {
+      if (n <= NB) gout[n * W + m + 1] = nxt[n];
+      prev[n] = cur[n];
+      cur[n] = nxt[n];
+    }
+  }
+}
+
+// Tables of roots [r0, r0+nr) of primitive quartet q for class (la..ld) in
+// two steps over the workgroup: the vertical recurrence of each (root, axis)
+// into gt (vrr_rt, columns in VGPRs), then one lane per (root, axis, u, v)
+// for built / received bra momenta (u, v): the bra transfer of the rows
+// g(u..u+v, m) and the ket transfer of the result, both in VGPRs, in the
+// operation order of table_rt.  Each pair is built on the centre of its
+// tighter primitive and transferred to the other.  Table of (root r, axis)
+// at out + (3*(r-r0)+axis)*T in the standard layout, without weight or
+// prefactor; gt holds 3*nr*G doubles.  Ends with a barrier.
+__device__ __forceinline__ void tables_fused(int la, int lb, int lc, int ld, const RtLayout &L,
+                                             int r0, int nr, const PrimQ &q, const BigGeom &g,
+                                             const BigStatic &S, double *out, double *gt) {
+  const int NB = la + lb, NK = lc + ld, W = NK + 1;
+  // uniform over the workgroup: one primitive quartet at a time
+  const bool swap_b = q.be > q.al, swap_k = q.de > q.ga;
+  const int ea = swap_b ? lb : la, eb = swap_b ? la : lb;  // built / received
+  const int ec = swap_k ? ld : lc, ed = swap_k ? lc : ld;
+  for (int task = threadIdx.x; task < 3 * nr; task += blockDim.x) {
+    const int r = r0 + task / 3, ax = task % 3;
+    const double x = S.rw[r];
+    const double fq = x * q.qz * q.ipq, fp = x * q.pz * q.ipq;
+    const double b00 = 0.5 * x * q.ipq;
+    const double b10 = (1.0 - fq) * q.i2p;
+    const double b01 = (1.0 - fp) * q.i2q;
+    const double c00 = (q.P[ax] - (swap_b ? g.A[ax] - g.AB[ax] : g.A[ax])) - fq * q.PQ[ax];
+    const double cp0 = (q.Q[ax] - (swap_k ? g.C[ax] - g.CD[ax] : g.C[ax])) + fp * q.PQ[ax];
+    vrr_rt(NB, NK, c00, cp0, b10, b01, b00, gt + task * L.G);
+  }
+  __syncthreads();
+  const int nuv = (ea + 1) * (eb + 1);
+  for (int idx = threadIdx.x; idx < 3 * nr * nuv; idx += blockDim.x) {
+    const int uv = idx % nuv, task = idx / nuv, u = uv / (eb + 1), v = uv % (eb + 1);
+    const int ax = task % 3;
+    const double sab = swap_b ? -g.AB[ax] : g.AB[ax];
+    const double scd = swap_k ? -g.CD[ax] : g.CD[ax];
+    const double *gg = gt + task * L.G + u * W;
+    // bra transfer, column by column: h(u, v; m) from g(u..u+v, m)
+    double t[kRtN];
+#pragma unroll
+    for (int m = 0; m < kRtN; ++m) {
+      double vec[kRtX];
+#pragma unroll
+      for (int i = 0; i < kRtX; ++i) vec[i] = (m <= NK && i <= v) ? gg[i * W + m] : 0.0;
+#pragma unroll
+      for (int st = 1; st < kRtX; ++st)
+        if (st <= v)
+#pragma unroll
+          for (int i = 0; i + 1 < kRtX; ++i) vec[i] = vec[i + 1] + sab * vec[i];
+      t[m] = vec[0];
+    }
+    // ket transfer in place over t, the outputs after each step
+    const int ia = swap_b ? v : u, ib = swap_b ? u : v;
+    double *oo = out + task * L.T + (ia * (lb + 1) + ib) * (lc + 1) * (ld + 1);
+#pragma unroll
+    for (int l = 0; l < kRtX; ++l) {
+      if (l > ed) break;
+      if (l > 0)
+#pragma unroll
+        for (int k = 0; k + 1 < kRtN; ++k) t[k] = t[k + 1] + scd * t[k];
+#pragma unroll
+      for (int k = 0; k < kRtX; ++k)
+        if (k <= ec) oo[(swap_k ? l : k) * (ld + 1) + (swap_k ? k : l)] = t[k];
+    }
+  }
+  __syncthreads();
+}
+
 // Contracted Cartesian block of quartet (pb | pk) into blk (N4 doubles).
 // Table offsets of every block element of class (la..ld), packed 21 bits
 // per axis (x | y << 21 | z << 42): one table for the whole launch, read with
@@ -237,8 +365,9 @@ __global__ void big_offsets_kernel(int la, int lb, int lc, int ld,
 __device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const RysDev &R, int pb,
                           int pk, const BigGeom &g, double omega, BigStatic &S, double *dyn,
                           const unsigned long long *__restrict__ eoff, double *__restrict__ blk) {
-  const int T = table_size(K.la, K.lb, K.lc, K.ld);
-  double *tab = dyn, *work = dyn + 3 * rb * T;
+  const RtLayout L(K.la, K.lb, K.lc, K.ld);
+  const int T = L.T;
+  double *tab = dyn, *gt = dyn + 3 * rb * T;
   const int ob = p.pp_off[pb], nbp = p.pp_n[pb], ok = p.pp_off[pk], nkp = p.pp_n[pk];
   if (nbp * nkp == 0)  // no primitive quartet: an empty block
     for (int e = threadIdx.x; e < K.N4; e += blockDim.x) blk[e] = 0.0;
@@ -250,7 +379,7 @@ __device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const Ry
       roots_rt(R, K.NR, q, S);
       for (int r0 = 0; r0 < K.NR; r0 += rb) {
         const int nr = K.NR - r0 < rb ? K.NR - r0 : rb;
-        tables_rt(K.la, K.lb, K.lc, K.ld, r0, nr, q, g, S, tab, work);
+        tables_fused(K.la, K.lb, K.lc, K.ld, L, r0, nr, q, g, S, tab, gt);
         // the first batch of the first primitive quartet writes every element
         // of the slot (no read of it, no separate clearing pass)
         const bool first = ib == 0 && ik == 0 && r0 == 0;
@@ -480,67 +609,6 @@ __device__ __forceinline__ double slice_terms(const double *blk, int s0, int ns,
 // table.  The digest copies the slice to LDS and sums every J'/K' entry the
 // slice touches against the quartet's six density sub-blocks, staged in LDS,
 // with one device atomic per entry and slice.
-// 512 lanes at about 250 VGPRs each fill a CU's register file: one workgroup
-// per CU, which may then take most of the LDS for its root batches (fewer
-// batches, so fewer table builds and barriers per work item).
-constexpr int kRtThreads = 512;
-constexpr int kRtN = 2 * kMaxL + 1;  // vertical-recurrence extent (n, m <= 12)
-constexpr int kRtX = kMaxL + 1;      // transfer extent (received momentum <= 6)
-constexpr size_t kRtLdsBudget = 150 * 1024;  // dynamic LDS of a root batch
-constexpr int kRtMaxWG = 4096;
-// Blocks of at least this many elements take jk_rt_kernel, smaller ones
-// jk_big_kernel (measured on a 20-atom s..g + l = 6 basis: the sliced kernel
-// is 1.3-3.4x faster from 70000 elements up, the per-quartet one up to 1.3x
-// faster at 63000 and below, where building the tables once per quartet
-// outweighs the global slot).
-constexpr int kRtMinElems = 65536;
-
-// Doubles per (root, axis) of the table steps: vertical-recurrence table,
-// bra-transferred table, final table.
-struct RtLayout {
-  int G, H, T;
-  __host__ __device__ RtLayout(int la, int lb, int lc, int ld)
-      : G((la + lb + 1) * (lc + ld + 1)), H((la + 1) * (lb + 1) * (lc + ld + 1)),
-        T((la + 1) * (lb + 1) * (lc + 1) * (ld + 1)) {}
-  __host__ __device__ int per_root() const { return 3 * (G + H + T); }
-};
-
-// Vertical recurrence of one (root, axis) into gout[n * (NK + 1) + m],
-// n <= NB, m <= NK (the recurrences of table_rt, column by column).
-__device__ __forceinline__ void vrr_rt(int NB, int NK, double c00, double cp0, double b10,
-                                       double b01, double b00, double *gout) {
-  const int W = NK + 1;
-  double prev[kRtN], cur[kRtN];
-#pragma unroll
-  for (int n = 0; n < kRtN; ++n) {
-    prev[n] = 0.0;
-    cur[n] = 0.0;
-  }
-  cur[0] = 1.0;
-  if (NB >= 1) cur[1] = c00;
-#pragma unroll
-  for (int n = 1; n + 1 < kRtN; ++n)
-    if (n < NB) cur[n + 1] = c00 * cur[n] + double(n) * b10 * cur[n - 1];
-#pragma unroll
-  for (int n = 0; n < kRtN; ++n)
-    if (n <= NB) gout[n * W] = cur[n];
-  for (int m = 0; m < NK; ++m) {
-    double nxt[kRtN];
-#pragma unroll
-    for (int n = 0; n < kRtN; ++n) {
-      double v = cp0 * cur[n] + double(m) * b01 * prev[n];
-      if (n >= 1) v += double(n) * b00 * cur[n - 1];
-      nxt[n] = v;
-    }
-#pragma unroll
-    for (int n = 0; n < kRtN; ++n) {
-      if (n <= NB) gout[n * W + m + 1] = nxt[n];
-      prev[n] = cur[n];
-      cur[n] = nxt[n];
-    }
-  }
-}
-
 template <int EPL>
 __global__ void __launch_bounds__(kRtThreads)
     jk_rt_kernel(int la, int lb, int lc, int ld, int rb, int nsl, BasisDev b,
@@ -1150,6 +1218,23 @@ void root_batch(int la, int lb, int lc, int ld, int nr, int &rb, size_t &bytes) 
   bytes = per * rb;
 }
 
+// LDS bytes per root of big_block's tables for class (la..ld): the x, y, z
+// tables and their vertical-recurrence tables.
+size_t block_root_bytes(int la, int lb, int lc, int ld) {
+  const RtLayout L(la, lb, lc, ld);
+  return size_t(3) * (L.T + L.G) * sizeof(double);
+}
+
+// Roots per table batch and the dynamic LDS bytes of big_block for class
+// (la..ld) in a budget: as many roots as it holds, at least one.
+void block_batch(int la, int lb, int lc, int ld, int &rb, size_t &bytes) {
+  const int nr = (la + lb + lc + ld) / 2 + 1;
+  const size_t per = block_root_bytes(la, lb, lc, ld);
+  rb = int(kBigLdsBudget / per);
+  rb = rb < 1 ? 1 : (rb > nr ? nr : rb);
+  bytes = per * rb;
+}
+
 // Roots per table batch and dynamic LDS bytes of jk_big_kernel for class
 // (la..ld) (measured on a 20-atom s..g + l = 6 basis: one batch at one
 // workgroup per CU is up to 1.5x faster for blocks under kBigOneElems
@@ -1157,8 +1242,7 @@ void root_batch(int la, int lb, int lc, int ld, int nr, int &rb, size_t &bytes) 
 void jk_big_batch(int la, int lb, int lc, int ld, int &rb, size_t &bytes) {
   const int nr = (la + lb + lc + ld) / 2 + 1;
   const size_t n4 = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
-  const size_t per =
-      size_t(3) * (table_size(la, lb, lc, ld) + table_work(la, lb, lc, ld)) * sizeof(double);
+  const size_t per = block_root_bytes(la, lb, lc, ld);
   if (per * nr <= kBigLdsTwo || per * nr > kBigLdsOne || n4 >= size_t(kBigOneElems)) {
     rb = int(kBigLdsTwo / per);
     rb = rb < 1 ? 1 : (rb > nr ? nr : rb);
@@ -1256,7 +1340,7 @@ hipError_t launch_eri_big(int la, int lb, int lc, int ld, const BasisDev &b, con
   const size_t slot = 2 * size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
   int rb;
   size_t lds;
-  root_batch(la, lb, lc, ld, (la + lb + lc + ld) / 2 + 1, rb, lds);
+  block_batch(la, lb, lc, ld, rb, lds);
   double *slots = big_prepare(la, lb, lc, ld, scratch, s);
   hipLaunchKernelGGL(eri_big_kernel, dim3(big_grid(qs.total, slot)), dim3(kBigThreads), lds, s, la,
                      lb, lc, ld, rb, b, p, r, qs, o,
@@ -1271,7 +1355,7 @@ hipError_t launch_schwarz_big(int la, int lb, const BasisDev &b, const PairsDev 
   const size_t slot = size_t(ncart(la)) * ncart(lb) * ncart(la) * ncart(lb);
   int rb;
   size_t lds;
-  root_batch(la, lb, la, lb, la + lb + 1, rb, lds);
+  block_batch(la, lb, la, lb, rb, lds);
   double *slots = big_prepare(la, lb, la, lb, scratch, s);
   hipLaunchKernelGGL(schwarz_big_kernel, dim3(big_grid(count, slot)), dim3(kBigThreads), lds, s,
                      la, lb, rb, b, p, r, first, count, q,
