@@ -25,10 +25,9 @@ constexpr int kBigWaves = kBigThreads / 64;
 constexpr int kBigMaxWG = 1024;
 constexpr size_t kBigLdsBudget = 64 * 1024;         // dynamic LDS per workgroup
 // jk_big_kernel's root batches: up to kBigLdsTwo bytes keep two workgroups on
-// a CU; blocks under kBigOneElems elements take all their roots in one batch
-// of up to kBigLdsOne bytes (one workgroup per CU) when that saves a batch.
-constexpr size_t kBigLdsTwo = 78 * 1024, kBigLdsOne = 150 * 1024;
-constexpr int kBigOneElems = 20000;
+// a CU (measured on a 20-atom s..g + l = 6 basis: as fast as or faster than
+// fewer, larger batches at one workgroup per CU for every class).
+constexpr size_t kBigLdsTwo = 78 * 1024;
 constexpr size_t kBigScratchBytes = size_t(1) << 30;  // all slots of one launch
 
 struct BigStatic {
@@ -1236,19 +1235,12 @@ void block_batch(int la, int lb, int lc, int ld, int &rb, size_t &bytes) {
 }
 
 // Roots per table batch and dynamic LDS bytes of jk_big_kernel for class
-// (la..ld) (measured on a 20-atom s..g + l = 6 basis: one batch at one
-// workgroup per CU is up to 1.5x faster for blocks under kBigOneElems
-// elements, two workgroups per CU with more batches up to 1.25x faster above).
+// (la..ld): as many roots as kBigLdsTwo holds, at least one.
 void jk_big_batch(int la, int lb, int lc, int ld, int &rb, size_t &bytes) {
   const int nr = (la + lb + lc + ld) / 2 + 1;
-  const size_t n4 = size_t(ncart(la)) * ncart(lb) * ncart(lc) * ncart(ld);
   const size_t per = block_root_bytes(la, lb, lc, ld);
-  if (per * nr <= kBigLdsTwo || per * nr > kBigLdsOne || n4 >= size_t(kBigOneElems)) {
-    rb = int(kBigLdsTwo / per);
-    rb = rb < 1 ? 1 : (rb > nr ? nr : rb);
-  } else {
-    rb = nr;
-  }
+  rb = int(kBigLdsTwo / per);
+  rb = rb < 1 ? 1 : (rb > nr ? nr : rb);
   bytes = per * rb;
 }
 
