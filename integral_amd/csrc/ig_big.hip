@@ -382,19 +382,45 @@ __device__ void big_block(const BigClass &K, int rb, const PairsDev &p, const Ry
         // the first batch of the first primitive quartet writes every element
         // of the slot (no read of it, no separate clearing pass)
         const bool first = ib == 0 && ik == 0 && r0 == 0;
-#pragma unroll 4
-        for (int e = threadIdx.x; e < K.N4; e += blockDim.x) {
-          const unsigned long long o = eoff[e];
-          const int ox = int(o & m21), oy = int((o >> 21) & m21), oz = int(o >> 42);
-          double v = 0.0;
+        // eight elements per lane and pass: their offset loads, table reads
+        // and slot updates are independent and issue together
+        for (int e0 = threadIdx.x; e0 < K.N4; e0 += 8 * kBigThreads) {
+          int ox[8], oy[8], oz[8];
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * kBigThreads;
+            const unsigned long long o = e < K.N4 ? eoff[e] : 0ull;  // 0: in-range dummy
+            ox[u] = int(o & m21);
+            oy[u] = int((o >> 21) & m21);
+            oz[u] = int(o >> 42);
+            v[u] = 0.0;
+          }
           for (int k = 0; k < nr; ++k) {
             const double *tx = tab + 3 * k * T;
-            v += S.rw[K.NR + r0 + k] * tx[ox] * tx[T + oy] * tx[2 * T + oz];
+            const double w = S.rw[K.NR + r0 + k];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] += w * tx[ox[u]] * tx[T + oy[u]] * tx[2 * T + oz[u]];
           }
-          if (first)
-            blk[e] = q.pref * v;
-          else
-            blk[e] += q.pref * v;
+          if (first) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int e = e0 + u * kBigThreads;
+              if (e < K.N4) blk[e] = q.pref * v[u];
+            }
+          } else {
+            double old[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int e = e0 + u * kBigThreads;
+              old[u] = e < K.N4 ? blk[e] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const int e = e0 + u * kBigThreads;
+              if (e < K.N4) blk[e] = old[u] + q.pref * v[u];
+            }
+          }
         }
         __syncthreads();  // tables are rebuilt for the next batch
       }
@@ -499,7 +525,8 @@ __global__ void __launch_bounds__(kBigThreads)
       __syncthreads();
       // one lane per J'/K' entry: the entry sums n1 x m2 terms
       // blk[base + i*S1 + j*S2] * Ds[i*m2 + j] (Ds the matching sub-block),
-      // eight terms per step with independent loads and partial sums
+      // sixteen (then eight) terms per step with independent loads and
+      // eight partial sums
       for (int o = threadIdx.x; o < NO; o += blockDim.x) {
         if (anti && o < NJ) continue;
         int n1, m2, S1, S2, base, row, col;
@@ -538,6 +565,16 @@ __global__ void __launch_bounds__(kBigThreads)
         const int nt = n1 * m2;
         double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         int i = 0, j = 0, t = 0;  // term t = i*m2 + j
+        for (; t + 16 <= nt; t += 16) {
+          double gv[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
+            gv[u] = blk[base + i * S1 + j * S2];
+            if (++j == m2) j = 0, ++i;
+          }
+#pragma unroll
+          for (int u = 0; u < 16; ++u) acc[u & 7] += gv[u] * Ds[t + u];
+        }
         for (; t + 8 <= nt; t += 8) {
           double gv[8];
 #pragma unroll
