@@ -225,10 +225,10 @@ constexpr size_t kRtLdsBudget = 150 * 1024;  // dynamic LDS of a root batch
 constexpr int kRtMaxWG = 4096;
 // Blocks of at least this many elements take jk_rt_kernel, smaller ones
 // jk_big_kernel (measured on a 20-atom s..g + l = 6 basis: the sliced kernel
-// is 1.3-3.4x faster from 70000 elements up, the per-quartet one up to 1.3x
-// faster at 63000 and below, where building the tables once per quartet
-// outweighs the global slot).
-constexpr int kRtMinElems = 65536;
+// is 1.3-2.4x faster for (ii|ig) and (ii|ii), 329280 and 614656 elements;
+// the per-quartet one 1.4-1.8x faster from 70000 to 219520 elements, where
+// building the tables once per quartet outweighs the global slot).
+constexpr int kRtMinElems = 262144;
 
 // Doubles per (root, axis) of the table steps: vertical-recurrence table,
 // bra-transferred table, final table.
