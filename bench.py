@@ -538,10 +538,12 @@ def highl_roofline(recs, ndens=1):
     n4 = 1
     for l in dom["cls"]:
         n4 *= (l + 1) * (l + 2) // 2
-    if max(dom["cls"]) <= 2:
+    # the library's split: l <= 3 compiled; runtime l on the sliced kernel
+    # from 262144 block elements, on the per-quartet kernel below
+    if max(dom["cls"]) <= 3:
         kern = "compiled lane/team kernel"
-    elif n4 >= 16384:
-        kern = "jk_rt_kernel<16> (runtime l, slices of 4096 elements)"
+    elif n4 >= 262144:
+        kern = "jk_rt_kernel<16> (runtime l, slices of 8192 elements)"
     else:
         kern = "jk_big_kernel (runtime l, one quartet per workgroup)"
     f_dom = class_flops(dom, ndens=ndens)
