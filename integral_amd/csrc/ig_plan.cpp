@@ -923,8 +923,12 @@ int jk_variant(const Plan &P, const ClassRange &X, const ClassRange &Y) {
 }
 
 // fp32: the set's quartets run with FP32 tables and block (lane kernels only;
-// a class scheduled on the team kernel takes the register kernel, and the
-// classes without lane kernels stay FP64).
+// a class scheduled on the team kernel takes the register kernel, one on the
+// OS/HGP block the register row-sum kernel, and the classes without lane
+// kernels stay FP64).  The FP32 sets stay off the OS/HGP block: a mixed build
+// of the nao 520 s,p,d workload at cut 1e-6 took an illegal memory access
+// after (ss|ss), (ps|ss) and (pp|ps) moved to that block, and its cause is
+// not found; every earlier mixed run at that size had them on Rys kernels.
 hipError_t jk_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const QuartetSet &qs,
                      const DensDev &dd, unsigned long long *ctr, hipStream_t s, int variant = -1,
                      bool fp32 = false) {
@@ -934,6 +938,7 @@ hipError_t jk_launch(Plan &P, const ClassRange &X, const ClassRange &Y, const Qu
     int v = variant >= 0 ? variant : jk_variant(P, X, Y);
     const bool team = v == kJkTeam || v == kJkTeamHrr;
     if (fp32 && team && lane) v = kJkReg;
+    if (fp32 && v == kJkHgp) v = kJkRowAcc;
     if (!jk_variant_fits(v, X.l1, X.l2, Y.l1, Y.l2, P.bdev, dd.neff, qs.list != nullptr))
       v = lane ? kJkReg : (jk_team_compiled(X.l1, X.l2, Y.l1, Y.l2) ? kJkTeam : kJkTeamHrr);
     if (fp32 && v != kJkTeam && v != kJkTeamHrr)
@@ -2734,7 +2739,8 @@ void batch_prepare(ig_batch &B, double tau, double cut, bool want_k, BatchPrep &
     BatchLaunch L{k, false, -1, 0, 0, sets.size(), seg.size()};
     if (jk_lane_compiled(X0.l1, X0.l2, Y0.l1, Y0.l2)) {
       int v = jk_variant(P0, X0, Y0);
-      if ((v == kJkTeam || v == kJkTeamHrr) && k.fp32) v = kJkRowAcc;
+      // FP32 sets take the register row-sum kernel, as in jk_launch
+      if ((v == kJkTeam || v == kJkTeamHrr || v == kJkHgp) && k.fp32) v = kJkRowAcc;
       if (v == kJkRowAcc || v == kJkHgp) {
         bool fits = true;
         for (auto &mc : kv.second) {
