@@ -1,0 +1,485 @@
+// hip_replay.cpp -- host stand-in for the HIP runtime, for a CPU build of the
+// library (its host code, compiled with --offload-host-only) that checks
+// the device work the host code would launch.
+//
+// Device memory is host memory from an allocation table; streams, events
+// and graphs do nothing; copies are memcpy.  Kernel launches arrive by name
+// (the launch stubs register their kernels' names at load time):
+//   * the planning kernels whose results the host reads back run here:
+//     Schwarz bounds (taken from a matrix the caller gives), the XCD probe,
+//     the density symmetry flag, the fill kernel;
+//   * every J/K launch of the compiled kernels (register, row-sum, team,
+//     team-HRR; FP64 and FP32; Rys and OS/HGP) is checked: each pointer of
+//     its argument views must lie in a live allocation of the size the views
+//     imply, the launch's dynamic LDS must hold what the row-sum kernel
+//     stages, and every quartet of its set, decoded as the kernels' row
+//     cursor does, must name pairs, primitives, shells, Cartesian functions
+//     and row-sum columns inside their tables;
+//   * other kernels do nothing.
+// A violation is recorded (the first few hundred, with the launch and the
+// quartet) for the caller: replay_violations().
+#include <cxxabi.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "ig_internal.h"
+
+using namespace ig;
+
+namespace {
+
+struct Alloc {
+  size_t bytes;
+};
+struct Config {
+  dim3 grid, block;
+  size_t shm = 0;
+  hipStream_t stream = nullptr;
+};
+// The tables are built on first use: the other objects' static initialisers
+// register their kernels and device variables before this file's globals
+// would be constructed.
+struct State {
+  std::map<uintptr_t, Alloc> alloc;
+  std::recursive_mutex mu;
+  std::vector<std::string> viol;
+  std::unordered_map<const void *, std::string> kname;
+  std::vector<double> schwarz;  // nsh x nsh
+  int nsh = 0;
+  long long jk_launches = 0, quartets = 0, other = 0;
+  std::vector<Config> cfg;
+};
+State &st() {
+  static State s;
+  return s;
+}
+#define g_alloc (st().alloc)
+#define g_mu (st().mu)
+#define g_viol (st().viol)
+#define g_kname (st().kname)
+#define g_schwarz (st().schwarz)
+#define g_nsh (st().nsh)
+#define g_jk_launches (st().jk_launches)
+#define g_quartets (st().quartets)
+#define g_other (st().other)
+#define g_cfg (st().cfg)
+
+void violation(const std::string &s) {
+  if (g_viol.size() < 400) g_viol.push_back(s);
+}
+
+// [p, p + bytes) inside one live allocation
+bool inside(const void *p, size_t bytes) {
+  if (!p) return bytes == 0;
+  const uintptr_t q = reinterpret_cast<uintptr_t>(p);
+  auto it = g_alloc.upper_bound(q);
+  if (it == g_alloc.begin()) return false;
+  --it;
+  return q >= it->first && q + bytes <= it->first + it->second.bytes;
+}
+
+void need(const std::string &launch, const char *what, const void *p, size_t bytes) {
+  if (bytes && !inside(p, bytes)) {
+    std::ostringstream s;
+    s << launch << ": " << what << " (" << bytes << " bytes at " << p
+      << ") outside every live allocation";
+    violation(s.str());
+  }
+}
+
+std::string demangle(const std::string &m) {
+  int st = 0;
+  char *d = abi::__cxa_demangle(m.c_str(), nullptr, nullptr, &st);
+  std::string r = (st == 0 && d) ? d : m;
+  std::free(d);
+  return r;
+}
+
+// integers of the first "Cls<a, b, c, d>" and the template arguments after it
+bool class_of(const std::string &n, int cls[4], std::vector<std::string> &rest) {
+  const size_t i = n.find("Cls<");
+  if (i == std::string::npos) return false;
+  if (std::sscanf(n.c_str() + i, "Cls<%d, %d, %d, %d>", &cls[0], &cls[1], &cls[2], &cls[3]) != 4)
+    return false;
+  size_t j = n.find('>', i);  // end of Cls<...>
+  size_t k = n.find('(', j);  // start of the parameter list
+  std::string t = n.substr(j + 1, k - j - 1);
+  rest.clear();
+  std::string cur;
+  for (char c : t) {
+    if (c == ',' || c == '>') {
+      if (!cur.empty()) rest.push_back(cur);
+      cur.clear();
+    } else if (c != ' ') {
+      cur.push_back(c);
+    }
+  }
+  if (!cur.empty()) rest.push_back(cur);
+  return true;
+}
+
+int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
+
+// The J/K launch of a compiled kernel: views, LDS and every quartet.
+void check_jk(const std::string &name, void **args, const Config &c) {
+  ++g_jk_launches;
+  const BasisDev &b = *static_cast<const BasisDev *>(args[0]);
+  const PairsDev &p = *static_cast<const PairsDev *>(args[1]);
+  const RysDev &R = *static_cast<const RysDev *>(args[2]);
+  const QuartetSet &qs = *static_cast<const QuartetSet *>(args[3]);
+  const DensDev &dd = *static_cast<const DensDev *>(args[4]);
+  int cls[4];
+  std::vector<std::string> rest;
+  if (!class_of(name, cls, rest)) {
+    violation("unparsed J/K kernel name " + name);
+    return;
+  }
+  const int la = cls[0], lb = cls[1], lc = cls[2], ld = cls[3];
+  const int na = ncart_h(la), nb = ncart_h(lb), nc = ncart_h(lc), nd = ncart_h(ld);
+  std::ostringstream tag;
+  tag << name.substr(0, name.find('(')) << " grid " << c.grid.x << " lds " << c.shm;
+  const std::string L = tag.str();
+  const size_t nsh = size_t(b.nsh), ncrt = size_t(b.nao_cart), n2 = ncrt * ncrt;
+  // argument views
+  need(L, "shell centres", b.xyz, nsh * 3 * sizeof(double));
+  need(L, "shell momenta", b.l, nsh * sizeof(int));
+  need(L, "shell kinds", b.kind, nsh * sizeof(int));
+  need(L, "function offsets", b.off_func, nsh * sizeof(int));
+  need(L, "Cartesian offsets", b.off_cart, nsh * sizeof(int));
+  need(L, "shell atoms", b.atom, nsh * sizeof(int));
+  need(L, "shell ranks", b.rank, nsh * sizeof(int));
+  need(L, "shells by momentum", b.lshells, nsh * sizeof(int));
+  need(L, "pair shells 1", p.sh1, size_t(p.n) * sizeof(int));
+  need(L, "pair shells 2", p.sh2, size_t(p.n) * sizeof(int));
+  need(L, "pair bounds", p.q, size_t(p.n) * sizeof(double));
+  need(L, "primitive offsets", p.pp_off, size_t(p.n) * sizeof(int));
+  need(L, "primitive counts", p.pp_n, size_t(p.n) * sizeof(int));
+  need(L, "Boys table", R.boys, size_t(kBoysRows) * kBoysCols * sizeof(double));
+  if (!inside(b.off_cart, nsh * sizeof(int)) || !inside(p.pp_off, size_t(p.n) * sizeof(int)) ||
+      !inside(p.pp_n, size_t(p.n) * sizeof(int)) || !inside(p.sh1, size_t(p.n) * sizeof(int)) ||
+      !inside(p.sh2, size_t(p.n) * sizeof(int)) || !inside(b.l, nsh * sizeof(int)) ||
+      !inside(b.rank, nsh * sizeof(int)) || !inside(b.lshells, nsh * sizeof(int)))
+    return;  // reported; the tables below cannot be read
+  size_t nprim = 0;
+  for (int k = 0; k < p.n; ++k)
+    nprim = std::max(nprim, size_t(std::max(0, p.pp_off[k])) + size_t(std::max(0, p.pp_n[k])));
+  for (const double *a : {p.zeta, p.px, p.py, p.pz, p.kc, p.alpha, p.beta, p.kz, p.i2z})
+    need(L, "primitive-pair table", a, nprim * sizeof(double));
+  need(L, "Cartesian densities", dd.dc, size_t(dd.neff) * n2 * sizeof(double));
+  need(L, "J accumulators", dd.jc, dd.copy_stride * size_t(dd.ncopy) * sizeof(double));
+  if (dd.kc) need(L, "K accumulators", dd.kc, dd.copy_stride * size_t(dd.ncopy) * sizeof(double));
+  if (dd.copy_stride < size_t(dd.neff) * n2) violation(L + ": accumulator copy stride too small");
+  need(L, "effective-density table", dd.eff_src, 2 * size_t(dd.neff) * sizeof(int));
+  need(L, "density bounds", dd.bound, size_t(dd.ndens) * nsh * nsh * sizeof(double));
+  if (qs.total <= 0) return;
+  if (qs.x_off < 0 || qs.x_n < 1 || qs.x_off + qs.x_n > p.n || qs.y_off < 0 || qs.y_n < 1 ||
+      qs.y_off + qs.y_n > p.n) {
+    violation(L + ": class ranges outside the pair table");
+    return;
+  }
+  // row-sum kernels: LDS of the staged Rys tables and the row sums
+  const bool rowacc = name.find("jk_rowacc_kernel") != std::string::npos;
+  int W = 0, dbase = 0;
+  if (rowacc) {
+    const int nC = b.lshell_off[lc + 1] - b.lshell_off[lc];
+    const int nD = b.lshell_off[ld + 1] - b.lshell_off[ld];
+    W = nC * nc + (lc == ld ? 0 : nD * nd);
+    dbase = lc == ld ? 0 : nC * nc;
+    const int eng = rest.size() >= 4 ? std::atoi(rest[3].c_str()) : 0;
+    const int nr = (la + lb + lc + ld) / 2 + 1;
+    const size_t nrys = eng == 0 ? size_t(2) * R.nsub[nr] * nr * kRysTerms : 0;
+    const size_t nacc = size_t(dd.neff) * (size_t(na) * nb + size_t(na + nb) * W);
+    if ((nrys + nacc) * sizeof(double) > c.shm) {
+      std::ostringstream s;
+      s << L << ": row-sum kernel stages " << (nrys + nacc) * sizeof(double)
+        << " bytes of LDS, the launch gives " << c.shm;
+      violation(s.str());
+    }
+    if (qs.list) violation(L + ": explicit list on the row-sum kernel");
+  }
+  // every quartet, decoded as the row cursor does
+  auto qfail = [&](long long t, const std::string &what) {
+    std::ostringstream s;
+    s << L << ", quartet " << t << ": " << what;
+    violation(s.str());
+  };
+  const size_t nrow = size_t(qs.x_n) * size_t(qs.nseg);
+  if (!qs.list) {
+    if (qs.nseg < 1) return violation(L + ": no segments");
+    // REPLAY_PLANT=rowptr: the check asks for one row pointer more than the
+    // set holds, so a correct build reports it (the checker's own test)
+    const char *plant = std::getenv("REPLAY_PLANT");
+    const size_t extra = plant && std::strcmp(plant, "rowptr") == 0 ? 1 : 0;
+    need(L, "row pointers", qs.rowptr, (nrow + 1 + extra) * sizeof(long long));
+    need(L, "segment kets", qs.qbase, nrow * sizeof(int));
+    if (!inside(qs.rowptr, (nrow + 1) * sizeof(long long)) || !inside(qs.qbase, nrow * sizeof(int)))
+      return;
+    if (qs.rowptr[0] != 0 || qs.rowptr[nrow] != qs.total)
+      return violation(L + ": row pointers do not span the set");
+    for (size_t s = 0; s < nrow; ++s)
+      if (qs.rowptr[s + 1] < qs.rowptr[s]) return violation(L + ": row pointers decrease");
+  } else {
+    need(L, "quartet list", qs.list, size_t(qs.total) * sizeof(int2));
+    if (!inside(qs.list, size_t(qs.total) * sizeof(int2))) return;
+  }
+  size_t sg = 0;
+  for (long long t = 0; t < qs.total; ++t) {
+    ++g_quartets;
+    int P, Q;
+    if (qs.list) {
+      P = qs.list[t].x;
+      Q = qs.list[t].y;
+    } else {
+      while (sg < nrow && qs.rowptr[sg + 1] <= t) ++sg;
+      if (sg >= nrow) return qfail(t, "segment past the set");
+      P = int(sg / size_t(qs.nseg));
+      Q = qs.qbase[sg] + int(t - qs.rowptr[sg]);
+    }
+    if (P < 0 || P >= qs.x_n || Q < 0 || Q >= qs.y_n) return qfail(t, "pair index outside the class");
+    const int pb = qs.x_off + P, pk = qs.y_off + Q;
+    for (int pr : {pb, pk})
+      if (p.pp_off[pr] < 0 || size_t(p.pp_off[pr]) + size_t(p.pp_n[pr]) > nprim)
+        return qfail(t, "primitive pairs outside the table");
+    const int ss[4] = {p.sh1[pb], p.sh2[pb], p.sh1[pk], p.sh2[pk]};
+    const int ls[4] = {la, lb, lc, ld}, ns[4] = {na, nb, nc, nd};
+    for (int i = 0; i < 4; ++i) {
+      if (ss[i] < 0 || size_t(ss[i]) >= nsh) return qfail(t, "shell index outside the basis");
+      if (b.l[ss[i]] != ls[i]) return qfail(t, "shell momentum differs from the class");
+      if (b.off_cart[ss[i]] < 0 || size_t(b.off_cart[ss[i]] + ns[i]) > ncrt)
+        return qfail(t, "Cartesian functions outside the matrices");
+    }
+    if (rowacc) {
+      const int colc = b.rank[ss[2]] * nc, cold = dbase + b.rank[ss[3]] * nd;
+      if (colc < 0 || colc + nc > (lc == ld ? W : dbase) || cold < 0 || cold + nd > W)
+        return qfail(t, "row-sum column outside the LDS rows");
+    }
+  }
+  // the row-sum kernel's workgroups: contiguous ranges, each starting its
+  // row cursor with a binary search over the segments
+  if (rowacc && !qs.list) {
+    const long long grid = c.grid.x ? c.grid.x : 1;
+    const long long chunk = (qs.total + grid - 1) / grid;
+    for (long long g = 0; g < grid; ++g) {
+      const long long begin = g * chunk;
+      if (begin >= qs.total) break;
+      int lo = 0, hi = int(nrow) - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (qs.rowptr[mid] <= begin)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      if (!(qs.rowptr[lo] <= begin && begin < qs.rowptr[lo + 1]))
+        qfail(begin, "row cursor's first segment does not hold the range start");
+    }
+  }
+}
+
+void run_schwarz(void **args, bool big) {
+  // schwarz_kernel(BasisDev, PairsDev, RysDev, int first, int count, double *q);
+  // schwarz_big_kernel(int la, int lb, int rb, BasisDev, PairsDev, RysDev,
+  // int first, int count, double *q, ...)
+  const int o = big ? 3 : 0;
+  const PairsDev &p = *static_cast<const PairsDev *>(args[o + 1]);
+  const int first = *static_cast<int *>(args[o + 3]), count = *static_cast<int *>(args[o + 4]);
+  double *q = *static_cast<double **>(args[o + 5]);
+  for (int t = 0; t < count; ++t) {
+    const int pr = first + t;
+    const int i = p.sh1[pr], j = p.sh2[pr];
+    q[pr] = (i < g_nsh && j < g_nsh) ? g_schwarz[size_t(i) * g_nsh + j] : 0.0;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the caller's side
+extern "C" void replay_set_schwarz(const double *q, int nsh) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  g_schwarz.assign(q, q + size_t(nsh) * nsh);
+  g_nsh = nsh;
+}
+extern "C" int replay_violation_count() { return int(g_viol.size()); }
+extern "C" const char *replay_violation(int k) {
+  return k >= 0 && k < int(g_viol.size()) ? g_viol[size_t(k)].c_str() : "";
+}
+extern "C" long long replay_jk_launches() { return g_jk_launches; }
+extern "C" long long replay_quartets() { return g_quartets; }
+extern "C" void replay_reset() {
+  g_viol.clear();
+  g_jk_launches = g_quartets = g_other = 0;
+}
+
+// ---------------------------------------------------------------------------
+// runtime
+extern "C" hipError_t hipMalloc(void **ptr, size_t size) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  const size_t n = size ? size : 8;
+  void *p = std::aligned_alloc(256, (n + 255) / 256 * 256);
+  if (!p) return hipErrorOutOfMemory;
+  std::memset(p, 0, n);
+  g_alloc[reinterpret_cast<uintptr_t>(p)] = Alloc{n};
+  *ptr = p;
+  return hipSuccess;
+}
+extern "C" hipError_t hipFree(void *ptr) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  if (!ptr) return hipSuccess;
+  auto it = g_alloc.find(reinterpret_cast<uintptr_t>(ptr));
+  if (it == g_alloc.end()) {
+    violation("hipFree of a pointer that is not a live allocation");
+    return hipErrorInvalidValue;
+  }
+  g_alloc.erase(it);
+  std::free(ptr);
+  return hipSuccess;
+}
+extern "C" hipError_t hipMemcpy(void *dst, const void *src, size_t n, hipMemcpyKind) {
+  if (n) std::memmove(dst, src, n);
+  return hipSuccess;
+}
+extern "C" hipError_t hipMemcpyAsync(void *dst, const void *src, size_t n, hipMemcpyKind,
+                                     hipStream_t) {
+  if (n) std::memmove(dst, src, n);
+  return hipSuccess;
+}
+extern "C" hipError_t hipMemset(void *dst, int v, size_t n) {
+  std::memset(dst, v, n);
+  return hipSuccess;
+}
+extern "C" hipError_t hipMemsetAsync(void *dst, int v, size_t n, hipStream_t) {
+  std::memset(dst, v, n);
+  return hipSuccess;
+}
+extern "C" hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned int) {
+  static int dummy[64];
+  static int k = 0;
+  *s = reinterpret_cast<hipStream_t>(&dummy[(k++) % 64]);
+  return hipSuccess;
+}
+extern "C" hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+extern "C" hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+extern "C" hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned int) {
+  return hipSuccess;
+}
+extern "C" hipError_t hipEventCreate(hipEvent_t *e) {
+  *e = reinterpret_cast<hipEvent_t>(new int(0));
+  return hipSuccess;
+}
+extern "C" hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned int) {
+  *e = reinterpret_cast<hipEvent_t>(new int(0));
+  return hipSuccess;
+}
+extern "C" hipError_t hipEventDestroy(hipEvent_t e) {
+  delete reinterpret_cast<int *>(e);
+  return hipSuccess;
+}
+extern "C" hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+extern "C" hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+extern "C" hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
+  *ms = 0.0f;
+  return hipSuccess;
+}
+extern "C" hipError_t hipGetLastError(void) { return hipSuccess; }
+extern "C" const char *hipGetErrorString(hipError_t) { return "replay"; }
+extern "C" hipError_t hipGetDeviceCount(int *n) {
+  *n = 1;
+  return hipSuccess;
+}
+extern "C" hipError_t hipGetDevice(int *d) {
+  *d = 0;
+  return hipSuccess;
+}
+extern "C" hipError_t hipDeviceSynchronize(void) { return hipSuccess; }
+extern "C" hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int) {
+  switch (a) {
+    case hipDeviceAttributeMultiprocessorCount: *v = 256; break;
+    case hipDeviceAttributeMaxSharedMemoryPerBlock: *v = 160 * 1024; break;
+    case hipDeviceAttributeMaxSharedMemoryPerMultiprocessor: *v = 160 * 1024; break;
+    default: *v = 0;
+  }
+  return hipSuccess;
+}
+extern "C" hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, const void *, int,
+                                                                    size_t) {
+  const char *e = std::getenv("REPLAY_PER_CU");
+  *n = e ? std::atoi(e) : 2;
+  return hipSuccess;
+}
+extern "C" hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+  *p = const_cast<void *>(sym);
+  return hipSuccess;
+}
+extern "C" hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+extern "C" hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+
+// kernel registration and launch
+extern "C" void **__hipRegisterFatBinary(const void *) {
+  static void *handle = nullptr;
+  return &handle;
+}
+extern "C" void __hipUnregisterFatBinary(void **) {}
+extern "C" void __hipRegisterFunction(void **, const void *host, char *, const char *dev,
+                                      unsigned int, uint3 *, uint3 *, dim3 *, dim3 *, int *) {
+  g_kname[host] = demangle(dev);
+}
+extern "C" void __hipRegisterVar(void **, void *var, char *, char *, int, size_t size, int, int) {
+  // a __device__ variable: its host shadow stands in for the device copy
+  g_alloc[reinterpret_cast<uintptr_t>(var)] = Alloc{size};
+}
+extern "C" hipError_t __hipPushCallConfiguration(dim3 grid, dim3 block, size_t shm,
+                                                 hipStream_t s) {
+  g_cfg.push_back(Config{grid, block, shm, s});
+  return hipSuccess;
+}
+extern "C" hipError_t __hipPopCallConfiguration(dim3 *grid, dim3 *block, size_t *shm,
+                                                hipStream_t *s) {
+  Config c = g_cfg.empty() ? Config{} : g_cfg.back();
+  if (!g_cfg.empty()) g_cfg.pop_back();
+  *grid = c.grid;
+  *block = c.block;
+  *shm = c.shm;
+  *s = c.stream;
+  return hipSuccess;
+}
+extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void **args,
+                                      size_t shm, hipStream_t s) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  auto it = g_kname.find(f);
+  const std::string name = it == g_kname.end() ? std::string("?") : it->second;
+  const Config c{grid, block, shm, s};
+  if (block.x * block.y * block.z > 1024) violation(name + ": more than 1024 lanes");
+  if (shm > 160 * 1024) violation(name + ": more than 160 KB of dynamic LDS");
+  auto has = [&](const char *k) { return name.find(k) != std::string::npos; };
+  if (has("schwarz_big_kernel")) {
+    run_schwarz(args, true);
+  } else if (has("schwarz_kernel<")) {
+    run_schwarz(args, false);
+  } else if (has("xcc_probe_kernel")) {
+    unsigned *hist = *static_cast<unsigned **>(args[0]);
+    for (int k = 0; k < 8; ++k) hist[k] = 512;
+  } else if (has("asym_flag_kernel")) {
+    int *flag = *static_cast<int **>(args[3]);
+    *flag = 0;
+  } else if (has("fill_kernel")) {
+    double *x = *static_cast<double **>(args[0]);
+    const size_t n = *static_cast<size_t *>(args[1]);
+    const double v = *static_cast<double *>(args[2]);
+    if (!inside(x, n * sizeof(double))) violation("fill_kernel outside every live allocation");
+    else for (size_t k = 0; k < n; ++k) x[k] = v;
+  } else if (has("jk_rowacc_kernel<") || has("jk_reg_kernel<") || has("jk_kernel<") ||
+             has("jk_hrr_kernel<")) {
+    check_jk(name, args, c);
+  } else {
+    ++g_other;
+  }
+  return hipSuccess;
+}

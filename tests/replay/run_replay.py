@@ -1,0 +1,63 @@
+"""Run J/K builds of a workload through the CPU replay build of the library
+(tests/replay/libintegral_replay.so: the library's host code over a host
+stand-in for the HIP runtime that checks every J/K launch) and print one
+JSON line per build: launches, quartets checked, violations.
+
+Usage: python tests/replay/run_replay.py WORKLOAD [cut ...]
+  WORKLOAD  direct (the nao 520 s,p,d molecule, two densities, tau 1e-12)
+            or highl (the nao 565 molecule with g and l = 6 shells, one
+            density; its runtime-l launches are not checked);
+  cut       FP32 cut of a mixed build (0: FP64); the default sequence is
+            FP64, mixed below 1e-6, mixed below 1e-8, FP64 again, all on one
+            contractor (the re-planning between them is part of the check).
+
+The Schwarz bounds, which the library computes with a device kernel, come
+from the oracle (the replay writes them where that kernel would)."""
+import ctypes
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import integral_amd as ig  # noqa: E402
+from integral_amd import synthetic  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def main():
+    workload = sys.argv[1] if len(sys.argv) > 1 else "direct"
+    cuts = [float(c) for c in sys.argv[2:]] or [0.0, 1e-6, 1e-8, 0.0]
+    ig.LIB_PATH = os.path.join(HERE, "libintegral_replay.so")
+    lib = ig.library()
+    lib.replay_set_schwarz.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.replay_violation.restype = ctypes.c_char_p
+    lib.replay_violation.argtypes = [ctypes.c_int]
+    lib.replay_jk_launches.restype = ctypes.c_longlong
+    lib.replay_quartets.restype = ctypes.c_longlong
+    if workload == "direct":
+        shells, dens, tau = synthetic.direct_jk_config()
+    else:
+        shells, D = synthetic.high_l_config()
+        dens, tau = [D], 1e-12
+    q = np.ascontiguousarray(O.Basis(shells).schwarz(), dtype=np.float64)
+    lib.replay_set_schwarz(q.ctypes.data, len(shells))
+    dc = synthetic.to_basis(shells).direct_contractor()
+    for cut in cuts:
+        lib.replay_reset()
+        dc.set_mixed_precision(cut)
+        dc.jk(list(dens), threshold=tau)
+        n = lib.replay_violation_count()
+        print(json.dumps({"workload": workload, "fp32_below": cut,
+                          "jk_launches": lib.replay_jk_launches(),
+                          "quartets_checked": lib.replay_quartets(), "violations": n,
+                          "first": [lib.replay_violation(k).decode() for k in range(min(n, 5))]}),
+              flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,53 @@
+"""CPU replay of the direct J/K launches (no GPU): the library's host code,
+compiled host-only over a host stand-in for the HIP runtime
+(tests/replay/hip_replay.cpp), plans and launches the nao 520 s,p,d
+workload's builds, FP64 and mixed at 1e-6 and 1e-8 on one contractor, and
+every J/K launch is checked: its views against the live allocations, the
+row-sum kernel's LDS, and every quartet of its set against the pair, shell,
+function and row-sum tables.  The planted-mistake run asks the checker for
+one row pointer more than each set holds and must be reported."""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPLAY = os.path.join(HERE, "replay")
+
+
+@pytest.fixture(scope="module")
+def replay_lib():
+    if not shutil.which("/opt/rocm/bin/hipcc"):
+        pytest.skip("hipcc not present: the replay build compiles the library's host code")
+    subprocess.run(["make", "-j8", "-C", REPLAY], check=True, capture_output=True, text=True,
+                   timeout=900)
+    return os.path.join(REPLAY, "libintegral_replay.so")
+
+
+def run(workload, cuts, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    out = subprocess.run([sys.executable, os.path.join(REPLAY, "run_replay.py"), workload]
+                         + [str(c) for c in cuts], check=True, capture_output=True, text=True,
+                         timeout=900, env=e)
+    return [json.loads(l) for l in out.stdout.splitlines() if l.startswith("{")]
+
+
+def test_direct_jk_launches_stay_in_range_fp64_and_mixed(replay_lib):
+    recs = run("direct", [0.0, 1e-6, 1e-8, 0.0])
+    assert [r["fp32_below"] for r in recs] == [0.0, 1e-6, 1e-8, 0.0]
+    for r in recs:
+        assert r["violations"] == 0, r["first"]
+        assert r["jk_launches"] > 0 and r["quartets_checked"] > 8e7
+    # the mixed builds launch the FP64 and FP32 part of each class pair
+    assert recs[1]["jk_launches"] > recs[0]["jk_launches"]
+    assert recs[3] == recs[0]
+
+
+def test_replay_reports_a_planted_undersized_row_pointer_array(replay_lib):
+    recs = run("direct", [0.0], env={"REPLAY_PLANT": "rowptr"})
+    assert recs[0]["violations"] > 0
+    assert "row pointers" in recs[0]["first"][0]
