@@ -8,13 +8,15 @@
 //   * the planning kernels whose results the host reads back run here:
 //     Schwarz bounds (taken from a matrix the caller gives), the XCD probe,
 //     the density symmetry flag, the fill kernel;
-//   * every J/K launch of the compiled kernels (register, row-sum, team,
-//     team-HRR; FP64 and FP32; Rys and OS/HGP) is checked: each pointer of
-//     its argument views must lie in a live allocation of the size the views
-//     imply, the launch's dynamic LDS must hold what the row-sum kernel
-//     stages, and every quartet of its set, decoded as the kernels' row
-//     cursor does, must name pairs, primitives, shells, Cartesian functions
-//     and row-sum columns inside their tables;
+//   * every J/K launch (compiled register, row-sum, team and team-HRR
+//     kernels, FP64 and FP32, Rys and OS/HGP; the runtime-l per-quartet and
+//     sliced kernels) is checked: each pointer of its argument views must
+//     lie in a live allocation of the size the views imply, the launch's
+//     dynamic LDS must hold what the kernel stages (row sums; root-batch
+//     tables and digest), the runtime-l kernels' offset table and block
+//     slots must fit their scratch, and every quartet of its set, decoded as
+//     the kernels' row cursor does, must name pairs, primitives, shells,
+//     Cartesian functions and row-sum columns inside their tables;
 //   * other kernels do nothing.
 // A violation is recorded (the first few hundred, with the launch and the
 // quartet) for the caller: replay_violations().
@@ -127,21 +129,18 @@ bool class_of(const std::string &n, int cls[4], std::vector<std::string> &rest) 
 
 int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
 
-// The J/K launch of a compiled kernel: views, LDS and every quartet.
-void check_jk(const std::string &name, void **args, const Config &c) {
+// The J/K launch of a kernel of class (la..ld) whose views (BasisDev,
+// PairsDev, RysDev, QuartetSet, DensDev) are arguments v[0..4]: the views,
+// the row-sum kernel's LDS (rest: its template arguments after the class)
+// and every quartet.
+void check_jk(const std::string &name, int la, int lb, int lc, int ld, void **v,
+              const std::vector<std::string> &rest, const Config &c) {
   ++g_jk_launches;
-  const BasisDev &b = *static_cast<const BasisDev *>(args[0]);
-  const PairsDev &p = *static_cast<const PairsDev *>(args[1]);
-  const RysDev &R = *static_cast<const RysDev *>(args[2]);
-  const QuartetSet &qs = *static_cast<const QuartetSet *>(args[3]);
-  const DensDev &dd = *static_cast<const DensDev *>(args[4]);
-  int cls[4];
-  std::vector<std::string> rest;
-  if (!class_of(name, cls, rest)) {
-    violation("unparsed J/K kernel name " + name);
-    return;
-  }
-  const int la = cls[0], lb = cls[1], lc = cls[2], ld = cls[3];
+  const BasisDev &b = *static_cast<const BasisDev *>(v[0]);
+  const PairsDev &p = *static_cast<const PairsDev *>(v[1]);
+  const RysDev &R = *static_cast<const RysDev *>(v[2]);
+  const QuartetSet &qs = *static_cast<const QuartetSet *>(v[3]);
+  const DensDev &dd = *static_cast<const DensDev *>(v[4]);
   const int na = ncart_h(la), nb = ncart_h(lb), nc = ncart_h(lc), nd = ncart_h(ld);
   std::ostringstream tag;
   tag << name.substr(0, name.find('(')) << " grid " << c.grid.x << " lds " << c.shm;
@@ -281,6 +280,54 @@ void check_jk(const std::string &name, void **args, const Config &c) {
         qfail(begin, "row cursor's first segment does not hold the range start");
     }
   }
+}
+
+// The runtime-l J/K launches: the element-offset table (and for the
+// per-quartet kernel the block slots) in scratch, and the dynamic LDS of
+// their root batches and digest, as the kernels lay them out; then the
+// views and quartets as for the compiled kernels.
+void check_jk_runtime_l(const std::string &name, void **args, const Config &c) {
+  const int la = *static_cast<int *>(args[0]), lb = *static_cast<int *>(args[1]),
+            lc = *static_cast<int *>(args[2]), ld = *static_cast<int *>(args[3]);
+  const int rb = *static_cast<int *>(args[4]);
+  const bool sliced = name.find("jk_rt_kernel") != std::string::npos;
+  const int na = ncart_h(la), nb = ncart_h(lb), nc = ncart_h(lc), nd = ncart_h(ld);
+  const size_t n4 = size_t(na) * nb * nc * nd;
+  const size_t nds = size_t(na) * nb + size_t(nc) * nd + size_t(na + nb) * (nc + nd);
+  const size_t G = size_t(la + lb + 1) * (lc + ld + 1), T = size_t(la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
+  const size_t H = size_t(la + 1) * (lb + 1) * (lc + ld + 1);
+  const int nr = (la + lb + lc + ld) / 2 + 1;
+  std::ostringstream tag;
+  tag << name.substr(0, name.find('(')) << " (" << la << lb << "|" << lc << ld << ") grid "
+      << c.grid.x << " lds " << c.shm;
+  const std::string L = tag.str();
+  if (rb < 1 || rb > nr) violation(L + ": root batch outside 1..n roots");
+  size_t lds_need;
+  void **views;
+  const unsigned long long *eoff;
+  if (sliced) {
+    const int nsl = *static_cast<int *>(args[5]);
+    const size_t slice = std::min(n4, size_t(512) * 16);
+    if (size_t(nsl) * 512 * 16 < n4) violation(L + ": slices do not cover the block");
+    lds_need = std::max(size_t(rb) * 3 * (G + H + T), nds + slice) * sizeof(double);
+    views = args + 6;
+    eoff = *static_cast<const unsigned long long **>(args[12]);
+  } else {
+    lds_need = std::max(size_t(rb) * 3 * (G + T), nds) * sizeof(double);
+    views = args + 5;
+    eoff = *static_cast<const unsigned long long **>(args[11]);
+    const double *slots = *static_cast<double **>(args[12]);
+    const size_t slot = *static_cast<size_t *>(args[13]);
+    if (slot < n4) violation(L + ": block slot smaller than the block");
+    need(L, "block slots", slots, size_t(c.grid.x) * slot * sizeof(double));
+  }
+  need(L, "element-offset table", eoff, n4 * sizeof(unsigned long long));
+  if (lds_need > c.shm) {
+    std::ostringstream s;
+    s << L << ": tables and digest need " << lds_need << " bytes of LDS";
+    violation(s.str());
+  }
+  check_jk(name, la, lb, lc, ld, views, {}, c);
 }
 
 void run_schwarz(void **args, bool big) {
@@ -477,7 +524,14 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
     else for (size_t k = 0; k < n; ++k) x[k] = v;
   } else if (has("jk_rowacc_kernel<") || has("jk_reg_kernel<") || has("jk_kernel<") ||
              has("jk_hrr_kernel<")) {
-    check_jk(name, args, c);
+    int cls[4];
+    std::vector<std::string> rest;
+    if (!class_of(name, cls, rest))
+      violation("unparsed J/K kernel name " + name);
+    else
+      check_jk(name, cls[0], cls[1], cls[2], cls[3], args, rest, c);
+  } else if (has("jk_big_kernel") || has("jk_rt_kernel<")) {
+    check_jk_runtime_l(name, args, c);
   } else {
     ++g_other;
   }

@@ -1,10 +1,10 @@
 """CPU replay of the direct J/K launches (no GPU): the library's host code,
 compiled host-only over a host stand-in for the HIP runtime
 (tests/replay/hip_replay.cpp), plans and launches the nao 520 s,p,d
-workload's builds, FP64 and mixed at 1e-6 and 1e-8 on one contractor, and
-every J/K launch is checked: its views against the live allocations, the
-row-sum kernel's LDS, and every quartet of its set against the pair, shell,
-function and row-sum tables.  The planted-mistake run asks the checker for
+workload's builds (FP64 and mixed at 1e-6 and 1e-8 on one contractor) and
+the nao 565 high-l workload's, and every J/K launch is checked: its views
+against the live allocations, the kernel's LDS and scratch, and every
+quartet of its set against the pair, shell, function and row-sum tables.  The planted-mistake run asks the checker for
 one row pointer more than each set holds and must be reported."""
 import json
 import os
@@ -45,6 +45,16 @@ def test_direct_jk_launches_stay_in_range_fp64_and_mixed(replay_lib):
     # the mixed builds launch the FP64 and FP32 part of each class pair
     assert recs[1]["jk_launches"] > recs[0]["jk_launches"]
     assert recs[3] == recs[0]
+
+
+def test_high_l_jk_launches_stay_in_range_fp64_and_mixed(replay_lib):
+    # the nao 565 workload with g and l = 6 shells: its compiled classes and
+    # the runtime-l per-quartet and sliced kernels (offset table, block
+    # slots, root-batch and digest LDS)
+    recs = run("highl", [0.0, 1e-7])
+    for r in recs:
+        assert r["violations"] == 0, r["first"]
+        assert r["jk_launches"] > 200 and r["quartets_checked"] > 5e6
 
 
 def test_replay_reports_a_planted_undersized_row_pointer_array(replay_lib):
