@@ -4,8 +4,9 @@ compiled host-only over a host stand-in for the HIP runtime
 workload's builds (FP64 and mixed at 1e-6 and 1e-8 on one contractor) and
 the nao 565 high-l workload's, and every J/K launch is checked: its views
 against the live allocations, the kernel's LDS and scratch, and every
-quartet of its set against the pair, shell, function and row-sum tables.  The planted-mistake run asks the checker for
-one row pointer more than each set holds and must be reported."""
+quartet of its set against the pair, shell, function and row-sum tables.
+The planted-mistake run asks the checker for one row pointer more than each
+set holds and must be reported."""
 import json
 import os
 import shutil
