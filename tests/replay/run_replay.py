@@ -11,6 +11,10 @@ Usage: python tests/replay/run_replay.py WORKLOAD [cut ...]
             FP64, mixed below 1e-6, mixed below 1e-8, FP64 again, all on one
             contractor (the re-planning between them is part of the check).
 
+With REPLAY_AUTOTUNE set, the contractor's autotuner runs first (every
+kernel variant of every class pair at full size) and its launches are
+checked as well.
+
 The Schwarz bounds, which the library computes with a device kernel, come
 from the oracle (the replay writes them where that kernel would)."""
 import ctypes
@@ -47,6 +51,25 @@ def main():
     q = np.ascontiguousarray(O.Basis(shells).schwarz(), dtype=np.float64)
     lib.replay_set_schwarz(q.ctypes.data, len(shells))
     dc = synthetic.to_basis(shells).direct_contractor()
+    if os.environ.get("REPLAY_AUTOTUNE"):
+        # the autotuner launches every kernel variant of every class pair at
+        # full size; its "device" densities are an allocation of the stand-in
+        lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        lib.ig_direct_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_void_p]
+        d = np.ascontiguousarray(np.stack(dens), dtype=np.float64)
+        dptr = ctypes.c_void_p()
+        lib.hipMalloc(ctypes.byref(dptr), d.nbytes)
+        ctypes.memmove(dptr, d.ctypes.data, d.nbytes)
+        lib.replay_reset()
+        rc = lib.ig_direct_autotune(dc._h, dptr, len(dens), float(tau), 1, 1, None)
+        n = lib.replay_violation_count()
+        print(json.dumps({"workload": workload, "autotune_rc": rc,
+                          "jk_launches": lib.replay_jk_launches(),
+                          "quartets_checked": lib.replay_quartets(), "violations": n,
+                          "first": [lib.replay_violation(k).decode() for k in range(min(n, 5))]}),
+              flush=True)
     for cut in cuts:
         lib.replay_reset()
         dc.set_mixed_precision(cut)

@@ -58,6 +58,16 @@ def test_high_l_jk_launches_stay_in_range_fp64_and_mixed(replay_lib):
         assert r["jk_launches"] > 200 and r["quartets_checked"] > 5e6
 
 
+def test_autotuner_trial_launches_stay_in_range(replay_lib):
+    # every kernel variant of every class pair at full size
+    recs = run("direct", [0.0], env={"REPLAY_AUTOTUNE": "1"})
+    tune = recs[0]
+    assert tune["autotune_rc"] == 0
+    assert tune["violations"] == 0, tune["first"]
+    assert tune["jk_launches"] > 100 and tune["quartets_checked"] > 3e8
+    assert recs[1]["violations"] == 0, recs[1]["first"]
+
+
 def test_replay_reports_a_planted_undersized_row_pointer_array(replay_lib):
     recs = run("direct", [0.0], env={"REPLAY_PLANT": "rowptr"})
     assert recs[0]["violations"] > 0
