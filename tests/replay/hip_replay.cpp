@@ -17,6 +17,9 @@
 //     slots must fit their scratch, and every quartet of its set, decoded as
 //     the kernels' row cursor does, must name pairs, primitives, shells,
 //     Cartesian functions and row-sum columns inside their tables;
+//   * the gradient launches (compiled lane kernels and the runtime-l kernel)
+//     get the same view and quartet checks, with their density views and
+//     force accumulator in place of the J/K ones;
 //   * other kernels do nothing.
 // A violation is recorded (the first few hundred, with the launch and the
 // quartet) for the caller: replay_violations().
@@ -133,14 +136,16 @@ int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
 // PairsDev, RysDev, QuartetSet, DensDev) are arguments v[0..4]: the views,
 // the row-sum kernel's LDS (rest: its template arguments after the class)
 // and every quartet.
+// A gradient launch has a GradDev as v[4] instead (grad true).
 void check_jk(const std::string &name, int la, int lb, int lc, int ld, void **v,
-              const std::vector<std::string> &rest, const Config &c) {
+              const std::vector<std::string> &rest, const Config &c, bool grad = false) {
   ++g_jk_launches;
   const BasisDev &b = *static_cast<const BasisDev *>(v[0]);
   const PairsDev &p = *static_cast<const PairsDev *>(v[1]);
   const RysDev &R = *static_cast<const RysDev *>(v[2]);
   const QuartetSet &qs = *static_cast<const QuartetSet *>(v[3]);
-  const DensDev &dd = *static_cast<const DensDev *>(v[4]);
+  static const DensDev none{};
+  const DensDev &dd = grad ? none : *static_cast<const DensDev *>(v[4]);
   const int na = ncart_h(la), nb = ncart_h(lb), nc = ncart_h(lc), nd = ncart_h(ld);
   std::ostringstream tag;
   tag << name.substr(0, name.find('(')) << " grid " << c.grid.x << " lds " << c.shm;
@@ -171,12 +176,30 @@ void check_jk(const std::string &name, int la, int lb, int lc, int ld, void **v,
     nprim = std::max(nprim, size_t(std::max(0, p.pp_off[k])) + size_t(std::max(0, p.pp_n[k])));
   for (const double *a : {p.zeta, p.px, p.py, p.pz, p.kc, p.alpha, p.beta, p.kz, p.i2z})
     need(L, "primitive-pair table", a, nprim * sizeof(double));
-  need(L, "Cartesian densities", dd.dc, size_t(dd.neff) * n2 * sizeof(double));
-  need(L, "J accumulators", dd.jc, dd.copy_stride * size_t(dd.ncopy) * sizeof(double));
-  if (dd.kc) need(L, "K accumulators", dd.kc, dd.copy_stride * size_t(dd.ncopy) * sizeof(double));
-  if (dd.copy_stride < size_t(dd.neff) * n2) violation(L + ": accumulator copy stride too small");
-  need(L, "effective-density table", dd.eff_src, 2 * size_t(dd.neff) * sizeof(int));
-  need(L, "density bounds", dd.bound, size_t(dd.ndens) * nsh * nsh * sizeof(double));
+  if (!grad) {
+    need(L, "Cartesian densities", dd.dc, size_t(dd.neff) * n2 * sizeof(double));
+    need(L, "J accumulators", dd.jc, dd.copy_stride * size_t(dd.ncopy) * sizeof(double));
+    if (dd.kc)
+      need(L, "K accumulators", dd.kc, dd.copy_stride * size_t(dd.ncopy) * sizeof(double));
+    if (dd.copy_stride < size_t(dd.neff) * n2)
+      violation(L + ": accumulator copy stride too small");
+    need(L, "effective-density table", dd.eff_src, 2 * size_t(dd.neff) * sizeof(int));
+    need(L, "density bounds", dd.bound, size_t(dd.ndens) * nsh * nsh * sizeof(double));
+  } else {
+    const GradDev &gd = *static_cast<const GradDev *>(v[4]);
+    const size_t nm = size_t(gd.ncoul + gd.nexch);
+    int natom = 0;
+    if (inside(b.atom, nsh * sizeof(int)))
+      for (size_t k = 0; k < nsh; ++k) natom = std::max(natom, b.atom[k] + 1);
+    if (gd.gamma) {
+      need(L, "two-particle density", gd.gamma, n2 * n2 * sizeof(double));
+    } else {
+      need(L, "density factors", gd.fac, nm * sizeof(double));
+      need(L, "density matrices", gd.mats, nm * 2 * n2 * sizeof(double));
+      need(L, "density block maxima", gd.bmax, nm * 2 * nsh * nsh * sizeof(double));
+    }
+    need(L, "force accumulator", gd.out, size_t(natom) * 3 * sizeof(double));
+  }
   if (qs.total <= 0) return;
   if (qs.x_off < 0 || qs.x_n < 1 || qs.x_off + qs.x_n > p.n || qs.y_off < 0 || qs.y_n < 1 ||
       qs.y_off + qs.y_n > p.n) {
@@ -532,6 +555,22 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
       check_jk(name, cls[0], cls[1], cls[2], cls[3], args, rest, c);
   } else if (has("jk_big_kernel") || has("jk_rt_kernel<")) {
     check_jk_runtime_l(name, args, c);
+  } else if (has("grad_lane_kernel<")) {
+    int cls[4];
+    std::vector<std::string> rest;
+    if (!class_of(name, cls, rest))
+      violation("unparsed gradient kernel name " + name);
+    else
+      check_jk(name, cls[0], cls[1], cls[2], cls[3], args, {}, c, true);
+  } else if (has("grad_big_kernel")) {
+    // grad_big_kernel(la, lb, lc, ld, rb, b, p, R, qs, gd, scratch, slot, counters):
+    // one slot per workgroup (the contracted block of the raised tables)
+    const int la = *static_cast<int *>(args[0]), lb = *static_cast<int *>(args[1]),
+              lc = *static_cast<int *>(args[2]), ld = *static_cast<int *>(args[3]);
+    const double *slots = *static_cast<double **>(args[10]);
+    const size_t slot = *static_cast<size_t *>(args[11]);
+    need(name, "gradient block slots", slots, size_t(c.grid.x) * slot * sizeof(double));
+    check_jk(name, la, lb, lc, ld, args + 5, {}, c, true);
   } else {
     ++g_other;
   }

@@ -6,7 +6,8 @@ JSON line per build: launches, quartets checked, violations.
 Usage: python tests/replay/run_replay.py WORKLOAD [cut ...]
   WORKLOAD  direct (the nao 520 s,p,d molecule, two densities, tau 1e-12)
             or highl (the nao 565 molecule with g and l = 6 shells, one
-            density; its runtime-l launches are not checked);
+            density) or grad (the nao 390 s,p,d molecule: the adjoint
+            forces' launches first, then the J/K builds);
   cut       FP32 cut of a mixed build (0: FP64); the default sequence is
             FP64, mixed below 1e-6, mixed below 1e-8, FP64 again, all on one
             contractor (the re-planning between them is part of the check).
@@ -45,12 +46,29 @@ def main():
     lib.replay_quartets.restype = ctypes.c_longlong
     if workload == "direct":
         shells, dens, tau = synthetic.direct_jk_config()
+    elif workload == "grad":
+        shells, D = synthetic.gradient_config()
+        dens, tau = [D], 1e-12
     else:
         shells, D = synthetic.high_l_config()
         dens, tau = [D], 1e-12
     q = np.ascontiguousarray(O.Basis(shells).schwarz(), dtype=np.float64)
     lib.replay_set_schwarz(q.ctypes.data, len(shells))
-    dc = synthetic.to_basis(shells).direct_contractor()
+    basis = synthetic.to_basis(shells)
+    if workload == "grad":
+        # the J/K energy's contraction and the adjoint forces (Coulomb 1,
+        # exchange -1/2), as one geometry of the energy-and-forces workload
+        gc = basis.direct_grad_contractor()
+        lib.replay_reset()
+        dg = gc.density([ig.GradDensityTerm(0.5, D, D)], [ig.GradDensityTerm(-0.25, D, D)])
+        gc.contract(dg, threshold=tau)
+        n = lib.replay_violation_count()
+        print(json.dumps({"workload": workload, "forces": True,
+                          "jk_launches": lib.replay_jk_launches(),
+                          "quartets_checked": lib.replay_quartets(), "violations": n,
+                          "first": [lib.replay_violation(k).decode() for k in range(min(n, 5))]}),
+              flush=True)
+    dc = basis.direct_contractor()
     if os.environ.get("REPLAY_AUTOTUNE"):
         # the autotuner launches every kernel variant of every class pair at
         # full size; its "device" densities are an allocation of the stand-in

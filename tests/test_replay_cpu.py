@@ -1,8 +1,9 @@
 """CPU replay of the direct J/K launches (no GPU): the library's host code,
 compiled host-only over a host stand-in for the HIP runtime
 (tests/replay/hip_replay.cpp), plans and launches the nao 520 s,p,d
-workload's builds (FP64 and mixed at 1e-6 and 1e-8 on one contractor) and
-the nao 565 high-l workload's, and every J/K launch is checked: its views
+workload's builds (FP64 and mixed at 1e-6 and 1e-8 on one contractor, and
+the autotuner's trials), the nao 565 high-l workload's and the nao 390
+workload's forces, and every J/K and gradient launch is checked: its views
 against the live allocations, the kernel's LDS and scratch, and every
 quartet of its set against the pair, shell, function and row-sum tables.
 The planted-mistake run asks the checker for one row pointer more than each
@@ -56,6 +57,15 @@ def test_high_l_jk_launches_stay_in_range_fp64_and_mixed(replay_lib):
     for r in recs:
         assert r["violations"] == 0, r["first"]
         assert r["jk_launches"] > 200 and r["quartets_checked"] > 5e6
+
+
+def test_gradient_launches_stay_in_range(replay_lib):
+    # the nao 390 energy-and-forces workload: adjoint force kernels, then J/K
+    recs = run("grad", [0.0])
+    assert recs[0].get("forces") and recs[0]["jk_launches"] > 50
+    for r in recs:
+        assert r["violations"] == 0, r["first"]
+        assert r["quartets_checked"] > 1e7
 
 
 def test_autotuner_trial_launches_stay_in_range(replay_lib):
