@@ -22,7 +22,9 @@
 //     force accumulator in place of the J/K ones;
 //   * other kernels do nothing.
 // A violation is recorded (the first few hundred, with the launch and the
-// quartet) for the caller: replay_violations().
+// quartet) for the caller: replay_violation_count() / replay_violation(k).
+// For the error-path tests an allocation or the launches of a named kernel
+// can be made to fail (replay_fail_malloc_after, replay_fail_launch).
 #include <cxxabi.h>
 
 #include <cstdlib>
@@ -60,6 +62,10 @@ struct State {
   int nsh = 0;
   long long jk_launches = 0, quartets = 0, other = 0;
   std::vector<Config> cfg;
+  // failure injection for the error-path tests
+  long long fail_malloc_after = -1;  // the k-th hipMalloc from now fails (-1: off)
+  std::string fail_launch;           // launches of kernels whose name holds it fail
+  hipError_t last = hipSuccess;      // the error hipGetLastError reports next
 };
 State &st() {
   static State s;
@@ -383,6 +389,13 @@ extern "C" const char *replay_violation(int k) {
 }
 extern "C" long long replay_jk_launches() { return g_jk_launches; }
 extern "C" long long replay_quartets() { return g_quartets; }
+// Failure injection: the k-th hipMalloc from now returns
+// hipErrorOutOfMemory (k < 0: off); every launch of a kernel whose name
+// contains `name` fails with hipErrorLaunchFailure, reported by the next
+// hipGetLastError (null or empty: off).
+extern "C" void replay_fail_malloc_after(long long k) { st().fail_malloc_after = k; }
+extern "C" void replay_fail_launch(const char *name) { st().fail_launch = name ? name : ""; }
+extern "C" long long replay_live_allocations() { return (long long)g_alloc.size(); }
 extern "C" void replay_reset() {
   g_viol.clear();
   g_jk_launches = g_quartets = g_other = 0;
@@ -392,6 +405,10 @@ extern "C" void replay_reset() {
 // runtime
 extern "C" hipError_t hipMalloc(void **ptr, size_t size) {
   std::lock_guard<std::recursive_mutex> g(g_mu);
+  if (st().fail_malloc_after >= 0 && st().fail_malloc_after-- == 0) {
+    *ptr = nullptr;
+    return hipErrorOutOfMemory;
+  }
   const size_t n = size ? size : 8;
   void *p = std::aligned_alloc(256, (n + 255) / 256 * 256);
   if (!p) return hipErrorOutOfMemory;
@@ -458,8 +475,16 @@ extern "C" hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
   *ms = 0.0f;
   return hipSuccess;
 }
-extern "C" hipError_t hipGetLastError(void) { return hipSuccess; }
-extern "C" const char *hipGetErrorString(hipError_t) { return "replay"; }
+extern "C" hipError_t hipGetLastError(void) {
+  const hipError_t e = st().last;
+  st().last = hipSuccess;
+  return e;
+}
+extern "C" const char *hipGetErrorString(hipError_t e) {
+  return e == hipErrorOutOfMemory     ? "out of memory (injected)"
+         : e == hipErrorLaunchFailure ? "launch failure (injected)"
+                                      : "replay";
+}
 extern "C" hipError_t hipGetDeviceCount(int *n) {
   *n = 1;
   return hipSuccess;
@@ -526,6 +551,10 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
   auto it = g_kname.find(f);
   const std::string name = it == g_kname.end() ? std::string("?") : it->second;
   const Config c{grid, block, shm, s};
+  if (!st().fail_launch.empty() && name.find(st().fail_launch) != std::string::npos) {
+    st().last = hipErrorLaunchFailure;
+    return hipErrorLaunchFailure;
+  }
   if (block.x * block.y * block.z > 1024) violation(name + ": more than 1024 lanes");
   if (shm > 160 * 1024) violation(name + ": more than 160 KB of dynamic LDS");
   auto has = [&](const char *k) { return name.find(k) != std::string::npos; };

@@ -82,3 +82,22 @@ def test_replay_reports_a_planted_undersized_row_pointer_array(replay_lib):
     recs = run("direct", [0.0], env={"REPLAY_PLANT": "rowptr"})
     assert recs[0]["violations"] > 0
     assert "row pointers" in recs[0]["first"][0]
+
+
+def test_injected_device_failures_are_reported_and_survived(replay_lib):
+    """A failed kernel launch, a failed allocation while a contractor is
+    created, and a failed allocation while a call re-plans for a new FP32
+    cut: each comes back as a DeviceError naming the step, leaves nothing
+    allocated behind, and the next call on the same objects succeeds."""
+    out = subprocess.run([sys.executable, os.path.join(REPLAY, "run_failures.py")],
+                         check=True, capture_output=True, text=True, timeout=900)
+    recs = {r["scenario"]: r for r in (json.loads(l) for l in out.stdout.splitlines()
+                                       if l.startswith("{"))}
+    assert set(recs) == {"launch", "alloc", "replan"}
+    assert recs["launch"]["error"].startswith("DeviceError: jk launch")
+    assert recs["alloc"]["error"].startswith("DeviceError: hipMalloc")
+    assert recs["replan"]["error"].startswith("DeviceError: hipMalloc")
+    assert recs["alloc"]["left_allocated"] == 0
+    for r in recs.values():
+        assert r["next_call_error"] is None
+        assert r["violations"] == 0
