@@ -66,6 +66,7 @@ struct State {
   long long fail_malloc_after = -1;  // the k-th hipMalloc from now fails (-1: off)
   std::string fail_launch;           // launches of kernels whose name holds it fail
   hipError_t last = hipSuccess;      // the error hipGetLastError reports next
+  long long mallocs = 0;             // hipMalloc calls since the last reset
 };
 State &st() {
   static State s;
@@ -396,15 +397,18 @@ extern "C" long long replay_quartets() { return g_quartets; }
 extern "C" void replay_fail_malloc_after(long long k) { st().fail_malloc_after = k; }
 extern "C" void replay_fail_launch(const char *name) { st().fail_launch = name ? name : ""; }
 extern "C" long long replay_live_allocations() { return (long long)g_alloc.size(); }
+extern "C" long long replay_malloc_calls() { return st().mallocs; }
 extern "C" void replay_reset() {
   g_viol.clear();
   g_jk_launches = g_quartets = g_other = 0;
+  st().mallocs = 0;
 }
 
 // ---------------------------------------------------------------------------
 // runtime
 extern "C" hipError_t hipMalloc(void **ptr, size_t size) {
   std::lock_guard<std::recursive_mutex> g(g_mu);
+  ++st().mallocs;
   if (st().fail_malloc_after >= 0 && st().fail_malloc_after-- == 0) {
     *ptr = nullptr;
     return hipErrorOutOfMemory;

@@ -14,7 +14,9 @@ Usage: python tests/replay/run_replay.py WORKLOAD [cut ...]
 
 With REPLAY_AUTOTUNE set, the contractor's autotuner runs first (every
 kernel variant of every class pair at full size) and its launches are
-checked as well.
+checked as well.  With REPLAY_SCRATCH_PATHS set, the builds also run
+through a worker's own workspace and through the no-allocation entry (two
+calls, the second on one density).
 
 The Schwarz bounds, which the library computes with a device kernel, come
 from the oracle (the replay writes them where that kernel would)."""
@@ -44,6 +46,7 @@ def main():
     lib.replay_violation.argtypes = [ctypes.c_int]
     lib.replay_jk_launches.restype = ctypes.c_longlong
     lib.replay_quartets.restype = ctypes.c_longlong
+    lib.replay_malloc_calls.restype = ctypes.c_longlong
     if workload == "direct":
         shells, dens, tau = synthetic.direct_jk_config()
     elif workload == "grad":
@@ -88,16 +91,50 @@ def main():
                           "quartets_checked": lib.replay_quartets(), "violations": n,
                           "first": [lib.replay_violation(k).decode() for k in range(min(n, 5))]}),
               flush=True)
+    def report(**kw):
+        n = lib.replay_violation_count()
+        rec = {"workload": workload}
+        rec.update(kw)
+        rec.update({"jk_launches": lib.replay_jk_launches(),
+                    "quartets_checked": lib.replay_quartets(), "violations": n,
+                    "first": [lib.replay_violation(k).decode() for k in range(min(n, 5))]})
+        print(json.dumps(rec), flush=True)
+
     for cut in cuts:
         lib.replay_reset()
         dc.set_mixed_precision(cut)
         dc.jk(list(dens), threshold=tau)
-        n = lib.replay_violation_count()
-        print(json.dumps({"workload": workload, "fp32_below": cut,
-                          "jk_launches": lib.replay_jk_launches(),
-                          "quartets_checked": lib.replay_quartets(), "violations": n,
-                          "first": [lib.replay_violation(k).decode() for k in range(min(n, 5))]}),
-              flush=True)
+        report(fp32_below=cut)
+    if os.environ.get("REPLAY_SCRATCH_PATHS"):
+        # a worker's own workspace (its scratch, the contractor's tables)
+        ws = ig.DirectWorkspace()
+        for cut in (0.0, 1e-6):
+            lib.replay_reset()
+            dc.set_mixed_precision(cut)
+            dc.jk(list(dens), threshold=tau, workspace=ws)
+            report(path="workspace", fp32_below=cut)
+        # the no-allocation entry: every view carved from the fixed device
+        # array; two calls, the second on fewer densities
+        dc.set_mixed_precision(0.0)
+        lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        lib.ig_direct_jk_device_noalloc.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_int,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        n = basis.nao()
+        for nd in (len(dens), 1):
+            d = np.ascontiguousarray(np.stack(dens[:nd]), dtype=np.float64)
+            bufs = []
+            for _ in range(3):
+                p = ctypes.c_void_p()
+                lib.hipMalloc(ctypes.byref(p), d.nbytes)
+                bufs.append(p)
+            ctypes.memmove(bufs[0], d.ctypes.data, d.nbytes)
+            lib.replay_reset()
+            rc = lib.ig_direct_jk_device_noalloc(dc._h, bufs[0], nd, float(tau), 1, bufs[1],
+                                                 bufs[2], 1, None)
+            msg = lib.ig_last_error_message().decode() if rc else ""
+            report(path="noalloc", ndens=nd, rc=rc, message=msg,
+                   device_allocations=lib.replay_malloc_calls())
 
 
 if __name__ == "__main__":

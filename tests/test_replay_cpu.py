@@ -68,6 +68,20 @@ def test_gradient_launches_stay_in_range(replay_lib):
         assert r["quartets_checked"] > 1e7
 
 
+def test_workspace_and_no_allocation_paths_stay_in_range(replay_lib):
+    recs = run("direct", [0.0], env={"REPLAY_SCRATCH_PATHS": "1"})
+    ws = [r for r in recs if r.get("path") == "workspace"]
+    na = [r for r in recs if r.get("path") == "noalloc"]
+    assert [r["fp32_below"] for r in ws] == [0.0, 1e-6]
+    assert [r["ndens"] for r in na] == [2, 1]
+    for r in recs:
+        assert r["violations"] == 0, r["first"]
+        assert r["jk_launches"] > 0
+    for r in na:  # every view carved from the fixed array: no allocation
+        assert r["rc"] == 0, r["message"]
+        assert r["device_allocations"] == 0
+
+
 def test_autotuner_trial_launches_stay_in_range(replay_lib):
     # every kernel variant of every class pair at full size
     recs = run("direct", [0.0], env={"REPLAY_AUTOTUNE": "1"})
