@@ -262,7 +262,7 @@ hipError_t launch_grad_lane(int la, int lb, int lc, int ld, const BasisDev &b, c
 // workgroup; two per workgroup for the dense ERI, which transforms in place)
 size_t big_scratch_elems(int la, int lb, int lc, int ld, long long total, int two_halves);
 // the same for the runtime-l J/K launcher (launch_jk_big), whose blocks of
-// 16384 elements or more stay in registers (the element offset table only)
+// 262144 elements or more stay in registers (the element offset table only)
 size_t jk_big_scratch_elems(int la, int lb, int lc, int ld, long long total);
 hipError_t launch_schwarz_big(int la, int lb, const BasisDev &b, const PairsDev &p,
                               const RysDev &r, int first, int count, double *qout,
