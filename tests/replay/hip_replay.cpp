@@ -19,7 +19,8 @@
 //     Cartesian functions and row-sum columns inside their tables;
 //   * the gradient launches (compiled lane kernels and the runtime-l kernel)
 //     get the same view and quartet checks, with their density views and
-//     force accumulator in place of the J/K ones;
+//     force accumulator in place of the J/K ones, and the ERI launches with
+//     their output buffer;
 //   * other kernels do nothing.
 // A violation is recorded (the first few hundred, with the launch and the
 // quartet) for the caller: replay_violation_count() / replay_violation(k).
@@ -145,7 +146,8 @@ int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
 // and every quartet.
 // A gradient launch has a GradDev as v[4] instead (grad true).
 void check_jk(const std::string &name, int la, int lb, int lc, int ld, void **v,
-              const std::vector<std::string> &rest, const Config &c, bool grad = false) {
+              const std::vector<std::string> &rest, const Config &c, bool grad = false,
+              bool eri = false) {
   ++g_jk_launches;
   const BasisDev &b = *static_cast<const BasisDev *>(v[0]);
   const PairsDev &p = *static_cast<const PairsDev *>(v[1]);
@@ -192,7 +194,7 @@ void check_jk(const std::string &name, int la, int lb, int lc, int ld, void **v,
       violation(L + ": accumulator copy stride too small");
     need(L, "effective-density table", dd.eff_src, 2 * size_t(dd.neff) * sizeof(int));
     need(L, "density bounds", dd.bound, size_t(dd.ndens) * nsh * nsh * sizeof(double));
-  } else {
+  } else if (!eri) {
     const GradDev &gd = *static_cast<const GradDev *>(v[4]);
     const size_t nm = size_t(gd.ncoul + gd.nexch);
     int natom = 0;
@@ -588,6 +590,39 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
       check_jk(name, cls[0], cls[1], cls[2], cls[3], args, rest, c);
   } else if (has("jk_big_kernel") || has("jk_rt_kernel<")) {
     check_jk_runtime_l(name, args, c);
+  } else if (has("eri_kernel<") || has("eri_big_kernel")) {
+    // eri_kernel<C>(b, p, R, qs, EriOut); eri_big_kernel(la, lb, lc, ld, rb,
+    // b, p, R, qs, EriOut, eoff, scratch, slot): views, quartets and the
+    // output buffer (sized by its mode, at least live)
+    int la, lb, lc, ld;
+    void **v;
+    if (has("eri_big_kernel")) {
+      la = *static_cast<int *>(args[0]);
+      lb = *static_cast<int *>(args[1]);
+      lc = *static_cast<int *>(args[2]);
+      ld = *static_cast<int *>(args[3]);
+      v = args + 5;
+      const double *slots = *static_cast<double **>(args[11]);
+      const size_t slot = *static_cast<size_t *>(args[12]);
+      need(name, "ERI block slots", slots, size_t(c.grid.x) * slot * sizeof(double));
+    } else {
+      int cls[4];
+      std::vector<std::string> rest;
+      if (!class_of(name, cls, rest)) {
+        violation("unparsed ERI kernel name " + name);
+        return hipSuccess;
+      }
+      la = cls[0], lb = cls[1], lc = cls[2], ld = cls[3];
+      v = args;
+    }
+    const EriOut &eo = *static_cast<const EriOut *>(v[4]);
+    const BasisDev &eb = *static_cast<const BasisDev *>(v[0]);
+    const size_t nao = size_t(eb.nao);
+    // the dense tensor spans nao^4 (the packed and DF modes at least one
+    // element: their extents depend on the builder's ranges)
+    need(name, "ERI output", eo.out,
+         (eo.mode == kEriDense ? nao * nao * nao * nao : size_t(1)) * sizeof(double));
+    check_jk(name, la, lb, lc, ld, v, {}, c, true, true);
   } else if (has("grad_lane_kernel<")) {
     int cls[4];
     std::vector<std::string> rest;

@@ -7,7 +7,8 @@ Usage: python tests/replay/run_replay.py WORKLOAD [cut ...]
   WORKLOAD  direct (the nao 520 s,p,d molecule, two densities, tau 1e-12)
             or highl (the nao 565 molecule with g and l = 6 shells, one
             density) or grad (the nao 390 s,p,d molecule: the adjoint
-            forces' launches first, then the J/K builds);
+            forces' launches first, then the J/K builds) or eri (the dense
+            tensor of the 10-atom nao 50 molecule);
   cut       FP32 cut of a mixed build (0: FP64); the default sequence is
             FP64, mixed below 1e-6, mixed below 1e-8, FP64 again, all on one
             contractor (the re-planning between them is part of the check).
@@ -49,6 +50,9 @@ def main():
     lib.replay_malloc_calls.restype = ctypes.c_longlong
     if workload == "direct":
         shells, dens, tau = synthetic.direct_jk_config()
+    elif workload == "eri":
+        shells, _ = synthetic.dense_eri_config()
+        dens, tau = [], 0.0
     elif workload == "grad":
         shells, D = synthetic.gradient_config()
         dens, tau = [D], 1e-12
@@ -58,6 +62,18 @@ def main():
     q = np.ascontiguousarray(O.Basis(shells).schwarz(), dtype=np.float64)
     lib.replay_set_schwarz(q.ctypes.data, len(shells))
     basis = synthetic.to_basis(shells)
+    if workload == "eri":
+        # the dense nao^4 tensor (every canonical quartet, both engines'
+        # kernels as the library picks them)
+        lib.replay_reset()
+        basis.eri()
+        n = lib.replay_violation_count()
+        print(json.dumps({"workload": workload, "dense": True,
+                          "jk_launches": lib.replay_jk_launches(),
+                          "quartets_checked": lib.replay_quartets(), "violations": n,
+                          "first": [lib.replay_violation(k).decode() for k in range(min(n, 5))]}),
+              flush=True)
+        return
     if workload == "grad":
         # the J/K energy's contraction and the adjoint forces (Coulomb 1,
         # exchange -1/2), as one geometry of the energy-and-forces workload

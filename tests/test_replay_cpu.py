@@ -82,6 +82,12 @@ def test_workspace_and_no_allocation_paths_stay_in_range(replay_lib):
         assert r["device_allocations"] == 0
 
 
+def test_dense_eri_launches_stay_in_range(replay_lib):
+    rec = run("eri", [])[0]
+    assert rec["dense"] and rec["jk_launches"] > 0 and rec["quartets_checked"] > 1e5
+    assert rec["violations"] == 0, rec["first"]
+
+
 def test_autotuner_trial_launches_stay_in_range(replay_lib):
     # every kernel variant of every class pair at full size
     recs = run("direct", [0.0], env={"REPLAY_AUTOTUNE": "1"})
