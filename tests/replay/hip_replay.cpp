@@ -370,10 +370,13 @@ void run_schwarz(void **args, bool big) {
   const PairsDev &p = *static_cast<const PairsDev *>(args[o + 1]);
   const int first = *static_cast<int *>(args[o + 3]), count = *static_cast<int *>(args[o + 4]);
   double *q = *static_cast<double **>(args[o + 5]);
+  // REPLAY_UNIT_SCHWARZ: every pair's bound is 1 (nothing screened; for
+  // plans over bases the caller's matrix does not describe)
+  const bool unit = std::getenv("REPLAY_UNIT_SCHWARZ") != nullptr;
   for (int t = 0; t < count; ++t) {
     const int pr = first + t;
     const int i = p.sh1[pr], j = p.sh2[pr];
-    q[pr] = (i < g_nsh && j < g_nsh) ? g_schwarz[size_t(i) * g_nsh + j] : 0.0;
+    q[pr] = unit ? 1.0 : (i < g_nsh && j < g_nsh) ? g_schwarz[size_t(i) * g_nsh + j] : 0.0;
   }
 }
 

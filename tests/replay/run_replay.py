@@ -8,7 +8,8 @@ Usage: python tests/replay/run_replay.py WORKLOAD [cut ...]
             or highl (the nao 565 molecule with g and l = 6 shells, one
             density) or grad (the nao 390 s,p,d molecule: the adjoint
             forces' launches first, then the J/K builds) or eri (the dense
-            tensor of the 10-atom nao 50 molecule);
+            tensor of the 10-atom nao 50 molecule) or batch (the batched
+            plans of six small molecules through ig_batch_check);
   cut       FP32 cut of a mixed build (0: FP64); the default sequence is
             FP64, mixed below 1e-6, mixed below 1e-8, FP64 again, all on one
             contractor (the re-planning between them is part of the check).
@@ -48,6 +49,24 @@ def main():
     lib.replay_jk_launches.restype = ctypes.c_longlong
     lib.replay_quartets.restype = ctypes.c_longlong
     lib.replay_malloc_calls.restype = ctypes.c_longlong
+    if workload == "batch":
+        # the batched plans of the six molecules of the batched tests, with
+        # unit Schwarz bounds (every quartet kept), through the library's
+        # own host walk of the batched launches (ig_batch_check)
+        os.environ["REPLAY_UNIT_SCHWARZ"] = "1"
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from test_gpu_batch import batch_molecules
+        mols = batch_molecules()
+        bc = ig.BatchContractor([synthetic.to_basis(s) for s, _ in mols])
+        for tau, want_k, cut in ((1e-12, True, 0.0), (1e-12, False, 0.0), (1e-9, True, 0.0),
+                                 (1e-12, True, 1e-8)):
+            bc.set_mixed_precision(cut)
+            lib.replay_reset()
+            msg = bc.check(tau, want_k)
+            print(json.dumps({"workload": workload, "tau": tau, "exchange": want_k,
+                              "fp32_below": cut, "check": msg,
+                              "violations": lib.replay_violation_count()}), flush=True)
+        return
     if workload == "direct":
         shells, dens, tau = synthetic.direct_jk_config()
     elif workload == "eri":

@@ -88,6 +88,15 @@ def test_dense_eri_launches_stay_in_range(replay_lib):
     assert rec["violations"] == 0, rec["first"]
 
 
+def test_batched_plans_pass_the_library_host_walk(replay_lib):
+    # ig_batch_check on the CPU: the six molecules of the batched tests,
+    # every quartet kept (unit Schwarz bounds), FP64 and FP32 below 1e-8
+    recs = run("batch", [])
+    assert len(recs) == 4
+    for r in recs:
+        assert r["check"] == "" and r["violations"] == 0, r
+
+
 def test_autotuner_trial_launches_stay_in_range(replay_lib):
     # every kernel variant of every class pair at full size
     recs = run("direct", [0.0], env={"REPLAY_AUTOTUNE": "1"})
