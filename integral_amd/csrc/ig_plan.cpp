@@ -2641,8 +2641,13 @@ extern "C" int ig_eri_grad_contract_device(const ig_basis *basis, int engine,
 // The batched graph's execution is switched off until the cause of an
 // illegal memory access it took on the GPU (small mixed s,p,d,f molecules,
 // after 64-molecule runs had passed) is found; ig_batch_jk_device fails with
-// IG_ERR_UNSUPPORTED meanwhile.
-constexpr bool kBatchGraphEnabled = false;
+// IG_ERR_UNSUPPORTED meanwhile.  Only the host-only build of the launch
+// checker (tests/replay) defines IG_BATCH_GRAPH=1, to walk the capture and
+// every graph launch on the CPU.
+#ifndef IG_BATCH_GRAPH
+#define IG_BATCH_GRAPH 0
+#endif
+constexpr bool kBatchGraphEnabled = IG_BATCH_GRAPH != 0;
 
 struct ig_batch {
   std::vector<std::unique_ptr<Plan>> plans;

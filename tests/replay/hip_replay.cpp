@@ -2,9 +2,14 @@
 // library (its host code, compiled with --offload-host-only) that checks
 // the device work the host code would launch.
 //
-// Device memory is host memory from an allocation table; streams, events
-// and graphs do nothing; copies are memcpy.  Kernel launches arrive by name
-// (the launch stubs register their kernels' names at load time):
+// Device memory is host memory from an allocation table; streams and
+// events do nothing; copies are memcpy.  A stream capture records the
+// launches on the capturing stream (with a copy of their arguments) instead
+// of running them, and each graph launch runs the recorded list against
+// memory as it is then; allocations, frees, synchronous copies, memsets and
+// launches on other streams while a stream captures are violations.  Kernel
+// launches arrive by name (the launch stubs register their kernels' names
+// at load time):
 //   * the planning kernels whose results the host reads back run here:
 //     Schwarz bounds (taken from a matrix the caller gives), the XCD probe,
 //     the density symmetry flag, the fill kernel;
@@ -21,6 +26,11 @@
 //     get the same view and quartet checks, with their density views and
 //     force accumulator in place of the J/K ones, and the ERI launches with
 //     their output buffer;
+//   * the batched row-sum launches get the checks of the single-basis
+//     kernel per basis, through the device arrays of views they read, and
+//     their zeroed work queue and LDS accumulator size;
+//   * the density split, Cartesian transform, block-maximum, fold and
+//     finishing kernels: every array against the sizes their loops cover;
 //   * other kernels do nothing.
 // A violation is recorded (the first few hundred, with the launch and the
 // quartet) for the caller: replay_violation_count() / replay_violation(k).
@@ -32,6 +42,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <sstream>
 #include <string>
 #include <unordered_map>
@@ -51,6 +62,16 @@ struct Config {
   size_t shm = 0;
   hipStream_t stream = nullptr;
 };
+// A kernel launch made while a stream captures: not run then, but recorded
+// with its configuration and a copy of its arguments (for the kernels whose
+// argument lists the replay knows), and run at every launch of the graph.
+struct Recorded {
+  std::string name;
+  Config c;
+  bool known = false;
+  std::vector<std::vector<char>> store;  // one copy per argument
+  std::vector<void *> args;
+};
 // The tables are built on first use: the other objects' static initialisers
 // register their kernels and device variables before this file's globals
 // would be constructed.
@@ -68,6 +89,13 @@ struct State {
   std::string fail_launch;           // launches of kernels whose name holds it fail
   hipError_t last = hipSuccess;      // the error hipGetLastError reports next
   long long mallocs = 0;             // hipMalloc calls since the last reset
+  // graph capture: the batched row-sum launches recorded while a stream
+  // captures, re-checked at every launch of the graph
+  bool capturing = false;
+  hipStream_t capture_stream = nullptr;
+  std::vector<Recorded> *rec = nullptr;
+  std::set<hipGraph_t> live_graphs;
+  long long graph_launches = 0;
 };
 State &st() {
   static State s;
@@ -216,7 +244,8 @@ void check_jk(const std::string &name, int la, int lb, int lc, int ld, void **v,
     return;
   }
   // row-sum kernels: LDS of the staged Rys tables and the row sums
-  const bool rowacc = name.find("jk_rowacc_kernel") != std::string::npos;
+  const bool rowacc = name.find("jk_rowacc_kernel") != std::string::npos ||
+                      name.find("jk_rowacc_batch_kernel") != std::string::npos;
   int W = 0, dbase = 0;
   if (rowacc) {
     const int nC = b.lshell_off[lc + 1] - b.lshell_off[lc];
@@ -362,6 +391,138 @@ void check_jk_runtime_l(const std::string &name, void **args, const Config &c) {
   check_jk(name, la, lb, lc, ld, views, {}, c);
 }
 
+// The basis views the density and finishing kernels read.
+void need_basis(const std::string &L, const BasisDev &b) {
+  const size_t nsh = size_t(b.nsh);
+  need(L, "shell momenta", b.l, nsh * sizeof(int));
+  need(L, "shell kinds", b.kind, nsh * sizeof(int));
+  need(L, "function offsets", b.off_func, nsh * sizeof(int));
+  need(L, "Cartesian offsets", b.off_cart, nsh * sizeof(int));
+  int lmax = 0;
+  if (inside(b.l, nsh * sizeof(int)))
+    for (size_t k = 0; k < nsh; ++k) lmax = std::max(lmax, b.l[k]);
+  if (lmax > kMaxL) violation(L + ": shell momentum beyond the c2s tables");
+  else {
+    const int nc = (lmax + 1) * (lmax + 2) / 2;
+    need(L, "c2s matrices", b.c2s, size_t(b.c2s_off[lmax] + (2 * lmax + 1) * nc) * sizeof(double));
+  }
+}
+
+// Density preparation, fold and finishing kernels: every array they index,
+// by the sizes their loops run over.
+void check_dens_kernel(const std::string &name, void **args, const Config &c) {
+  auto has = [&](const char *k) { return name.find(k) != std::string::npos; };
+  const std::string L = name.substr(0, name.find('('));
+  const size_t D = sizeof(double);
+  if (has("split_kernel")) {  // (nao, d, ndens, has_anti, ds, da)
+    const size_t nao = size_t(*static_cast<int *>(args[0]));
+    const size_t nd = size_t(*static_cast<int *>(args[2]));
+    const int anti = *static_cast<int *>(args[3]);
+    need(L, "densities", *static_cast<double **>(args[1]), nd * nao * nao * D);
+    need(L, "symmetric part", *static_cast<double **>(args[4]), nd * nao * nao * D);
+    if (anti) need(L, "antisymmetric part", *static_cast<double **>(args[5]), nd * nao * nao * D);
+  } else if (has("to_cart_kernel")) {  // (b, mf, nmats, mc)
+    const BasisDev &b = *static_cast<const BasisDev *>(args[0]);
+    const size_t nm = size_t(*static_cast<int *>(args[2]));
+    need_basis(L, b);
+    need(L, "function-space matrices", *static_cast<double **>(args[1]),
+         nm * size_t(b.nao) * b.nao * D);
+    need(L, "Cartesian matrices", *static_cast<double **>(args[3]),
+         nm * size_t(b.nao_cart) * b.nao_cart * D);
+  } else if (has("block_max_kernel")) {  // (b, x, nmats, out)
+    const BasisDev &b = *static_cast<const BasisDev *>(args[0]);
+    const size_t nm = size_t(*static_cast<int *>(args[2]));
+    need_basis(L, b);
+    need(L, "matrices", *static_cast<double **>(args[1]), nm * size_t(b.nao) * b.nao * D);
+    need(L, "block maxima", *static_cast<double **>(args[3]), nm * size_t(b.nsh) * b.nsh * D);
+  } else if (has("fold_copies_kernel")) {  // (x, n, stride, ncopy)
+    const size_t n = *static_cast<size_t *>(args[1]), stride = *static_cast<size_t *>(args[2]);
+    const int ncopy = *static_cast<int *>(args[3]);
+    if (stride < n) violation(L + ": copy stride below the copy size");
+    need(L, "accumulator copies", *static_cast<double **>(args[0]),
+         (size_t(std::max(ncopy, 1) - 1) * stride + n) * D);
+  } else if (has("jk_finish_kernel")) {  // (b, d, J, K)
+    const BasisDev &b = *static_cast<const BasisDev *>(args[0]);
+    const DensDev &d = *static_cast<const DensDev *>(args[1]);
+    const size_t n2 = size_t(b.nao) * b.nao, nc2 = size_t(b.nao_cart) * b.nao_cart;
+    need_basis(L, b);
+    need(L, "J output", *static_cast<double **>(args[2]), size_t(d.ndens) * n2 * D);
+    double *K = *static_cast<double **>(args[3]);
+    if (d.kc && K) need(L, "K output", K, size_t(d.ndens) * n2 * D);
+    need(L, "J accumulators", d.jc, size_t(d.neff) * nc2 * D);
+    if (d.kc) need(L, "K accumulators", d.kc, size_t(d.neff) * nc2 * D);
+    need(L, "effective-density table", d.eff_src, 2 * size_t(d.neff) * sizeof(int));
+    if (inside(d.eff_src, 2 * size_t(d.neff) * sizeof(int)))
+      for (int e = 0; e < d.neff; ++e)
+        if (d.eff_src[e] < 0 || d.eff_src[e] >= d.ndens) violation(L + ": effective density source out of range");
+  }
+  (void)c;
+}
+
+// One launch of jk_rowacc_batch_kernel(bs, ps, R, sets, dds, seg, nb, queue,
+// chunk, nacc, counters): its device arrays of per-basis views, read at the
+// time of the launch, and every basis's views and quartets.
+void check_batch(const std::string &L, void **args, const Config &c) {
+  const BasisDev *bs = *static_cast<const BasisDev **>(args[0]);
+  const PairsDev *ps = *static_cast<const PairsDev **>(args[1]);
+  const RysDev &R = *static_cast<const RysDev *>(args[2]);
+  const QuartetSet *sets = *static_cast<const QuartetSet **>(args[3]);
+  const DensDev *dds = *static_cast<const DensDev **>(args[4]);
+  const long long *seg = *static_cast<const long long **>(args[5]);
+  const size_t nb = size_t(*static_cast<int *>(args[6]));
+  const unsigned *queue = *static_cast<unsigned **>(args[7]);
+  const long long chunk = *static_cast<long long *>(args[8]);
+  const int nacc = *static_cast<int *>(args[9]);
+  need(L, "basis views", bs, nb * sizeof(BasisDev));
+  need(L, "pair views", ps, nb * sizeof(PairsDev));
+  need(L, "quartet sets", sets, nb * sizeof(QuartetSet));
+  need(L, "density views", dds, nb * sizeof(DensDev));
+  need(L, "set prefix sums", seg, (nb + 1) * sizeof(long long));
+  need(L, "work queue", queue, sizeof(unsigned));
+  if (chunk < 1) violation(L + ": chunk below one quartet");
+  if (!inside(bs, nb * sizeof(BasisDev)) || !inside(ps, nb * sizeof(PairsDev)) ||
+      !inside(sets, nb * sizeof(QuartetSet)) || !inside(dds, nb * sizeof(DensDev)) ||
+      !inside(seg, (nb + 1) * sizeof(long long)))
+    return;
+  if (queue && inside(queue, sizeof(unsigned)) && *queue != 0)
+    violation(L + ": work queue not zero at the launch");
+  int cls[4];
+  std::vector<std::string> rest;
+  if (!class_of(L, cls, rest)) return violation("unparsed batched kernel name " + L);
+  if (seg[0] != 0) violation(L + ": prefix sums do not start at 0");
+  for (size_t m = 0; m < nb; ++m) {
+    if (seg[m + 1] - seg[m] != sets[m].total) {
+      std::ostringstream o;
+      o << L << ": basis " << m << " set total differs from the prefix sums";
+      violation(o.str());
+    }
+    if (sets[m].total <= 0) continue;
+    // the LDS the kernel zeroes and the row sums of this basis
+    const BasisDev &b = bs[m];
+    const int nc = ncart_h(cls[2]), nd = ncart_h(cls[3]);
+    {
+      const int nC = b.lshell_off[cls[2] + 1] - b.lshell_off[cls[2]];
+      const int nD = b.lshell_off[cls[3] + 1] - b.lshell_off[cls[3]];
+      const long long W = (long long)nC * nc + (cls[2] == cls[3] ? 0 : (long long)nD * nd);
+      const long long need_acc =
+          (long long)dds[m].neff * ((long long)ncart_h(cls[0]) * ncart_h(cls[1]) +
+                                     (long long)(ncart_h(cls[0]) + ncart_h(cls[1])) * W);
+      if (need_acc > nacc) {
+        std::ostringstream o;
+        o << L << ": basis " << m << " needs " << need_acc << " accumulator doubles, the launch "
+          << "zeroes " << nacc;
+        violation(o.str());
+      }
+    }
+    void *v[5] = {const_cast<BasisDev *>(&bs[m]), const_cast<PairsDev *>(&ps[m]),
+                  const_cast<RysDev *>(&R), const_cast<QuartetSet *>(&sets[m]),
+                  const_cast<DensDev *>(&dds[m])};
+    // the row-sum checks of the single-basis kernel (the launch's LDS holds
+    // the Rys tables and nacc doubles, the widest basis's row sums)
+    check_jk(L + " basis " + std::to_string(m), cls[0], cls[1], cls[2], cls[3], v, rest, c);
+  }
+}
+
 void run_schwarz(void **args, bool big) {
   // schwarz_kernel(BasisDev, PairsDev, RysDev, int first, int count, double *q);
   // schwarz_big_kernel(int la, int lb, int rb, BasisDev, PairsDev, RysDev,
@@ -411,8 +572,15 @@ extern "C" void replay_reset() {
 
 // ---------------------------------------------------------------------------
 // runtime
+// A call that real HIP refuses (or that would run outside the graph) while
+// a stream captures in thread-local mode.
+static void not_in_capture(const char *what) {
+  if (st().capturing) violation(std::string(what) + " while a stream captures");
+}
+
 extern "C" hipError_t hipMalloc(void **ptr, size_t size) {
   std::lock_guard<std::recursive_mutex> g(g_mu);
+  not_in_capture("hipMalloc");
   ++st().mallocs;
   if (st().fail_malloc_after >= 0 && st().fail_malloc_after-- == 0) {
     *ptr = nullptr;
@@ -429,6 +597,7 @@ extern "C" hipError_t hipMalloc(void **ptr, size_t size) {
 extern "C" hipError_t hipFree(void *ptr) {
   std::lock_guard<std::recursive_mutex> g(g_mu);
   if (!ptr) return hipSuccess;
+  not_in_capture("hipFree");
   auto it = g_alloc.find(reinterpret_cast<uintptr_t>(ptr));
   if (it == g_alloc.end()) {
     violation("hipFree of a pointer that is not a live allocation");
@@ -439,19 +608,23 @@ extern "C" hipError_t hipFree(void *ptr) {
   return hipSuccess;
 }
 extern "C" hipError_t hipMemcpy(void *dst, const void *src, size_t n, hipMemcpyKind) {
+  not_in_capture("hipMemcpy");
   if (n) std::memmove(dst, src, n);
   return hipSuccess;
 }
 extern "C" hipError_t hipMemcpyAsync(void *dst, const void *src, size_t n, hipMemcpyKind,
                                      hipStream_t) {
+  not_in_capture("hipMemcpyAsync (a copy node of host memory the graph keeps reading)");
   if (n) std::memmove(dst, src, n);
   return hipSuccess;
 }
 extern "C" hipError_t hipMemset(void *dst, int v, size_t n) {
+  not_in_capture("hipMemset");
   std::memset(dst, v, n);
   return hipSuccess;
 }
 extern "C" hipError_t hipMemsetAsync(void *dst, int v, size_t n, hipStream_t) {
+  not_in_capture("hipMemsetAsync (a memset node)");
   std::memset(dst, v, n);
   return hipSuccess;
 }
@@ -462,7 +635,10 @@ extern "C" hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned int) {
   return hipSuccess;
 }
 extern "C" hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
-extern "C" hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+extern "C" hipError_t hipStreamSynchronize(hipStream_t) {
+  not_in_capture("hipStreamSynchronize");
+  return hipSuccess;
+}
 extern "C" hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned int) {
   return hipSuccess;
 }
@@ -479,7 +655,10 @@ extern "C" hipError_t hipEventDestroy(hipEvent_t e) {
   return hipSuccess;
 }
 extern "C" hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
-extern "C" hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+extern "C" hipError_t hipEventSynchronize(hipEvent_t) {
+  not_in_capture("hipEventSynchronize");
+  return hipSuccess;
+}
 extern "C" hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) {
   *ms = 0.0f;
   return hipSuccess;
@@ -502,7 +681,10 @@ extern "C" hipError_t hipGetDevice(int *d) {
   *d = 0;
   return hipSuccess;
 }
-extern "C" hipError_t hipDeviceSynchronize(void) { return hipSuccess; }
+extern "C" hipError_t hipDeviceSynchronize(void) {
+  not_in_capture("hipDeviceSynchronize");
+  return hipSuccess;
+}
 extern "C" hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int) {
   switch (a) {
     case hipDeviceAttributeMultiprocessorCount: *v = 256; break;
@@ -522,8 +704,61 @@ extern "C" hipError_t hipGetSymbolAddress(void **p, const void *sym) {
   *p = const_cast<void *>(sym);
   return hipSuccess;
 }
-extern "C" hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+// graphs: a capture records the launches on the capturing stream; the graph
+// is that list, its executable the same list, and each graph launch runs
+// every recorded launch against the memory as it is then
+static void dispatch(const std::string &name, const Config &c, void **args);
+extern "C" hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  if (st().capturing) violation("a capture begun inside another");
+  st().capturing = true;
+  st().capture_stream = s;
+  st().rec = new std::vector<Recorded>();
+  return hipSuccess;
+}
+extern "C" hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *graph) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  if (!st().capturing || s != st().capture_stream) violation("hipStreamEndCapture on a stream that does not capture");
+  st().capturing = false;
+  st().capture_stream = nullptr;
+  *graph = reinterpret_cast<hipGraph_t>(st().rec);
+  st().rec = nullptr;
+  return hipSuccess;
+}
+extern "C" hipError_t hipGraphInstantiate(hipGraphExec_t *exec, hipGraph_t graph,
+                                          hipGraphNode_t *, char *, size_t) {
+  // the executable shares the graph's list: a graph destroyed while its
+  // executable lives is reported at the executable's next launch
+  *exec = reinterpret_cast<hipGraphExec_t>(graph);
+  st().live_graphs.insert(graph);
+  return hipSuccess;
+}
+extern "C" hipError_t hipGraphLaunch(hipGraphExec_t exec, hipStream_t) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  not_in_capture("hipGraphLaunch");
+  ++st().graph_launches;
+  auto *list = reinterpret_cast<std::vector<Recorded> *>(exec);
+  if (!st().live_graphs.count(reinterpret_cast<hipGraph_t>(exec))) {
+    violation("hipGraphLaunch of an executable whose graph is destroyed");
+    return hipErrorInvalidValue;
+  }
+  for (Recorded &r : *list) {
+    if (!r.known) {
+      ++g_other;
+      continue;
+    }
+    dispatch(r.name, r.c, r.args.data());
+  }
+  return hipSuccess;
+}
+extern "C" hipError_t hipGraphDestroy(hipGraph_t graph) {
+  if (!graph) return hipSuccess;
+  st().live_graphs.erase(graph);
+  delete reinterpret_cast<std::vector<Recorded> *>(graph);
+  return hipSuccess;
+}
 extern "C" hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+extern "C" long long replay_graph_launches() { return st().graph_launches; }
 
 // kernel registration and launch
 extern "C" void **__hipRegisterFatBinary(const void *) {
@@ -554,6 +789,27 @@ extern "C" hipError_t __hipPopCallConfiguration(dim3 *grid, dim3 *block, size_t 
   *s = c.stream;
   return hipSuccess;
 }
+// Argument sizes of the kernels a graph launch re-runs (empty: unknown).
+static std::vector<size_t> arg_sizes(const std::string &name) {
+  auto has = [&](const char *k) { return name.find(k) != std::string::npos; };
+  const size_t P = sizeof(void *), I = sizeof(int), D = sizeof(double);
+  const size_t B = sizeof(BasisDev), Pr = sizeof(PairsDev), R = sizeof(RysDev),
+               Q = sizeof(QuartetSet), Dd = sizeof(DensDev);
+  if (has("fill_kernel")) return {P, sizeof(size_t), D};
+  if (has("split_kernel")) return {I, P, I, I, P, P};
+  if (has("to_cart_kernel") || has("block_max_kernel")) return {B, P, I, P};
+  if (has("fold_copies_kernel")) return {P, sizeof(size_t), sizeof(size_t), I};
+  if (has("jk_finish_kernel")) return {B, sizeof(DensDev), P, P};
+  if (has("jk_rowacc_batch_kernel<"))
+    return {P, P, R, P, P, P, I, P, sizeof(long long), I, P};
+  if (has("jk_rowacc_kernel<") || has("jk_reg_kernel<") || has("jk_kernel<") ||
+      has("jk_hrr_kernel<"))
+    return {B, Pr, R, Q, Dd, D, P};
+  if (has("jk_big_kernel")) return {I, I, I, I, I, B, Pr, R, Q, Dd, D, P, P, sizeof(size_t), P};
+  if (has("jk_rt_kernel<")) return {I, I, I, I, I, I, B, Pr, R, Q, Dd, D, P, P};
+  return {};
+}
+
 extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void **args,
                                       size_t shm, hipStream_t s) {
   std::lock_guard<std::recursive_mutex> g(g_mu);
@@ -564,6 +820,35 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
     st().last = hipErrorLaunchFailure;
     return hipErrorLaunchFailure;
   }
+  if (st().capturing) {
+    if (s != st().capture_stream) {
+      // thread-local capture lets this run at once, outside the graph
+      violation(name.substr(0, name.find('(')) + ": launched on another stream while a stream captures");
+      dispatch(name, c, args);
+      return hipSuccess;
+    }
+    Recorded r;
+    r.name = name;
+    r.c = c;
+    const std::vector<size_t> sz = arg_sizes(name);
+    r.known = !sz.empty();
+    for (size_t k = 0; k < sz.size(); ++k) {
+      const char *a = static_cast<const char *>(args[k]);
+      r.store.emplace_back(a, a + sz[k]);
+    }
+    st().rec->push_back(std::move(r));
+    // the argument pointers into the copies (stable once the list is built)
+    Recorded &last = st().rec->back();
+    for (auto &v : last.store) last.args.push_back(v.data());
+    return hipSuccess;
+  }
+  dispatch(name, c, args);
+  return hipSuccess;
+}
+
+static void dispatch(const std::string &name, const Config &c, void **args) {
+  const dim3 block = c.block;
+  const size_t shm = c.shm;
   if (block.x * block.y * block.z > 1024) violation(name + ": more than 1024 lanes");
   if (shm > 160 * 1024) violation(name + ": more than 160 KB of dynamic LDS");
   auto has = [&](const char *k) { return name.find(k) != std::string::npos; };
@@ -583,6 +868,9 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
     const double v = *static_cast<double *>(args[2]);
     if (!inside(x, n * sizeof(double))) violation("fill_kernel outside every live allocation");
     else for (size_t k = 0; k < n; ++k) x[k] = v;
+  } else if (has("split_kernel") || has("to_cart_kernel") || has("block_max_kernel") ||
+             has("fold_copies_kernel") || has("jk_finish_kernel")) {
+    check_dens_kernel(name, args, c);
   } else if (has("jk_rowacc_kernel<") || has("jk_reg_kernel<") || has("jk_kernel<") ||
              has("jk_hrr_kernel<")) {
     int cls[4];
@@ -591,6 +879,11 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
       violation("unparsed J/K kernel name " + name);
     else
       check_jk(name, cls[0], cls[1], cls[2], cls[3], args, rest, c);
+  } else if (has("jk_rowacc_batch_kernel<")) {
+    check_batch(name, args, c);
+    // the kernel's workgroups drain the queue: every chunk taken once
+    unsigned *queue = *static_cast<unsigned **>(args[7]);
+    if (inside(queue, sizeof(unsigned))) *queue += c.grid.x;
   } else if (has("jk_big_kernel") || has("jk_rt_kernel<")) {
     check_jk_runtime_l(name, args, c);
   } else if (has("eri_kernel<") || has("eri_big_kernel")) {
@@ -613,7 +906,7 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
       std::vector<std::string> rest;
       if (!class_of(name, cls, rest)) {
         violation("unparsed ERI kernel name " + name);
-        return hipSuccess;
+        return;
       }
       la = cls[0], lb = cls[1], lc = cls[2], ld = cls[3];
       v = args;
@@ -645,5 +938,4 @@ extern "C" hipError_t hipLaunchKernel(const void *f, dim3 grid, dim3 block, void
   } else {
     ++g_other;
   }
-  return hipSuccess;
 }

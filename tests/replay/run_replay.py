@@ -9,7 +9,9 @@ Usage: python tests/replay/run_replay.py WORKLOAD [cut ...]
             density) or grad (the nao 390 s,p,d molecule: the adjoint
             forces' launches first, then the J/K builds) or eri (the dense
             tensor of the 10-atom nao 50 molecule) or batch (the batched
-            plans of six small molecules through ig_batch_check);
+            plans of six small molecules through ig_batch_check) or
+            batchgraph (the same molecules through the batched graph: its
+            capture and every graph launch);
   cut       FP32 cut of a mixed build (0: FP64); the default sequence is
             FP64, mixed below 1e-6, mixed below 1e-8, FP64 again, all on one
             contractor (the re-planning between them is part of the check).
@@ -36,6 +38,87 @@ sys.path.insert(0, ROOT)
 import integral_amd as ig  # noqa: E402
 from integral_amd import synthetic  # noqa: E402
 from oracle import oracle as O  # noqa: E402
+
+
+def batch_graph(lib):
+    """The batched graph (compiled in for this build) over the sequence of
+    the batched GPU tests: one capture and four graph launches with
+    allocations and frees between them; re-captures for a new threshold
+    with J only and new output buffers, for FP32 below 1e-8 and back to
+    FP64; autotune and a loaded schedule, each followed by a re-capture.
+    Each call prints one JSON line.  While a stream captures, launches are
+    recorded, not run, with a copy of their arguments; every graph launch
+    runs the recorded list against the memory as it is at that launch, so a
+    pointer baked into the graph that has gone stale is reported there."""
+    os.environ["REPLAY_UNIT_SCHWARZ"] = "1"
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_gpu_batch import batch_molecules
+    vp = ctypes.c_void_p
+    lib.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    lib.hipFree.argtypes = [vp]
+    lib.ig_batch_jk_device.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int, vp, vp, vp]
+    lib.ig_batch_autotune.argtypes = [vp, ctypes.c_size_t, vp, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_int, vp]
+    lib.replay_graph_launches.restype = ctypes.c_longlong
+    mols = batch_molecules()
+    bases = [synthetic.to_basis(s) for s, _ in mols]
+
+    def dev(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        p = vp()
+        assert lib.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+        ctypes.memmove(p, a.ctypes.data, a.nbytes)
+        return p
+
+    def arr(ps):
+        return (vp * len(ps))(*[p.value for p in ps])
+
+    def per_molecule(tau):  # the reference builds between the batched calls
+        for (_, D), b in zip(mols, bases):
+            b.direct_contractor().jk([D], threshold=tau)
+
+    def call(step, bc, dens, J, K, tau):
+        lib.replay_reset()
+        rc = lib.ig_batch_jk_device(bc._h, arr(dens), tau, 1 if K is not None else 0, arr(J),
+                                    arr(K) if K is not None else None, None)
+        n = lib.replay_violation_count()
+        print(json.dumps({"workload": "batchgraph", "step": step, "rc": rc,
+                          "message": lib.ig_last_error_message().decode() if rc else "",
+                          "graph_launches": lib.replay_graph_launches(),
+                          "jk_launches": lib.replay_jk_launches(),
+                          "quartets_checked": lib.replay_quartets(), "violations": n,
+                          "first": [lib.replay_violation(k).decode() for k in range(min(n, 5))]}),
+              flush=True)
+
+    zeros = [np.zeros_like(D) for _, D in mols]
+    bc = ig.BatchContractor(bases)
+    per_molecule(1e-12)
+    dens = [dev(D) for _, D in mols]
+    J = [dev(z) for z in zeros]
+    K = [dev(z) for z in zeros]
+    for rep in range(4):
+        call("replay %d" % rep, bc, dens, J, K, 1e-12)
+        tmp = vp()  # other work between the replays
+        lib.hipMalloc(ctypes.byref(tmp), 512 * 512 * 8)
+        lib.hipFree(tmp)
+    per_molecule(1e-9)
+    J2 = [dev(z) for z in zeros]
+    for rep in range(2):
+        call("J only, tau 1e-9, new buffers %d" % rep, bc, dens, J2, None, 1e-9)
+    for p in J2:
+        lib.hipFree(p)
+    bc.set_mixed_precision(1e-8)
+    call("FP32 below 1e-8", bc, dens, J, K, 1e-12)
+    bc.set_mixed_precision(0.0)
+    call("FP64 again", bc, dens, J, K, 1e-12)
+    m = int(np.argmax([D.shape[0] for _, D in mols]))
+    rc = lib.ig_batch_autotune(bc._h, m, dens[m], 1e-12, 1, 1, None)
+    assert rc == 0, lib.ig_last_error_message()
+    for rep in range(2):
+        call("after autotune %d" % rep, bc, dens, J, K, 1e-12)
+    sched = bases[0].direct_contractor().schedule()
+    bc.set_schedule([dict(e, variant=0) for e in sched])
+    call("loaded schedule", bc, dens, J, K, 1e-12)
 
 
 def main():
@@ -66,6 +149,9 @@ def main():
             print(json.dumps({"workload": workload, "tau": tau, "exchange": want_k,
                               "fp32_below": cut, "check": msg,
                               "violations": lib.replay_violation_count()}), flush=True)
+        return
+    if workload == "batchgraph":
+        batch_graph(lib)
         return
     if workload == "direct":
         shells, dens, tau = synthetic.direct_jk_config()

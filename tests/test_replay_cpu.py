@@ -97,6 +97,21 @@ def test_batched_plans_pass_the_library_host_walk(replay_lib):
         assert r["check"] == "" and r["violations"] == 0, r
 
 
+def test_batched_graph_launches_stay_in_range_over_replays_and_recaptures(replay_lib):
+    # the batched graph path (compiled into this build only) over the batched
+    # GPU tests' sequence: every launch recorded at capture is run again, with
+    # its baked arguments, at each graph launch against the live allocations;
+    # no capture-unsafe call and no launch off the capturing stream
+    recs = run("batchgraph", [])
+    assert [r["graph_launches"] for r in recs] == list(range(1, 12))
+    for r in recs:
+        assert r["rc"] == 0, r["message"]
+        assert r["violations"] == 0, (r["step"], r["first"])
+        assert r["jk_launches"] > 100 and r["quartets_checked"] > 5e5
+    # replays of one graph check the same launches
+    assert len({(r["jk_launches"], r["quartets_checked"]) for r in recs[:4]}) == 1
+
+
 def test_autotuner_trial_launches_stay_in_range(replay_lib):
     # every kernel variant of every class pair at full size
     recs = run("direct", [0.0], env={"REPLAY_AUTOTUNE": "1"})
