@@ -69,6 +69,11 @@ struct Recorded {
   std::string name;
   Config c;
   bool known = false;
+  // a launch of the host-executed kernels (tests/emu): its job, run at each
+  // graph launch
+  void (*run)(void *) = nullptr;
+  void (*del)(void *) = nullptr;
+  void *job = nullptr;
   std::vector<std::vector<char>> store;  // one copy per argument
   std::vector<void *> args;
 };
@@ -586,9 +591,11 @@ extern "C" hipError_t hipMalloc(void **ptr, size_t size) {
     *ptr = nullptr;
     return hipErrorOutOfMemory;
   }
+  // exactly n bytes (256-aligned), so a sanitizer's redzone starts at the
+  // allocation's end
   const size_t n = size ? size : 8;
-  void *p = std::aligned_alloc(256, (n + 255) / 256 * 256);
-  if (!p) return hipErrorOutOfMemory;
+  void *p = nullptr;
+  if (posix_memalign(&p, 256, n) != 0) return hipErrorOutOfMemory;
   std::memset(p, 0, n);
   g_alloc[reinterpret_cast<uintptr_t>(p)] = Alloc{n};
   *ptr = p;
@@ -687,7 +694,13 @@ extern "C" hipError_t hipDeviceSynchronize(void) {
 }
 extern "C" hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int) {
   switch (a) {
-    case hipDeviceAttributeMultiprocessorCount: *v = 256; break;
+    case hipDeviceAttributeMultiprocessorCount: {
+      // REPLAY_CU_COUNT: fewer CUs, so persistent grids stay small when the
+      // kernels are executed on the host
+      const char *e = std::getenv("REPLAY_CU_COUNT");
+      *v = e ? std::atoi(e) : 256;
+      break;
+    }
     case hipDeviceAttributeMaxSharedMemoryPerBlock: *v = 160 * 1024; break;
     case hipDeviceAttributeMaxSharedMemoryPerMultiprocessor: *v = 160 * 1024; break;
     default: *v = 0;
@@ -743,6 +756,10 @@ extern "C" hipError_t hipGraphLaunch(hipGraphExec_t exec, hipStream_t) {
     return hipErrorInvalidValue;
   }
   for (Recorded &r : *list) {
+    if (r.run) {
+      r.run(r.job);
+      continue;
+    }
     if (!r.known) {
       ++g_other;
       continue;
@@ -754,6 +771,8 @@ extern "C" hipError_t hipGraphLaunch(hipGraphExec_t exec, hipStream_t) {
 extern "C" hipError_t hipGraphDestroy(hipGraph_t graph) {
   if (!graph) return hipSuccess;
   st().live_graphs.erase(graph);
+  for (Recorded &r : *reinterpret_cast<std::vector<Recorded> *>(graph))
+    if (r.del) r.del(r.job);
   delete reinterpret_cast<std::vector<Recorded> *>(graph);
   return hipSuccess;
 }
@@ -938,4 +957,30 @@ static void dispatch(const std::string &name, const Config &c, void **args) {
   } else {
     ++g_other;
   }
+}
+
+// Hooks of the host-executed kernels (tests/emu/emu_core.cpp): a launch while
+// the stream captures is recorded as a job that every graph launch runs.
+extern "C" int replay_emu_capturing(hipStream_t s) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  if (!st().capturing) return 0;
+  if (s != st().capture_stream) {
+    violation("a launch on another stream while a stream captures");
+    return 0;
+  }
+  return 1;
+}
+extern "C" void replay_emu_record(void (*run)(void *), void (*del)(void *), void *job,
+                                  const char *name) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  Recorded r;
+  r.name = name;
+  r.run = run;
+  r.del = del;
+  r.job = job;
+  st().rec->push_back(std::move(r));
+}
+extern "C" void replay_emu_violation(const char *what) {
+  std::lock_guard<std::recursive_mutex> g(g_mu);
+  violation(what);
 }
