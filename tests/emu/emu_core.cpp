@@ -12,6 +12,7 @@
 #include <cxxabi.h>
 #include <dlfcn.h>
 #include <map>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -170,6 +171,49 @@ void resolve(std::vector<Lane *> &grp, int wave_base, int n) {
   }
 }
 
+// Advance wavefront w by one segment: its runnable lanes run to their next
+// wave operation, barrier or end; then a wave operation every stopped live
+// lane of it waits at is resolved (part of the wavefront when it diverged).
+// Returns whether anything moved.
+bool step_wave(int w, int n) {
+  const int b0 = w * 64, b1 = std::min(n, b0 + 64);
+  bool moved = false;
+  for (int t = b0; t < b1; ++t)
+    if (g_lanes[t].state == kRun) {
+      run_lane(g_lanes[t]);
+      moved = true;
+    }
+  int live = 0;
+  std::vector<Lane *> grp;
+  for (int t = b0; t < b1; ++t) {
+    Lane &L = g_lanes[t];
+    if (L.state != kDone) ++live;
+    if (L.state == kWave) grp.push_back(&L);
+  }
+  if (grp.empty()) return moved;
+  // one operation at a time: the first lane's; others wait
+  std::vector<Lane *> same;
+  for (Lane *L : grp)
+    if (L->op == grp[0]->op && L->width == grp[0]->width) same.push_back(L);
+  if (same.size() != grp.size()) note("different wave operations pending in one wavefront");
+  if (int(same.size()) != live) note("wave operation with part of the wavefront (divergent)");
+  resolve(same, b0, n);
+  for (Lane *L : same) L->state = kRun;
+  return true;
+}
+
+// EMU_SCHEDULE=random:SEED: each step advances one wavefront, drawn at random
+// among those that can move, so wavefronts drift apart between barriers as
+// on the hardware (default: every wavefront one segment per pass, in order).
+std::mt19937_64 *schedule_rng() {
+  static std::mt19937_64 *rng = [] {
+    const char *e = std::getenv("EMU_SCHEDULE");
+    if (!e || std::strncmp(e, "random:", 7) != 0) return (std::mt19937_64 *)nullptr;
+    return new std::mt19937_64(std::strtoull(e + 7, nullptr, 10));
+  }();
+  return rng;
+}
+
 void run_block(const Job &J) {
   const int n = int(J.block.x * J.block.y * J.block.z);
   if (int(g_lanes.size()) < n) g_lanes.resize(n);
@@ -181,31 +225,23 @@ void run_block(const Job &J) {
     L.tz = t / (J.block.x * J.block.y);
   }
   const int nw = (n + 63) / 64;
+  std::mt19937_64 *rng = schedule_rng();
+  std::vector<int> cand;
   for (;;) {
-    for (int t = 0; t < n; ++t)
-      if (g_lanes[t].state == kRun) run_lane(g_lanes[t]);
-    bool progress = false;
-    for (int w = 0; w < nw; ++w) {
-      const int b0 = w * 64, b1 = std::min(n, b0 + 64);
-      int live = 0;
-      std::vector<Lane *> grp;
-      for (int t = b0; t < b1; ++t) {
-        Lane &L = g_lanes[t];
-        if (L.state != kDone) ++live;
-        if (L.state == kWave) grp.push_back(&L);
-      }
-      if (grp.empty()) continue;
-      // one operation at a time: the first lane's; others wait
-      std::vector<Lane *> same;
-      for (Lane *L : grp)
-        if (L->op == grp[0]->op && L->width == grp[0]->width) same.push_back(L);
-      if (same.size() != grp.size()) note("different wave operations pending in one wavefront");
-      if (int(same.size()) != live) note("wave operation with part of the wavefront (divergent)");
-      resolve(same, b0, n);
-      for (Lane *L : same) L->state = kRun;
-      progress = true;
+    bool moved = false;
+    if (rng) {
+      cand.clear();
+      for (int w = 0; w < nw; ++w)
+        for (int t = w * 64; t < std::min(n, w * 64 + 64); ++t)
+          if (g_lanes[t].state == kRun || g_lanes[t].state == kWave) {
+            cand.push_back(w);
+            break;
+          }
+      if (!cand.empty()) moved = step_wave(cand[(*rng)() % cand.size()], n);
+    } else {
+      for (int w = 0; w < nw; ++w) moved = step_wave(w, n) || moved;
     }
-    if (progress) continue;
+    if (moved) continue;
     int live = 0, bar = 0;
     for (int t = 0; t < n; ++t) {
       if (g_lanes[t].state != kDone) ++live;
