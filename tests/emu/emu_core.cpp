@@ -306,6 +306,11 @@ void *dyn_lds() { return g_lds; }
 unsigned xcc() { return g_block_linear % 8; }
 
 void submit(Job *j, hipStream_t s) {
+  // EMU_TRACE: each launch's kernel and shape on stderr
+  static const bool trace = std::getenv("EMU_TRACE") != nullptr;
+  if (trace)
+    std::fprintf(stderr, "emu launch %s grid %u block %u lds %zu\n", j->name.c_str(), j->grid.x,
+                 j->block.x, j->shm);
   if (replay_emu_capturing(s)) {
     replay_emu_record(&run_job, &del_job, j, j->name.c_str());
     return;

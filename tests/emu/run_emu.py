@@ -12,11 +12,15 @@ runtime preloaded: LD_PRELOAD=libasan.so, ASAN_OPTIONS=detect_leaks=0)
   batchgraph  the same molecules through the batched graph: a capture, a
               second graph launch, re-captures for J only at tau 1e-9, for
               FP32 below 1e-8 and back to FP64; every molecule's J and K
-              after each graph launch;
+              after each graph launch (EMU_BATCH_SHORT: the first two
+              graph launches only);
   eri         dense ERI tensors: the s..f reference basis and three-shell
               bases with a g, h and i shell (the runtime-l kernels);
   grad        direct gradients: the s..f reference basis (Cartesian and
-              spherical) and a three-shell basis with a g shell.
+              spherical) and three-shell bases with a g and an h shell;
+  highl       direct J/K of three-shell bases with a g, h and i shell (the
+              runtime-l per-quartet kernel, and the sliced one for the
+              (ii|ii) block of 614656 elements), FP64 and FP32 below 1e-7.
 """
 import ctypes
 import json
@@ -119,6 +123,8 @@ def batch_graph():
     K = [dev(np.zeros_like(D)) for _, D in mols]
     for rep in range(2):
         call("replay %d" % rep, bc, dens, J, K, 1e-12)
+    if os.environ.get("EMU_BATCH_SHORT"):  # the capture and one replay only
+        return
     J2 = [dev(np.zeros_like(D)) for _, D in mols]
     call("J only, tau 1e-9", bc, dens, J2, None, 1e-9)
     bc.set_mixed_precision(1e-8)
@@ -149,7 +155,7 @@ def eri():
 def grad():
     import bases as B
     cases = [("s..f cart", B.grad_basis(False)), ("s..f sph", B.grad_basis(True)),
-             ("l=4", three_shell(4, True))]
+             ("l=4", three_shell(4, True)), ("l=5", three_shell(5, True))]
     for name, shells in cases:
         n = B.nfunc(shells)
         coul, exch = B.grad_terms(n)
@@ -164,5 +170,27 @@ def grad():
              scale=float(np.max(np.abs(o))))
 
 
+def highl():
+    rng = np.random.default_rng(20261203)
+    for l in (4, 5, 6):
+        for sph in (True, False):
+            shells = three_shell(l, sph)
+            b = synthetic.to_basis(shells)
+            n = b.nao()
+            C = rng.normal(size=(n, 2)) * 0.1
+            D = C @ C.T
+            oJ, oK = O.Basis(shells).direct_jk([D], 1e-12)
+            dc = b.direct_contractor()
+            for cut in (0.0, 1e-7):
+                dc.set_mixed_precision(cut)
+                lib.replay_reset()
+                t = time.time()
+                J, K = dc.jk([D], threshold=1e-12)
+                emit(workload="highl", l=l, sph=sph, fp32_below=cut,
+                     seconds=round(time.time() - t, 2), dJ=maxdiff(J[0], oJ[0]),
+                     dK=maxdiff(K[0], oK[0]), scale=float(np.max(np.abs(oJ[0]))))
+
+
 if __name__ == "__main__":
-    {"direct": direct, "batchgraph": batch_graph, "eri": eri, "grad": grad}[sys.argv[1]]()
+    {"direct": direct, "batchgraph": batch_graph, "eri": eri, "grad": grad,
+     "highl": highl}[sys.argv[1]]()

@@ -6,8 +6,9 @@ lanes), with host AddressSanitizer on every global and LDS access, the host
 code and the runtime stand-in of tests/replay around them.  Results are
 compared with the oracle at the GPU tests' tolerances: direct J/K of the
 batched tests' six molecules (FP64 and FP32 below 1e-6), the batched graph
-over capture, replay and re-captures, dense ERI up to l = 6 (runtime-l
-kernels) and direct gradients."""
+over capture, replay and re-captures (also under a random wavefront
+schedule), direct J/K, dense ERI and gradients with g, h and i shells (the
+runtime-l kernels) and the s..f gradients."""
 import json
 import os
 import shutil
@@ -72,19 +73,30 @@ def test_dense_eri_on_host_executed_kernels_matches_oracle(emu_env):
 
 def test_gradients_on_host_executed_kernels_match_oracle(emu_env):
     recs = run(emu_env, "grad")
-    assert len(recs) == 3
+    assert [r["basis"] for r in recs] == ["s..f cart", "s..f sph", "l=4", "l=5"]
     for r in recs:
         assert r["violations"] == 0, r["first"]
         assert r["dF"] < 1e-12, r
+
+
+def test_runtime_l_jk_on_host_executed_kernels_matches_oracle(emu_env):
+    # g, h and i shells: the per-quartet runtime-l kernel, and the sliced one
+    # for the (ii|ii) block (614656 elements, above the 262144 split)
+    recs = run(emu_env, "highl")
+    assert len(recs) == 12
+    for r in recs:
+        assert r["violations"] == 0, r["first"]
+        bar = 1e-12 if r["fp32_below"] == 0.0 else 1e-9
+        assert r["dJ"] < bar and r["dK"] < bar, r
 
 
 def test_batched_graph_under_a_random_wavefront_schedule(emu_env):
     # wavefronts advanced one at a time in a seeded random order, so they
     # drift apart between barriers as on the hardware: the LDS row sums,
     # flushes and work-queue reads still give the oracle's J and K
-    env = dict(emu_env, EMU_SCHEDULE="random:7")
+    env = dict(emu_env, EMU_SCHEDULE="random:7", EMU_BATCH_SHORT="1")
     recs = run(env, "batchgraph")
-    assert len(recs) == 5
+    assert [r["graph_launches"] for r in recs] == [1, 2]
     for r in recs:
         assert r["rc"] == 0 and r["violations"] == 0, r
         bar = 1e-9 if "FP32" in r["step"] else 1e-12
