@@ -11,7 +11,8 @@ runtime preloaded: LD_PRELOAD=libasan.so, ASAN_OPTIONS=detect_leaks=0)
               with FP32 below 1e-6;
   batchgraph  the same molecules through the batched graph: a capture, a
               second graph launch, re-captures for J only at tau 1e-9, for
-              FP32 below 1e-8 and back to FP64; every molecule's J and K
+              FP32 below 1e-8 and back to FP64, after autotune and with
+              every class on the team kernels; every molecule's J and K
               after each graph launch (EMU_BATCH_SHORT: the first two
               graph launches only);
   eri         dense ERI tensors: the s..f reference basis and three-shell
@@ -131,6 +132,16 @@ def batch_graph():
     call("FP32 below 1e-8", bc, dens, J, K, 1e-12)
     bc.set_mixed_precision(0.0)
     call("FP64 again", bc, dens, J, K, 1e-12)
+    # the tuned schedule, then every class on the team kernels (per-molecule
+    # launches inside the graph), each captured again
+    lib.ig_batch_autotune.argtypes = [vp, ctypes.c_size_t, vp, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_int, vp]
+    big = int(np.argmax([D.shape[0] for _, D in mols]))
+    assert lib.ig_batch_autotune(bc._h, big, dens[big], 1e-12, 1, 1, None) == 0
+    call("after autotune", bc, dens, J, K, 1e-12)
+    sched = synthetic.to_basis(mols[0][0]).direct_contractor().schedule()
+    bc.set_schedule([dict(e, variant=0) for e in sched])
+    call("team kernels schedule", bc, dens, J, K, 1e-12)
 
 
 def three_shell(l, sph):

@@ -56,7 +56,7 @@ def test_batched_graph_on_host_executed_kernels_matches_oracle(emu_env):
     # the batched graph's own kernel (work-queue chunks over the six
     # molecules, nao 1-126) over graph launches and re-captures
     recs = run(emu_env, "batchgraph")
-    assert [r["graph_launches"] for r in recs] == [1, 2, 3, 4, 5]
+    assert [r["graph_launches"] for r in recs] == [1, 2, 3, 4, 5, 6, 7]
     for r in recs:
         assert r["rc"] == 0 and r["violations"] == 0, r
         bar = 1e-9 if "FP32" in r["step"] else 1e-12
