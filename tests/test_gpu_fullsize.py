@@ -14,6 +14,11 @@ sample of elements and 32 bilinear probes u^T M v (each sums every element).
   quartet count exact; then the mixed-precision
   build (FP32 below Q_ij Q_kl = 1e-8) with its error against the oracle
   stated and bounded.
+- nao 390, 30 atoms, spherical s,s,p,p,d (the energy-and-forces workload):
+  the adjoint gradient of E = 1/2 <D, J[D]> - 1/4 <D, K[D]> at tau 1e-12
+  against the oracle's forces over every canonical quartet
+  (tests/golden/gen_grad_oracle.py), per force component at 1e-10 of the
+  largest force, and the forces summing to zero.
 - nao 565 with g shells and an l = 6 shell on every fourth atom, one
   density: the FP64 build (Rys rules up to 13 roots) at the same bar.  Its
   fixture comes from the oracle run on the shells ordered so that every
@@ -29,6 +34,7 @@ import numpy as np
 import pytest
 import torch
 
+import integral_amd as ig
 from integral_amd import synthetic
 
 pytestmark = pytest.mark.gpu
@@ -112,3 +118,24 @@ def test_direct_jk_high_l_nao565_matches_oracle():
     assert abs(st[0] - g["quartets"]) <= 2
     check(J[0], g["J"][0], n, g["probe_seed"], 1e-12)
     check(K[0], g["K"][0], n, g["probe_seed"] + 1, 1e-12)
+
+
+def gradient_error(shells, D, rec):
+    """Forces of the direct gradient contractor for the record's energy
+    terms and threshold; (max |F - F_oracle| / max |F_oracle|, max |sum F|)."""
+    gc = synthetic.to_basis(shells).direct_grad_contractor()
+    dens = gc.density([ig.GradDensityTerm(rec["coulomb"], D, D)],
+                      [ig.GradDensityTerm(rec["exchange"], D, D)])
+    ref = np.array(rec["forces"])
+    F = np.asarray(gc.contract(dens, threshold=rec["tau"])).reshape(ref.shape)
+    return (float(np.max(np.abs(F - ref)) / np.max(np.abs(ref))),
+            float(np.max(np.abs(F.sum(axis=0)))))
+
+
+def test_direct_gradient_nao390_matches_oracle():
+    rec = load("grad_c4_oracle.json")
+    shells, D = synthetic.gradient_config()
+    note("direct gradient nao 390 against the oracle's forces")
+    err, drift = gradient_error(shells, D, rec)
+    note("max |dF| / max |F| %.3e, max |sum F| %.3e" % (err, drift))
+    assert err < 1e-10 and drift < 1e-9

@@ -307,3 +307,21 @@ def test_direct_grad_screening_error_bounded():
     exact = ob.direct_grad(t, t, 0.0)
     scr = ob.direct_grad(t, t, 1e-12)
     assert np.max(np.abs(exact - scr)) < 100 * 1e-12
+
+
+def test_full_size_gradient_fixture_is_consistent():
+    """tests/golden/grad_c4_oracle.json (the oracle's forces for the nao 390
+    energy-and-forces workload): 30 atoms of 3 components, translation
+    invariant (forces sum to zero, as test_direct_grad_invariants shows for
+    the oracle at small size), and made for the workload's energy terms."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "grad_c4_oracle.json")
+    with open(path) as f:
+        rec = json.load(f)
+    F = np.array(rec["forces"])
+    assert F.shape == (30, 3)
+    assert (rec["coulomb"], rec["exchange"], rec["tau"]) == (0.5, -0.25, 1e-12)
+    assert np.max(np.abs(F.sum(axis=0))) < 1e-11 * max(1.0, np.max(np.abs(F)))
+    assert np.max(np.abs(F)) > 0.1
