@@ -21,7 +21,9 @@ runtime preloaded: LD_PRELOAD=libasan.so, ASAN_OPTIONS=detect_leaks=0)
               spherical) and three-shell bases with a g and an h shell;
   highl       direct J/K of three-shell bases with a g, h and i shell (the
               runtime-l per-quartet kernel, and the sliced one for the
-              (ii|ii) block of 614656 elements), FP64 and FP32 below 1e-7.
+              (ii|ii) block of 614656 elements), FP64 and FP32 below 1e-7;
+  scratch     J/K through a worker's workspace and through the
+              no-allocation entry (two densities, then one).
 """
 import ctypes
 import json
@@ -202,6 +204,52 @@ def highl():
                      dK=maxdiff(K[0], oK[0]), scale=float(np.max(np.abs(oJ[0]))))
 
 
+def scratch():
+    """The J/K builds through a worker's own workspace and through the
+    no-allocation entry (views carved from the library's fixed device
+    array): two molecules, two densities and then one, against the oracle."""
+    from test_gpu_batch import batch_molecules
+    vp = ctypes.c_void_p
+    lib.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    lib.ig_direct_jk_device_noalloc.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_double,
+                                                ctypes.c_int, vp, vp, ctypes.c_int, vp]
+    lib.replay_malloc_calls.restype = ctypes.c_longlong
+    rng = np.random.default_rng(20261204)
+    for m in (0, 5):
+        shells, D = batch_molecules()[m]
+        n = D.shape[0]
+        C = rng.normal(size=(n, 3)) * 0.1
+        dens = [D, C @ C.T]
+        oJ, oK = O.Basis(shells).direct_jk(dens, 1e-12)
+        dc = synthetic.to_basis(shells).direct_contractor()
+        ws = ig.DirectWorkspace()
+        lib.replay_reset()
+        J, K = dc.jk(list(dens), threshold=1e-12, workspace=ws)
+        emit(workload="scratch", path="workspace", molecule=m,
+             dJ=max(maxdiff(J[k], oJ[k]) for k in range(2)),
+             dK=max(maxdiff(K[k], oK[k]) for k in range(2)))
+        for nd in (2, 1):
+            d = np.ascontiguousarray(np.stack(dens[:nd]), dtype=np.float64)
+            bufs = []
+            for k in range(3):
+                p = vp()
+                assert lib.hipMalloc(ctypes.byref(p), d.nbytes) == 0
+                z = d if k == 0 else np.zeros_like(d)
+                ctypes.memmove(p, z.ctypes.data, z.nbytes)
+                bufs.append(p)
+            lib.replay_reset()
+            rc = lib.ig_direct_jk_device_noalloc(dc._h, bufs[0], nd, 1e-12, 1, bufs[1], bufs[2],
+                                                 1, None)
+            allocs = lib.replay_malloc_calls()
+            out = [np.empty_like(d), np.empty_like(d)]
+            for k in range(2):
+                ctypes.memmove(out[k].ctypes.data, bufs[k + 1], d.nbytes)
+            emit(workload="scratch", path="noalloc", molecule=m, ndens=nd, rc=rc,
+                 device_allocations=allocs,
+                 dJ=max(maxdiff(out[0][k], oJ[k]) for k in range(nd)),
+                 dK=max(maxdiff(out[1][k], oK[k]) for k in range(nd)))
+
+
 if __name__ == "__main__":
     {"direct": direct, "batchgraph": batch_graph, "eri": eri, "grad": grad,
-     "highl": highl}[sys.argv[1]]()
+     "highl": highl, "scratch": scratch}[sys.argv[1]]()

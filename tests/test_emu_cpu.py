@@ -101,3 +101,18 @@ def test_batched_graph_under_a_random_wavefront_schedule(emu_env):
         assert r["rc"] == 0 and r["violations"] == 0, r
         bar = 1e-9 if "FP32" in r["step"] else 1e-12
         assert r["dJ"] < bar and r["dK"] < bar, r
+
+
+def test_workspace_and_no_allocation_entry_on_host_executed_kernels(emu_env):
+    # the no-allocation entry twice per molecule (two densities, then one)
+    # and on a second molecule after the first: every view carved from the
+    # library's fixed array, J and K at the oracle's values, no allocation
+    recs = run(emu_env, "scratch")
+    assert [(r["path"], r["molecule"]) for r in recs] == [
+        ("workspace", 0), ("noalloc", 0), ("noalloc", 0),
+        ("workspace", 5), ("noalloc", 5), ("noalloc", 5)]
+    for r in recs:
+        assert r["violations"] == 0, r["first"]
+        assert r["dJ"] < 1e-12 and r["dK"] < 1e-12, r
+        if r["path"] == "noalloc":
+            assert r["rc"] == 0 and r["device_allocations"] == 0, r
