@@ -116,3 +116,15 @@ def test_workspace_and_no_allocation_entry_on_host_executed_kernels(emu_env):
         assert r["dJ"] < 1e-12 and r["dK"] < 1e-12, r
         if r["path"] == "noalloc":
             assert r["rc"] == 0 and r["device_allocations"] == 0, r
+
+
+def test_gpu_tests_of_the_next_rows_pass_on_host_executed_kernels(emu_env):
+    # the GPU test files of density fitting, the packed ERI and EriBuilder,
+    # and the erf operator (closed forms, the omega -> infinity limit, the
+    # builder's ranges) run unchanged against the host-executed library
+    out = subprocess.run([sys.executable, os.path.join(EMU, "run_gpu_tests.py"),
+                          "tests/test_gpu_df.py", "tests/test_gpu_packed.py",
+                          "tests/test_gpu_erf.py"], capture_output=True, text=True,
+                         timeout=1500, env=emu_env)
+    assert out.returncode == 0, out.stdout[-3000:] + out.stderr[-2000:]
+    assert "14 passed" in out.stdout and "failed" not in out.stdout, out.stdout[-2000:]
